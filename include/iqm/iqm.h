@@ -1,0 +1,159 @@
+/*
+ * iqm.h -- C ABI of the MI355X SSIM and FSIM/FSIMc library (libiqm_hip.so).
+ *
+ * Drop-in boundary for the reference's per-method interface (cadik/IQM):
+ *   src/gpu/ssim/ssim.h:25-35   class SSIM { SSIMResult computeMetric(rt, image, ref); }
+ *   src/gpu/fsim/fsim.h:33-36   class FSIM { FSIMResult computeMetric(rt, image, ref); }
+ *   src/input_image.h:11-15     struct InputImage { int width, height; vector<u8> data; }
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * The C++ classes in iqm/iqm.hpp wrap these calls with the reference's
+ * exceptions and result structs.
+ *
+ * Conventions:
+ *   - every function returns IQM_OK (0) or a negative-free status code below;
+ *     iqm_last_error() gives the message of the calling thread's last failure;
+ *   - one context = one device + one HIP stream; a context is not thread safe,
+ *     separate contexts may be used from separate threads;
+ *   - host pointers are owned by the caller; the context owns device memory,
+ *     FFT plans and per-size constants;
+ *   - images are interleaved 8-bit with 1 (gray), 3 (RGB) or 4 (RGBA)
+ *     channels; gray reads as R=G=B and alpha is ignored, as in the
+ *     reference's forced-RGBA loading (src/main.cpp:37-40).
+ */
+#ifndef IQM_IQM_H
+#define IQM_IQM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    IQM_OK = 0,
+    IQM_ERR_INVALID = 1,        /* null pointer, bad channel count, bad params     */
+    IQM_ERR_SIZE_MISMATCH = 2,  /* main.cpp:74-76 "Compared images must have the same size" */
+    IQM_ERR_TOO_SMALL = 3,      /* SSIM crop needs W,H >= kernel_size (ssim.cpp:390-400) */
+    IQM_ERR_UNSUPPORTED = 4,    /* size outside the FFT planner's range             */
+    IQM_ERR_DEVICE = 5,         /* HIP runtime failure                              */
+    IQM_ERR_NOMEM = 6
+};
+
+typedef struct iqm_ctx iqm_ctx;
+
+/* Host image view (src/input_image.h:11-15, generalised to 1/3/4 channels and
+ * a row stride in bytes; row_stride 0 means width * channels). */
+typedef struct {
+    const uint8_t* data;
+    int32_t width, height, channels, row_stride;
+} iqm_image;
+
+/* SSIM parameters, defaults of src/gpu/ssim/ssim.h:31-34. */
+typedef struct {
+    int32_t kernel_size; /* odd, 3..31 */
+    float k1, k2, sigma;
+} iqm_ssim_params;
+
+/* Per-pixel FSIM outputs on the downscaled wd x hd grid (all nullable host
+ * pointers, wd*hd floats each).  The reference computes these in
+ * fsim_phase_congruency.glsl and fsim_final_multiply.glsl:38,48,57 but
+ * returns none of them. */
+typedef struct {
+    float* pc_in;
+    float* pc_ref;
+    float* pcm;
+    float* sim;
+    float* simc;
+} iqm_fsim_maps;
+
+/* Stage-level detail of one FSIM evaluation (all nullable host pointers). */
+typedef struct {
+    float* yiq_in;      /* 3 planes I, Q, Y of the downscaled input (F2)       */
+    float* yiq_ref;
+    float* grad_in;     /* Scharr magnitude (F3)                               */
+    float* grad_ref;
+    float* an_e;        /* 8 x 2 planes (An, E) for [img*4 + o] (F12)          */
+    float* keys;        /* 8 planes |EO_{img, s=0, o}|^2 (F13 median input)    */
+    double em_n[4];     /* F10 */
+    double est[8];      /* F14 */
+    double median[8];   /* F13, [img*4 + o] */
+    double noise_power[8];
+    double T[8];        /* F15 noise thresholds */
+    double sums[3];     /* F17: sum pcm, sum sim, sum simc */
+} iqm_fsim_detail;
+
+/* ---- context -------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (replaces VulkanRuntime,
+ * src/gpu/base/vulkan_runtime.cpp:22-67). */
+int iqm_create(int device, iqm_ctx** out);
+void iqm_destroy(iqm_ctx* ctx);
+const char* iqm_last_error(void);
+const char* iqm_device_name(iqm_ctx* ctx);
+void iqm_ssim_default_params(iqm_ssim_params* p);
+
+/* ---- SSIM ------------------------------------------------------------ */
+
+/* SSIM::computeMetric (src/gpu/ssim/ssim.cpp:71-243) + computeMSSIM
+ * (:386-401).  map_out (nullable): width*height floats, row-major. */
+int iqm_ssim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref,
+             const iqm_ssim_params* params, float* map_out, double* mssim_out);
+
+/* Batched SSIM on device-resident inputs: n pairs of w x h x ch images at
+ * d_img + i*pair_stride and d_ref + i*pair_stride (bytes, tightly packed
+ * rows).  d_maps (nullable, device): n*w*h floats.  mssim_out: n host doubles. */
+int iqm_ssim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                       int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                       const iqm_ssim_params* params, float* d_maps, double* mssim_out);
+
+/* ---- FSIM / FSIMc ------------------------------------------------------ */
+
+/* Downscale factor and grid (src/gpu/fsim/fsim.cpp:83,91-92,165-168). */
+void iqm_fsim_dims(int32_t w, int32_t h, int32_t* F, int32_t* wd, int32_t* hd);
+
+/* FSIM::computeMetric (src/gpu/fsim/fsim.cpp:80-163): returns FSIM and FSIMc
+ * (double; the reference's FSIMResult holds floats).  maps and detail are
+ * nullable. */
+int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* fsim_out,
+             double* fsimc_out, iqm_fsim_maps* maps, iqm_fsim_detail* detail);
+
+/* Batched FSIM on device-resident inputs (same size per call), laid out as
+ * for iqm_ssim_batch_dev.  d_simc_maps (nullable, device): n*wd*hd floats.
+ * fsim_out / fsimc_out: n host doubles each (nullable). */
+int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                       int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                       float* d_simc_maps, double* fsim_out, double* fsimc_out);
+
+/* ---- FFT (replaces VkFFT: fsim.cpp:370-437 plan, :527-550 append) ------- */
+
+/* In-place batched 2-D C2C FFT of `batch` interleaved complex float planes
+ * of w x h (w contiguous), host memory.  dir = -1 forward (unnormalised),
+ * +1 inverse; normalize != 0 divides by w*h. */
+int iqm_fft2d(iqm_ctx* ctx, float* data, int32_t w, int32_t h, int32_t batch, int32_t dir,
+              int32_t normalize);
+
+/* ---- device memory and stream helpers for callers without their own ------ */
+
+int iqm_dev_alloc(iqm_ctx* ctx, size_t bytes, void** out);
+int iqm_dev_free(iqm_ctx* ctx, void* p);
+int iqm_memcpy_h2d(iqm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int iqm_memcpy_d2h(iqm_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* The context's hipStream_t (every kernel of the context runs on it). */
+void* iqm_stream(iqm_ctx* ctx);
+int iqm_sync(iqm_ctx* ctx);
+
+/* ---- stage timing ------------------------------------------------------- */
+
+/* When on, every kernel launch is bracketed by hipEvents on the context's
+ * stream and its duration is accumulated per kernel name. */
+int iqm_set_stage_timing(iqm_ctx* ctx, int32_t on);
+/* Copies up to `max` (name, total ms, launches) triples; returns the count. */
+int32_t iqm_stage_times(iqm_ctx* ctx, int32_t max, const char** names, double* total_ms,
+                        int64_t* launches);
+void iqm_reset_stage_times(iqm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

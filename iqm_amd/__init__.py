@@ -1,0 +1,309 @@
+"""MI355X-native SSIM and FSIM/FSIMc for the cadik/IQM method interface.
+
+This module is the ctypes binding of libiqm_hip.so (include/iqm/iqm.h).  It
+mirrors the reference's per-method interface -- ``SSIM().compute_metric(ctx,
+image, ref)`` and ``FSIM().compute_metric(ctx, image, ref)``, after
+src/gpu/ssim/ssim.h:25-35 and src/gpu/fsim/fsim.h:33-36 -- and raises the
+reference's error messages.  There is no CPU fallback: when the HIP library
+is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libiqm_hip.so")
+
+IQM_OK = 0
+IQM_ERR_INVALID = 1
+IQM_ERR_SIZE_MISMATCH = 2
+IQM_ERR_TOO_SMALL = 3
+IQM_ERR_UNSUPPORTED = 4
+IQM_ERR_DEVICE = 5
+IQM_ERR_NOMEM = 6
+
+# every symbol include/iqm/iqm.h declares
+EXPORTS = (
+    "iqm_create", "iqm_destroy", "iqm_last_error", "iqm_device_name", "iqm_ssim_default_params",
+    "iqm_ssim", "iqm_ssim_batch_dev", "iqm_fsim_dims", "iqm_fsim", "iqm_fsim_batch_dev",
+    "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
+    "iqm_stream", "iqm_sync", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
+)
+
+
+class IqmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("channels", ctypes.c_int32), ("row_stride", ctypes.c_int32)]
+
+
+class SsimParams(ctypes.Structure):
+    _fields_ = [("kernel_size", ctypes.c_int32), ("k1", ctypes.c_float), ("k2", ctypes.c_float),
+                ("sigma", ctypes.c_float)]
+
+
+class FsimMaps(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("pc_in", "pc_ref", "pcm", "sim", "simc")]
+
+
+class FsimDetail(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("yiq_in", "yiq_ref", "grad_in", "grad_ref", "an_e", "keys")] + [
+        ("em_n", ctypes.c_double * 4), ("est", ctypes.c_double * 8), ("median", ctypes.c_double * 8),
+        ("noise_power", ctypes.c_double * 8), ("T", ctypes.c_double * 8), ("sums", ctypes.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libiqm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C iqm_amd/csrc` "
+                          "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+    L.iqm_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.iqm_destroy.argtypes = [vp]
+    L.iqm_destroy.restype = None
+    L.iqm_last_error.restype = ctypes.c_char_p
+    L.iqm_device_name.argtypes = [vp]
+    L.iqm_device_name.restype = ctypes.c_char_p
+    L.iqm_ssim_default_params.argtypes = [ctypes.POINTER(SsimParams)]
+    L.iqm_ssim_default_params.restype = None
+    L.iqm_ssim.argtypes = [vp, ctypes.POINTER(Image), ctypes.POINTER(Image), ctypes.POINTER(SsimParams), vp, dp]
+    L.iqm_ssim_batch_dev.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, ctypes.POINTER(SsimParams), vp, dp]
+    L.iqm_fsim_dims.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.iqm_fsim_dims.restype = None
+    L.iqm_fsim.argtypes = [vp, ctypes.POINTER(Image), ctypes.POINTER(Image), dp, dp,
+                           ctypes.POINTER(FsimMaps), ctypes.POINTER(FsimDetail)]
+    L.iqm_fsim_batch_dev.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, vp, dp, dp]
+    L.iqm_fft2d.argtypes = [vp, vp, i32, i32, i32, i32, i32]
+    L.iqm_dev_alloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.iqm_dev_free.argtypes = [vp, vp]
+    L.iqm_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    L.iqm_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    L.iqm_stream.argtypes = [vp]
+    L.iqm_stream.restype = vp
+    L.iqm_sync.argtypes = [vp]
+    L.iqm_set_stage_timing.argtypes = [vp, i32]
+    L.iqm_stage_times.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(i64)]
+    L.iqm_stage_times.restype = i32
+    L.iqm_reset_stage_times.argtypes = [vp]
+    L.iqm_reset_stage_times.restype = None
+    for name in EXPORTS:
+        getattr(L, name)
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != IQM_OK:
+        raise IqmError(rc, lib().iqm_last_error().decode())
+
+
+def _image(a: np.ndarray) -> tuple[Image, np.ndarray]:
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    if a.ndim == 2:
+        h, w = a.shape
+        ch = 1
+    elif a.ndim == 3:
+        h, w, ch = a.shape
+    else:
+        raise ValueError("image must be (h, w) or (h, w, ch)")
+    return Image(a.ctypes.data, w, h, ch, w * ch), a
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fsim_dims(w: int, h: int) -> tuple[int, int, int]:
+    F, wd, hd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib().iqm_fsim_dims(w, h, ctypes.byref(F), ctypes.byref(wd), ctypes.byref(hd))
+    return F.value, wd.value, hd.value
+
+
+class Context:
+    """One HIP device and stream (stands where VulkanRuntime stands in the reference)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().iqm_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def device_name(self) -> str:
+        return lib().iqm_device_name(self.handle).decode()
+
+    @property
+    def stream(self) -> int:
+        return lib().iqm_stream(self.handle) or 0
+
+    def sync(self) -> None:
+        _check(lib().iqm_sync(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().iqm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # device memory for device-resident batches
+    def alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        _check(lib().iqm_dev_alloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def free(self, ptr: int) -> None:
+        _check(lib().iqm_dev_free(self.handle, ptr))
+
+    def upload(self, dst: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        _check(lib().iqm_memcpy_h2d(self.handle, dst, a.ctypes.data, a.nbytes))
+
+    def download(self, src: int, out: np.ndarray) -> np.ndarray:
+        _check(lib().iqm_memcpy_d2h(self.handle, out.ctypes.data, src, out.nbytes))
+        return out
+
+    def set_stage_timing(self, on: bool) -> None:
+        _check(lib().iqm_set_stage_timing(self.handle, 1 if on else 0))
+
+    def reset_stage_times(self) -> None:
+        lib().iqm_reset_stage_times(self.handle)
+
+    def stage_times(self) -> dict:
+        n = lib().iqm_stage_times(self.handle, 0, None, None, None)
+        if n <= 0:
+            return {}
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        lib().iqm_stage_times(self.handle, n, names, ms, cnt)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(n)}
+
+    def fft2d(self, x: np.ndarray, inverse: bool = False, normalize: bool = False) -> np.ndarray:
+        """Batched 2-D FFT of (..., h, w) complex data in float32."""
+        a = np.ascontiguousarray(x, dtype=np.complex64).copy()
+        h, w = a.shape[-2:]
+        batch = int(np.prod(a.shape[:-2])) if a.ndim > 2 else 1
+        _check(lib().iqm_fft2d(self.handle, a.ctypes.data, w, h, batch, 1 if inverse else -1,
+                               int(normalize)))
+        return a
+
+
+@dataclass
+class SSIMResult:
+    """src/gpu/ssim/ssim.h:17-23"""
+    image_data: np.ndarray | None
+    width: int
+    height: int
+    mssim: float
+
+
+@dataclass
+class FSIMResult:
+    """src/gpu/fsim/fsim.h:27-31, extended with the per-pixel maps."""
+    fsim: float
+    fsimc: float
+    maps: dict = field(default_factory=dict)
+    detail: dict = field(default_factory=dict)
+
+
+class SSIM:
+    """src/gpu/ssim/ssim.h:25-35"""
+
+    def __init__(self, ctx: Context | None = None):
+        self.kernel_size = 11
+        self.k_1 = 0.01
+        self.k_2 = 0.03
+        self.sigma = 1.5
+
+    def params(self) -> SsimParams:
+        return SsimParams(self.kernel_size, self.k_1, self.k_2, self.sigma)
+
+    def compute_metric(self, ctx: Context, image: np.ndarray, ref: np.ndarray,
+                       want_map: bool = True) -> SSIMResult:
+        a, ka = _image(image)
+        b, kb = _image(ref)
+        m = np.empty((a.height, a.width), np.float32) if want_map else None
+        ms = np.zeros(1, np.float64)
+        p = self.params()
+        _check(lib().iqm_ssim(ctx.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(p),
+                              m.ctypes.data if m is not None else None, _dptr(ms)))
+        return SSIMResult(m, a.width, a.height, float(ms[0]))
+
+    def compute_batch_dev(self, ctx: Context, d_img: int, d_ref: int, w: int, h: int, ch: int,
+                          pair_stride: int, n: int, d_maps: int | None = None) -> np.ndarray:
+        out = np.zeros(n, np.float64)
+        p = self.params()
+        _check(lib().iqm_ssim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
+                                        ctypes.byref(p), d_maps, _dptr(out)))
+        return out
+
+
+class FSIM:
+    """src/gpu/fsim/fsim.h:33-36"""
+
+    MAPS = ("pc_in", "pc_ref", "pcm", "sim", "simc")
+
+    def __init__(self, ctx: Context | None = None):
+        pass
+
+    def compute_metric(self, ctx: Context, image: np.ndarray, ref: np.ndarray, maps=(),
+                       detail: bool = False) -> FSIMResult:
+        a, ka = _image(image)
+        b, kb = _image(ref)
+        _, wd, hd = fsim_dims(a.width, a.height)
+        N = wd * hd
+        fs = np.zeros(1, np.float64)
+        fc = np.zeros(1, np.float64)
+        keep = {}
+        mp = None
+        if maps:
+            mp = FsimMaps()
+            for name in maps:
+                arr = np.empty((hd, wd), np.float32)
+                keep[name] = arr
+                setattr(mp, name, arr.ctypes.data)
+        det = None
+        dk = {}
+        if detail:
+            det = FsimDetail()
+            for name, planes in (("yiq_in", 3), ("yiq_ref", 3), ("grad_in", 1), ("grad_ref", 1),
+                                 ("an_e", 16), ("keys", 8)):
+                arr = np.empty((planes, hd, wd) if planes > 1 else (hd, wd), np.float32)
+                dk[name] = arr
+                setattr(det, name, arr.ctypes.data)
+        _check(lib().iqm_fsim(ctx.handle, ctypes.byref(a), ctypes.byref(b), _dptr(fs), _dptr(fc),
+                              ctypes.byref(mp) if mp is not None else None,
+                              ctypes.byref(det) if det is not None else None))
+        if det is not None:
+            for name in ("em_n", "est", "median", "noise_power", "T", "sums"):
+                dk[name] = np.array(getattr(det, name)[:])
+        return FSIMResult(float(fs[0]), float(fc[0]), keep, dk)
+
+    def compute_batch_dev(self, ctx: Context, d_img: int, d_ref: int, w: int, h: int, ch: int,
+                          pair_stride: int, n: int, d_simc_maps: int | None = None):
+        fs = np.zeros(n, np.float64)
+        fc = np.zeros(n, np.float64)
+        _check(lib().iqm_fsim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
+                                        d_simc_maps, _dptr(fs), _dptr(fc)))
+        return fs, fc

@@ -1,0 +1,433 @@
+// Context implementation and the extern "C" entry points of include/iqm/iqm.h.
+#include "ctx.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace iqm {
+
+Ctx::Ctx(int device) : device_(device) {
+    int count = 0;
+    IQM_HIP(hipGetDeviceCount(&count));
+    IQM_CHECK(device >= 0 && device < count, IQM_ERR_INVALID,
+              "device ordinal " + std::to_string(device) + " out of range");
+    IQM_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    IQM_HIP(hipGetDeviceProperties(&prop, device));
+    name_ = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    IQM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Ctx::~Ctx() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& kv : ws_) (void)hipFree(kv.second.first);
+    for (auto& kv : line_plans_) (void)hipFree(kv.second.dev_tw);
+    for (auto& kv : fsim_plans_) {
+        (void)hipFree(kv.second.lg);
+        (void)hipFree(kv.second.ang);
+    }
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (auto e : free_events_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* Ctx::workspace(int slot, size_t bytes) {
+    auto& e = ws_[slot];
+    if (e.second < bytes) {
+        if (e.first) {
+            IQM_HIP(hipStreamSynchronize(stream_));
+            IQM_HIP(hipFree(e.first));
+            e.first = nullptr;
+            e.second = 0;
+        }
+        size_t want = std::max(bytes, (size_t)256);
+        if (hipMalloc(&e.first, want) != hipSuccess) {
+            (void)hipGetLastError();
+            e.first = nullptr;
+            throw Error(IQM_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+        }
+        e.second = want;
+    }
+    return e.first;
+}
+
+void Ctx::sync() {
+    IQM_HIP(hipStreamSynchronize(stream_));
+    if (timing_ || !pending_.empty()) collect_timing();
+}
+
+hipEvent_t Ctx::take_event() {
+    if (!free_events_.empty()) {
+        hipEvent_t e = free_events_.back();
+        free_events_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    IQM_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Ctx::collect_timing() {
+    for (auto& p : pending_) {
+        IQM_HIP(hipEventSynchronize(p.second.second));
+        float ms = 0.f;
+        IQM_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        auto& t = totals_[p.first];
+        t.first += ms;
+        t.second += 1;
+        free_events_.push_back(p.second.first);
+        free_events_.push_back(p.second.second);
+    }
+    pending_.clear();
+}
+
+void Ctx::set_timing(bool on) {
+    if (!on) sync();
+    timing_ = on;
+}
+
+int Ctx::stage_times(int max, const char** names, double* ms, int64_t* launches) {
+    sync();
+    names_cache_.clear();
+    for (auto& kv : totals_) names_cache_.push_back(kv.first);
+    int i = 0;
+    for (auto& kv : totals_) {
+        if (i >= max) break;
+        if (names) names[i] = names_cache_[i].c_str();
+        if (ms) ms[i] = kv.second.first;
+        if (launches) launches[i] = kv.second.second;
+        ++i;
+    }
+    return (int)totals_.size();
+}
+
+void Ctx::reset_times() {
+    sync();
+    totals_.clear();
+}
+
+}  // namespace iqm
+
+/* ------------------------------------------------------------------------ */
+
+struct iqm_ctx {
+    std::unique_ptr<iqm::Ctx> c;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return IQM_OK;
+    } catch (const iqm::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(IQM_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(IQM_ERR_DEVICE, e.what());
+    }
+}
+
+/* Validates an image view and returns its tight row size. */
+void check_image(const iqm_image* im, const char* what) {
+    IQM_CHECK(im && im->data, IQM_ERR_INVALID, std::string("null ") + what + " image");
+    IQM_CHECK(im->width > 0 && im->height > 0, IQM_ERR_INVALID, std::string("empty ") + what + " image");
+    IQM_CHECK(im->channels == 1 || im->channels == 3 || im->channels == 4, IQM_ERR_INVALID,
+              std::string(what) + " image must have 1, 3 or 4 channels");
+    const int tight = im->width * im->channels;
+    IQM_CHECK(im->row_stride == 0 || im->row_stride >= tight, IQM_ERR_INVALID,
+              std::string(what) + " row_stride smaller than width * channels");
+}
+
+void check_pair(const iqm_image* a, const iqm_image* b) {
+    check_image(a, "input");
+    check_image(b, "reference");
+    IQM_CHECK(a->width == b->width && a->height == b->height, IQM_ERR_SIZE_MISMATCH,
+              "Compared images must have the same size");
+    IQM_CHECK(a->channels == b->channels, IQM_ERR_INVALID,
+              "input and reference must have the same channel count");
+}
+
+/* Copies an image view into a tight device buffer. */
+void upload(iqm::Ctx& c, int slot, const iqm_image* im, uint8_t** dev) {
+    const size_t row = (size_t)im->width * im->channels;
+    const size_t stride = im->row_stride ? (size_t)im->row_stride : row;
+    uint8_t* d = (uint8_t*)c.workspace(slot, row * im->height);
+    if (stride == row) {
+        IQM_HIP(hipMemcpyAsync(d, im->data, row * im->height, hipMemcpyHostToDevice, c.stream()));
+    } else {
+        IQM_HIP(hipMemcpy2DAsync(d, row, im->data, stride, row, im->height, hipMemcpyHostToDevice,
+                                 c.stream()));
+    }
+    *dev = d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* iqm_last_error(void) { return g_last_error.c_str(); }
+
+int iqm_create(int device, iqm_ctx** out) {
+    if (!out) return fail(IQM_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    return guarded([&] {
+        auto* h = new iqm_ctx;
+        try {
+            h->c.reset(new iqm::Ctx(device));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void iqm_destroy(iqm_ctx* ctx) { delete ctx; }
+
+const char* iqm_device_name(iqm_ctx* ctx) { return ctx ? ctx->c->name().c_str() : ""; }
+
+void iqm_ssim_default_params(iqm_ssim_params* p) {
+    if (!p) return;
+    p->kernel_size = 11; /* ssim.h:31-34 */
+    p->k1 = 0.01f;
+    p->k2 = 0.03f;
+    p->sigma = 1.5f;
+}
+
+int iqm_ssim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref,
+             const iqm_ssim_params* params, float* map_out, double* mssim_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        check_pair(img, ref);
+        iqm_ssim_params p;
+        iqm_ssim_default_params(&p);
+        if (params) p = *params;
+        iqm::Ctx& c = *ctx->c;
+        uint8_t *da, *db;
+        upload(c, iqm::WS_IN_IMG, img, &da);
+        upload(c, iqm::WS_IN_REF, ref, &db);
+        const long N = (long)img->width * img->height;
+        float* dmap = map_out ? (float*)c.workspace(iqm::WS_MAPS, sizeof(float) * N) : nullptr;
+        double ms = 0.0;
+        c.ssim_batch(da, db, img->width, img->height, img->channels, 0, 1, p.kernel_size, p.k1,
+                     p.k2, p.sigma, dmap, &ms);
+        if (map_out) {
+            IQM_HIP(hipMemcpyAsync(map_out, dmap, sizeof(float) * N, hipMemcpyDeviceToHost, c.stream()));
+        }
+        c.sync();
+        if (mssim_out) *mssim_out = ms;
+    });
+}
+
+int iqm_ssim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                       int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                       const iqm_ssim_params* params, float* d_maps, double* mssim_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_img && d_ref && w > 0 && h > 0 && n > 0, IQM_ERR_INVALID, "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        iqm_ssim_params p;
+        iqm_ssim_default_params(&p);
+        if (params) p = *params;
+        std::vector<double> ms(n);
+        ctx->c->ssim_batch(d_img, d_ref, w, h, ch, pair_stride, n, p.kernel_size, p.k1, p.k2,
+                           p.sigma, d_maps, ms.data());
+        if (mssim_out) std::memcpy(mssim_out, ms.data(), sizeof(double) * n);
+    });
+}
+
+void iqm_fsim_dims(int32_t w, int32_t h, int32_t* F, int32_t* wd, int32_t* hd) {
+    int f = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+    if (F) *F = f;
+    if (wd) *wd = (int)std::round((float)w / (float)f);
+    if (hd) *hd = (int)std::round((float)h / (float)f);
+}
+
+int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* fsim_out,
+             double* fsimc_out, iqm_fsim_maps* maps, iqm_fsim_detail* detail) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        check_pair(img, ref);
+        iqm::Ctx& c = *ctx->c;
+        uint8_t *da, *db;
+        upload(c, iqm::WS_IN_IMG, img, &da);
+        upload(c, iqm::WS_IN_REF, ref, &db);
+        int F, wd, hd;
+        iqm_fsim_dims(img->width, img->height, &F, &wd, &hd);
+        const long N = (long)wd * hd;
+        iqm::FsimBatchOut out;
+        float* dm = nullptr;
+        if (maps) {
+            dm = (float*)c.workspace(iqm::WS_MAPS, sizeof(float) * 5 * N);
+            out.d_simc = dm;
+            out.d_pcm = dm + N;
+            out.d_sim = dm + 2 * N;
+            out.d_pc = dm + 3 * N;
+        }
+        double* dres = c.fsim_batch(da, db, img->width, img->height, img->channels, 0, 1, out);
+        double res[8];
+        IQM_HIP(hipMemcpyAsync(res, dres, sizeof(res), hipMemcpyDeviceToHost, c.stream()));
+        auto d2h = [&](float* dst, const float* src, long cnt) {
+            if (dst) IQM_HIP(hipMemcpyAsync(dst, src, sizeof(float) * cnt, hipMemcpyDeviceToHost, c.stream()));
+        };
+        if (maps) {
+            d2h(maps->simc, dm, N);
+            d2h(maps->pcm, dm + N, N);
+            d2h(maps->sim, dm + 2 * N, N);
+            d2h(maps->pc_in, dm + 3 * N, N);
+            d2h(maps->pc_ref, dm + 4 * N, N);
+        }
+        double stats[32];
+        if (detail) {
+            iqm::Ctx::FsimLayout L = c.fsim_layout(N, 1, c.fsim_ws_base());
+            if (detail->yiq_in) {
+                d2h(detail->yiq_in, L.yiq + 2 * N, N);      /* I */
+                d2h(detail->yiq_in + N, L.yiq + 3 * N, N);  /* Q */
+                d2h(detail->yiq_in + 2 * N, L.yiq, N);      /* Y */
+            }
+            if (detail->yiq_ref) {
+                d2h(detail->yiq_ref, L.yiq + 4 * N, N);
+                d2h(detail->yiq_ref + N, L.yiq + 5 * N, N);
+                d2h(detail->yiq_ref + 2 * N, L.yiq + N, N);
+            }
+            d2h(detail->grad_in, L.grad, N);
+            d2h(detail->grad_ref, L.grad + N, N);
+            d2h(detail->an_e, L.an_e, 16 * N);
+            d2h(detail->keys, L.keys, 8 * N);
+            IQM_HIP(hipMemcpyAsync(stats, L.stats, sizeof(stats), hipMemcpyDeviceToHost, c.stream()));
+        }
+        c.sync();
+        if (fsim_out) *fsim_out = res[3];
+        if (fsimc_out) *fsimc_out = res[4];
+        if (detail) {
+            const iqm::FsimSizePlan& P = c.fsim_plan(wd, hd);
+            for (int i = 0; i < 4; ++i) detail->em_n[i] = P.em_n[i];
+            for (int i = 0; i < 8; ++i) {
+                detail->est[i] = P.est[i];
+                detail->median[i] = stats[i * 4 + 0];
+                detail->noise_power[i] = stats[i * 4 + 1];
+                detail->T[i] = stats[i * 4 + 2];
+            }
+            for (int i = 0; i < 3; ++i) detail->sums[i] = res[i];
+        }
+    });
+}
+
+int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                       int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                       float* d_simc_maps, double* fsim_out, double* fsimc_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_img && d_ref && w > 0 && h > 0 && n > 0, IQM_ERR_INVALID, "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        iqm::Ctx& c = *ctx->c;
+        iqm::FsimBatchOut out;
+        out.d_simc = d_simc_maps;
+        double* dres = c.fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+        if (fsim_out || fsimc_out) {
+            std::vector<double> res((size_t)8 * n);
+            IQM_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
+            c.sync();
+            for (int i = 0; i < n; ++i) {
+                if (fsim_out) fsim_out[i] = res[(size_t)8 * i + 3];
+                if (fsimc_out) fsimc_out[i] = res[(size_t)8 * i + 4];
+            }
+        }
+    });
+}
+
+int iqm_fft2d(iqm_ctx* ctx, float* data, int32_t w, int32_t h, int32_t batch, int32_t dir,
+              int32_t normalize) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(data && w > 0 && h > 0 && batch > 0 && (dir == 1 || dir == -1), IQM_ERR_INVALID,
+                  "invalid FFT arguments");
+        iqm::Ctx& c = *ctx->c;
+        const size_t bytes = sizeof(float2) * (size_t)w * h * batch;
+        float2* d = (float2*)c.workspace(iqm::WS_FFT, bytes);
+        IQM_HIP(hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, c.stream()));
+        c.fft2d_dev(d, w, h, batch, dir, normalize != 0);
+        IQM_HIP(hipMemcpyAsync(data, d, bytes, hipMemcpyDeviceToHost, c.stream()));
+        c.sync();
+    });
+}
+
+int iqm_dev_alloc(iqm_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(IQM_ERR_INVALID, "null argument");
+    return guarded([&] {
+        IQM_HIP(hipSetDevice(ctx->c->device()));
+        if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            throw iqm::Error(IQM_ERR_NOMEM, "device allocation failed");
+        }
+    });
+}
+
+int iqm_dev_free(iqm_ctx* ctx, void* p) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        ctx->c->sync();
+        IQM_HIP(hipFree(p));
+    });
+}
+
+int iqm_memcpy_h2d(iqm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c->stream()));
+        ctx->c->sync();
+    });
+}
+
+int iqm_memcpy_d2h(iqm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c->stream()));
+        ctx->c->sync();
+    });
+}
+
+void* iqm_stream(iqm_ctx* ctx) { return ctx ? (void*)ctx->c->stream() : nullptr; }
+
+int iqm_sync(iqm_ctx* ctx) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] { ctx->c->sync(); });
+}
+
+int iqm_set_stage_timing(iqm_ctx* ctx, int32_t on) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] { ctx->c->set_timing(on != 0); });
+}
+
+int32_t iqm_stage_times(iqm_ctx* ctx, int32_t max, const char** names, double* total_ms,
+                        int64_t* launches) {
+    if (!ctx) return -1;
+    int r = -1;
+    if (guarded([&] { r = ctx->c->stage_times(max, names, total_ms, launches); }) != IQM_OK) return -1;
+    return r;
+}
+
+void iqm_reset_stage_times(iqm_ctx* ctx) {
+    if (ctx) guarded([&] { ctx->c->reset_times(); });
+}
+
+}  // extern "C"

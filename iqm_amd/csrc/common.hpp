@@ -1,0 +1,57 @@
+// Shared definitions of the HIP library: error handling, complex helpers,
+// launch constants.  Every kernel in this library is written for gfx950
+// (wave64, 160 KB LDS per CU); no other target is built.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "iqm/iqm.h"
+
+namespace iqm {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define IQM_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::iqm::Error(IQM_ERR_DEVICE,                                          \
+                               std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define IQM_CHECK(cond, code, msg)                                                      \
+    do {                                                                                \
+        if (!(cond)) throw ::iqm::Error((code), (msg));                                 \
+    } while (0)
+
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) {
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) {
+    return make_float2(a.x * s, a.y * s);
+}
+/* multiply by s*i, s = +1 or -1 */
+template <int S>
+__host__ __device__ __forceinline__ float2 cmul_si(float2 a) {
+    return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+constexpr int kBlock = 256;          // default workgroup size (4 wavefronts)
+constexpr int kMaxLdsBytes = 65536;  // per-workgroup LDS budget used by every kernel
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace iqm

@@ -1,0 +1,242 @@
+// Line FFTs in LDS: the project's replacement of VkFFT
+// (reference call sites src/gpu/fsim/fsim.cpp:370-437 and :527-550).
+//
+// A workgroup transforms `nlines` lines of length n held in LDS with a
+// Stockham autosort schedule: stage j of radix r reads element i + t*(n/r)
+// of the source buffer, applies the twiddle exp(dir*2*pi*i*t*k/(r*p)) with
+// k = i mod p (p = product of the earlier radices), runs a radix-r DFT and
+// writes element (i/p)*r*p + k + t*p of the destination buffer.  Buffers
+// ping-pong between stages and the result ends in natural order.
+//
+// Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies; a length
+// whose prime factors are all <= 13 is planned directly.  Twiddles are
+// computed in double on the host and stored as float.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace iqm {
+
+constexpr int kMaxStages = 16;
+
+/* Kernel-argument view of a line plan (passed by value). */
+struct LinePlan {
+    int n;
+    int nst;
+    int rad[kMaxStages];
+    const float2* tw; /* exp(-2 pi i j / n), j in [0, n) */
+};
+
+/* ---- small-radix DFTs (y_t = sum_q x_q exp(S*2*pi*i*q*t/R)) -------------- */
+
+template <int R>
+struct OddTab;
+template <>
+struct OddTab<3> {
+    static constexpr float C[3] = {1.0f, -0.5f, -0.5f};
+    static constexpr float S[3] = {0.0f, 0.86602540378443864676f, -0.86602540378443864676f};
+};
+template <>
+struct OddTab<5> {
+    static constexpr float C[5] = {1.0f, 0.30901699437494742410f, -0.80901699437494742410f,
+                                   -0.80901699437494742410f, 0.30901699437494742410f};
+    static constexpr float S[5] = {0.0f, 0.95105651629515357212f, 0.58778525229247312917f,
+                                   -0.58778525229247312917f, -0.95105651629515357212f};
+};
+template <>
+struct OddTab<7> {
+    static constexpr float C[7] = {1.0f, 0.62348980185873353053f, -0.22252093395631440429f,
+                                   -0.90096886790241912624f, -0.90096886790241912624f,
+                                   -0.22252093395631440429f, 0.62348980185873353053f};
+    static constexpr float S[7] = {0.0f, 0.78183148246802980871f, 0.97492791218182360702f,
+                                   0.43388373911755812048f, -0.43388373911755812048f,
+                                   -0.97492791218182360702f, -0.78183148246802980871f};
+};
+template <>
+struct OddTab<11> {
+    static constexpr float C[11] = {1.0f, 0.84125353283118116886f, 0.41541501300188642553f,
+                                    -0.14231483827328514044f, -0.65486073394528506406f,
+                                    -0.95949297361449738989f, -0.95949297361449738989f,
+                                    -0.65486073394528506406f, -0.14231483827328514044f,
+                                    0.41541501300188642553f, 0.84125353283118116886f};
+    static constexpr float S[11] = {0.0f, 0.54064081745559758211f, 0.90963199535451837141f,
+                                    0.98982144188093273238f, 0.75574957435425828377f,
+                                    0.28173255684142969771f, -0.28173255684142969771f,
+                                    -0.75574957435425828377f, -0.98982144188093273238f,
+                                    -0.90963199535451837141f, -0.54064081745559758211f};
+};
+template <>
+struct OddTab<13> {
+    static constexpr float C[13] = {1.0f, 0.88545602565320989590f, 0.56806474673115580251f,
+                                    0.12053668025532305335f, -0.35460488704253562597f,
+                                    -0.74851074817110109863f, -0.97094181742605202716f,
+                                    -0.97094181742605202716f, -0.74851074817110109863f,
+                                    -0.35460488704253562597f, 0.12053668025532305335f,
+                                    0.56806474673115580251f, 0.88545602565320989590f};
+    static constexpr float S[13] = {0.0f, 0.46472317204376854566f, 0.82298386589365639458f,
+                                    0.99270887409805399280f, 0.93501624268541482344f,
+                                    0.66312265824079520238f, 0.23931566428755776715f,
+                                    -0.23931566428755776715f, -0.66312265824079520238f,
+                                    -0.93501624268541482344f, -0.99270887409805399280f,
+                                    -0.82298386589365639458f, -0.46472317204376854566f};
+};
+
+/* Odd prime radix: pairs q and R-q share the cosine terms. */
+template <int R, int S>
+__host__ __device__ __forceinline__ void dft_odd(float2* x) {
+    constexpr int H = (R - 1) / 2;
+    float2 a[H + 1], b[H + 1];
+    float2 y0 = x[0];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+        a[q] = cadd(x[q], x[R - q]);
+        b[q] = csub(x[q], x[R - q]);
+        y0 = cadd(y0, a[q]);
+    }
+    float2 out[R];
+    out[0] = y0;
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        float2 P = x[0];
+        float2 Q = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+            const float c = OddTab<R>::C[(q * t) % R];
+            const float s = OddTab<R>::S[(q * t) % R];
+            P.x += a[q].x * c;
+            P.y += a[q].y * c;
+            Q.x += b[q].x * s;
+            Q.y += b[q].y * s;
+        }
+        const float2 iq = cmul_si<S>(Q);
+        out[t] = cadd(P, iq);
+        out[R - t] = csub(P, iq);
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) x[t] = out[t];
+}
+
+template <int S>
+__host__ __device__ __forceinline__ void dft2(float2* x) {
+    float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+template <int S>
+__host__ __device__ __forceinline__ void dft4(float2* x) {
+    float2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+    float2 a2 = cadd(x[1], x[3]), a3 = cmul_si<S>(csub(x[1], x[3]));
+    x[0] = cadd(a0, a2);
+    x[2] = csub(a0, a2);
+    x[1] = cadd(a1, a3);
+    x[3] = csub(a1, a3);
+}
+
+template <int S>
+__host__ __device__ __forceinline__ void dft8(float2* x) {
+    /* two radix-4 DFTs on the even and odd samples, then a radix-2 combine */
+    float2 e[4] = {x[0], x[2], x[4], x[6]};
+    float2 o[4] = {x[1], x[3], x[5], x[7]};
+    dft4<S>(e);
+    dft4<S>(o);
+    const float h = 0.70710678118654752440f;
+    /* w8^k = exp(S*2*pi*i*k/8) */
+    float2 w1 = make_float2(h, S * h), w3 = make_float2(-h, S * h);
+    o[1] = cmul(o[1], w1);
+    o[2] = cmul_si<S>(o[2]);
+    o[3] = cmul(o[3], w3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        x[k] = cadd(e[k], o[k]);
+        x[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <int R, int S>
+__host__ __device__ __forceinline__ void dft_r(float2* x) {
+    if constexpr (R == 2) dft2<S>(x);
+    else if constexpr (R == 4) dft4<S>(x);
+    else if constexpr (R == 8) dft8<S>(x);
+    else dft_odd<R, S>(x);
+}
+
+/* One Stockham stage over nlines lines (line l at src + l*ld). */
+template <int R, int S>
+__host__ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int ld,
+                                                        int nlines, int n, int p,
+                                                        const float2* tw, int tid, int nthr) {
+    const int T = n / R;
+    const int total = nlines * T;
+    const int tws = n / (R * p);
+    for (int idx = tid; idx < total; idx += nthr) {
+        const int line = idx / T;
+        const int i = idx - line * T;
+        const int k = i % p;
+        const float2* s = src + line * ld;
+        float2 x[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) x[t] = s[i + t * T];
+        if (k != 0) {
+#pragma unroll
+            for (int t = 1; t < R; ++t) {
+                float2 w = tw[t * k * tws];
+                if (S > 0) w.y = -w.y;
+                x[t] = cmul(x[t], w);
+            }
+        }
+        dft_r<R, S>(x);
+        float2* d = dst + line * ld + (i - k) * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) d[t * p] = x[t];
+    }
+}
+
+template <int S>
+__host__ __device__ __forceinline__ void stockham_dispatch(int r, const float2* src, float2* dst,
+                                                           int ld, int nlines, int n, int p,
+                                                           const float2* tw, int tid, int nthr) {
+    switch (r) {
+        case 2: stockham_stage<2, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 3: stockham_stage<3, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 4: stockham_stage<4, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 5: stockham_stage<5, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 7: stockham_stage<7, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 8: stockham_stage<8, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 11: stockham_stage<11, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        case 13: stockham_stage<13, S>(src, dst, ld, nlines, n, p, tw, tid, nthr); break;
+        default: break;
+    }
+}
+
+/* Transforms nlines lines starting in buf0; returns the buffer holding the
+ * result (buf0 or buf1).  Ends with a workgroup barrier. */
+template <int S>
+__device__ __forceinline__ float2* lds_fft(float2* buf0, float2* buf1, int ld, int nlines,
+                                           const LinePlan& P) {
+    float2* src = buf0;
+    float2* dst = buf1;
+    int p = 1;
+    for (int st = 0; st < P.nst; ++st) {
+        const int r = P.rad[st];
+        stockham_dispatch<S>(r, src, dst, ld, nlines, P.n, p, P.tw, threadIdx.x, blockDim.x);
+        __syncthreads();
+        float2* t = src;
+        src = dst;
+        dst = t;
+        p *= r;
+    }
+    return src;
+}
+
+/* Host-side planner: radices for n (largest power-of-two radices first,
+ * then the odd primes); false when n has a prime factor above 13. */
+bool plan_radices(int n, std::vector<int>& rad);
+
+/* LDS row pitch (in float2) used for a line of length n: odd, so that the
+ * transposed loads of the column kernels spread over the LDS banks. */
+__host__ __device__ __forceinline__ int lds_pitch(int n) { return (n % 2 == 0) ? n + 1 : n; }
+
+}  // namespace iqm

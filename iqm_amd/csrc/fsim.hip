@@ -1,0 +1,768 @@
+// FSIM / FSIMc on MI355X.
+//
+// Pipeline per batch of same-size pairs (reference: FSIM::computeMetric,
+// src/gpu/fsim/fsim.cpp:80-163), 7 launches:
+//   fsim_prep          F2: YIQ of 255-scaled RGB, F x F box average
+//   fsim_grad_rowfft   F3: Scharr magnitude; forward row FFT of Y_in + i*Y_ref,
+//                      split into the two row spectra (F4-F5, first half)
+//   fsim_colpass       forward column FFT -> spectrum, times H_{s,o} (F9),
+//                      inverse column FFT (F11, first half); 16 planes/image
+//   fsim_rowpass       inverse row FFT (F11, second half) + response epilogue
+//                      (F12) + |EO_{s=0}|^2 keys and their top-bit histogram
+//   fsim_select        exact upper median by radix select (F13) -> noise
+//                      power and threshold T (F15) in double
+//   fsim_pcfinal       phase congruency (F15), final multiply (F16), FP64
+//                      block partial sums (F17)
+//   fsim_finalize      fixed-order sum of the partials, FSIM and FSIMc
+// Per-size constants (filter planes, EM_n of F10, est of F14) are computed
+// once per (wd, hd) by Ctx::fsim_plan and cached.
+#include "ctx.hpp"
+#include "fft.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace iqm {
+
+/* ---- filter bank, FP32 like the reference shaders --------------------- */
+
+/* fsim_lowpassfilter.glsl:32-35 (same in log_gabor and angular): ifftshift
+ * order, u = ((x + n/2) mod n - n/2) / n */
+__device__ __forceinline__ float freq_coord(int x, int n) {
+    unsigned shifted = ((unsigned)x + (unsigned)n / 2u) % (unsigned)n;
+    return (float)((int)shifted - n / 2) / (float)n;
+}
+
+__global__ __launch_bounds__(kBlock) void fsim_tables_kernel(float* lg, float* ang, int wd,
+                                                             int hd) {
+    const long N = (long)wd * hd;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int y = (int)(idx / wd), x = (int)(idx - (long)y * wd);
+    const float u = freq_coord(x, wd), v = freq_coord(y, hd);
+    const float radius = sqrtf(u * u + v * v);
+    /* fsim_lowpassfilter.glsl:37-38, cutoff 0.45, order 15 */
+    const float lowpass = 1.0f / (1.0f + powf(radius / 0.45f, 2.0f * 15.0f));
+    /* fsim_log_gabor.glsl:24-48 */
+    const float ls = logf(0.55f);
+    for (int s = 0; s < 4; ++s) {
+        const float wavelength = 6.0f * (float)(1 << s);
+        const float frequency = 1.0f / wavelength;
+        const float l = logf(radius / frequency);
+        float value = expf(-(l * l) / (2.0f * (ls * ls))) * lowpass;
+        if (x == 0 && y == 0) value = 0.0f;
+        lg[s * N + idx] = value;
+    }
+    /* fsim_angular.glsl:24-46 */
+    const float pi = 3.141592653589f;
+    const float thetaSigma = pi / 4.0f / 1.2f;
+    const float theta = atan2f(-v, u);
+    const float st = sinf(theta), ct = cosf(theta);
+    for (int o = 0; o < 4; ++o) {
+        const float angle = ((float)o * pi) / 4.0f;
+        const float ds = st * cosf(angle) - ct * sinf(angle);
+        const float dc = ct * cosf(angle) + st * sinf(angle);
+        const float dtheta = fabsf(atan2f(ds, dc));
+        ang[o * N + idx] = expf(-(dtheta * dtheta) / (2.0f * (thetaSigma * thetaSigma)));
+    }
+}
+
+/* Partial sums for EM_n[o] = sum_k (LG_0 * A_o)^2 over every bin (F10). */
+__global__ __launch_bounds__(kBlock) void fsim_emn_kernel(const float* lg, const float* ang,
+                                                          long N, double* partial) {
+    __shared__ double sh[kBlock];
+    const int o = blockIdx.y;
+    double acc = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (long)gridDim.x * blockDim.x) {
+        float hv = lg[i] * ang[o * N + i];
+        acc += (double)hv * (double)hv;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[o * gridDim.x + blockIdx.x] = sh[0];
+}
+
+/* Plane z = 4 o + s holds (LG_s * A_o, 0) (fsim_filter_combinations.glsl:51-52). */
+__global__ __launch_bounds__(kBlock) void fsim_bank_planes_kernel(const float* lg,
+                                                                  const float* ang, long N,
+                                                                  float2* planes) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int z = blockIdx.y, s = z & 3, o = z >> 2;
+    planes[z * N + idx] = make_float2(lg[s * N + idx] * ang[o * N + idx], 0.0f);
+}
+
+/* est (F14): the reference multiplies by N and only covers complex elements
+ * t < ceil(N/2) (fsim_mult_filters.glsl:30-33 bounds x = 2*gid by N); the
+ * other half of its buffer is never written and is read as zero here. */
+__global__ __launch_bounds__(kBlock) void fsim_est_kernel(const float2* h, long N, long half,
+                                                          double* partial) {
+    __shared__ double sh[2][kBlock];
+    const int o = blockIdx.y;
+    double a2 = 0, cr = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < half;
+         t += (long)gridDim.x * blockDim.x) {
+        double A[4];
+        for (int s = 0; s < 4; ++s) A[s] = (double)h[(4 * o + s) * N + t].x;
+        double s2 = 0, c = 0;
+        for (int s = 0; s < 4; ++s) s2 += A[s] * A[s] * (double)N;
+        for (int i = 0; i < 3; ++i)
+            for (int j = i + 1; j < 4; ++j) c += A[i] * A[j] * (double)N;
+        a2 += s2;
+        cr += c;
+    }
+    sh[0][threadIdx.x] = a2;
+    sh[1][threadIdx.x] = cr;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[(2 * o) * gridDim.x + blockIdx.x] = sh[0][0];
+        partial[(2 * o + 1) * gridDim.x + blockIdx.x] = sh[1][0];
+    }
+}
+
+const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
+    auto key = std::make_pair(wd, hd);
+    auto it = fsim_plans_.find(key);
+    if (it != fsim_plans_.end()) return it->second;
+    FsimSizePlan P;
+    P.wd = wd;
+    P.hd = hd;
+    P.N = (long)wd * hd;
+    const long N = P.N;
+    IQM_HIP(hipMalloc(&P.lg, sizeof(float) * 4 * N));
+    IQM_HIP(hipMalloc(&P.ang, sizeof(float) * 4 * N));
+    launch("fsim_tables", fsim_tables_kernel, dim3(ceil_div(N, kBlock)), dim3(kBlock), 0, P.lg,
+           P.ang, wd, hd);
+
+    const int nb = 64;
+    double* partial = (double*)workspace(WS_PLAN_TMP, sizeof(double) * 8 * nb + sizeof(float2) * 16 * N + 256);
+    float2* planes = (float2*)((char*)partial + ((sizeof(double) * 8 * nb + 255) / 256) * 256);
+    launch("fsim_emn", fsim_emn_kernel, dim3(nb, 4), dim3(kBlock), 0, (const float*)P.lg,
+           (const float*)P.ang, N, partial);
+    std::vector<double> hp(8 * nb);
+    IQM_HIP(hipMemcpyAsync(hp.data(), partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, stream_));
+    IQM_HIP(hipStreamSynchronize(stream_));
+    for (int o = 0; o < 4; ++o) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += hp[o * nb + b];
+        P.em_n[o] = s;
+    }
+    launch("fsim_bank_planes", fsim_bank_planes_kernel, dim3(ceil_div(N, kBlock), 16),
+           dim3(kBlock), 0, (const float*)P.lg, (const float*)P.ang, N, planes);
+    fft2d_dev(planes, wd, hd, 16, +1, true);
+    launch("fsim_est", fsim_est_kernel, dim3(nb, 4), dim3(kBlock), 0, (const float2*)planes, N,
+           (N + 1) / 2, partial);
+    IQM_HIP(hipMemcpyAsync(hp.data(), partial, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, stream_));
+    IQM_HIP(hipStreamSynchronize(stream_));
+    for (int j = 0; j < 8; ++j) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += hp[j * nb + b];
+        P.est[j] = s;
+    }
+    return fsim_plans_.emplace(key, P).first->second;
+}
+
+/* ---- hot path kernels -------------------------------------------------- */
+
+constexpr int kHistBins = 2048;
+
+/* F2: fsim_downsample.glsl:38-54.  yiq planes per pair: Y_in, Y_ref, I_in,
+ * Q_in, I_ref, Q_ref.  Block (0,0) of each pair also clears the pair's
+ * median histograms. */
+__global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
+                                                           const uint8_t* __restrict__ ref,
+                                                           long pair_stride, int w, int h,
+                                                           int ch, int F, int wd, int hd,
+                                                           float* __restrict__ yiq,
+                                                           unsigned* __restrict__ hist) {
+    const int pair = blockIdx.z;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        unsigned* hp = hist + (long)pair * 8 * kHistBins;
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        for (int i = tid; i < 8 * kHistBins; i += blockDim.x * blockDim.y) hp[i] = 0u;
+    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= wd || y >= hd) return;
+    const long N = (long)wd * hd;
+    const long idx = (long)y * wd + x;
+    const float scaler = (float)F * (float)F;
+    const int xs = x * F - F / 2, ys = y * F - F / 2;
+    float out[2][3];
+    for (int im = 0; im < 2; ++im) {
+        const uint8_t* src = (im ? ref : img) + (long)pair * pair_stride;
+        float si = 0.f, sq = 0.f, sy = 0.f;
+        for (int j = xs; j < xs + F; ++j) {
+            for (int k = ys; k < ys + F; ++k) {
+                if (j >= 0 && j < w && k >= 0 && k < h) {
+                    const uint8_t* p = src + ((long)k * w + j) * ch;
+                    float R, G, B;
+                    if (ch == 1) { R = G = B = (float)p[0]; }
+                    else { R = (float)p[0]; G = (float)p[1]; B = (float)p[2]; }
+                    si += R * 0.596f - G * 0.274f - B * 0.322f;
+                    sq += R * 0.211f - G * 0.523f + B * 0.312f;
+                    sy += R * 0.299f + G * 0.587f + B * 0.114f;
+                }
+            }
+        }
+        out[im][0] = si / scaler;
+        out[im][1] = sq / scaler;
+        out[im][2] = sy / scaler;
+    }
+    float* base = yiq + (long)pair * 6 * N;
+    base[0 * N + idx] = out[0][2];
+    base[1 * N + idx] = out[1][2];
+    base[2 * N + idx] = out[0][0];
+    base[3 * N + idx] = out[0][1];
+    base[4 * N + idx] = out[1][0];
+    base[5 * N + idx] = out[1][1];
+}
+
+/* F3 (fsim_gradientmap.glsl:38-56) and the row half of F5 on the packed
+ * rows Y_in + i*Y_ref; the two real-input row spectra are split with
+ * X(k) = (Z(k) + conj Z(-k)) / 2, Y(k) = (Z(k) - conj Z(-k)) / (2i). */
+template <int RB>
+__global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* __restrict__ yiq,
+                                                                  float* __restrict__ grad,
+                                                                  float2* __restrict__ A,
+                                                                  int wd, int hd, LinePlan P) {
+    extern __shared__ float2 lds[];
+    const int ld = lds_pitch(wd);
+    float2* b0 = lds;
+    float2* b1 = lds + RB * ld;
+    const int pair = blockIdx.y;
+    const long N = (long)wd * hd;
+    const int y0 = blockIdx.x * RB;
+    const int nr = min(RB, hd - y0);
+    const float* Yin = yiq + (long)pair * 6 * N;
+    const float* Yref = Yin + N;
+    const float vert[9] = {3.f, 0.f, -3.f, 10.f, 0.f, -10.f, 3.f, 0.f, -3.f};
+    const float hori[9] = {3.f, 10.f, 3.f, 0.f, 0.f, 0.f, -3.f, -10.f, -3.f};
+    for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
+        const int r = idx / wd, x = idx - r * wd, y = y0 + r;
+        const long g = (long)y * wd + x;
+        for (int im = 0; im < 2; ++im) {
+            const float* Y = im ? Yref : Yin;
+            float vs = 0.f, hs = 0.f;
+            for (int j = -1; j < 2; ++j) {
+                for (int k = -1; k < 2; ++k) {
+                    const int px = x + j, py = y + k;
+                    if (px >= 0 && px < wd && py >= 0 && py < hd) {
+                        const float v = Y[(long)py * wd + px];
+                        const int i3 = (j + 1) + (k + 1) * 3;
+                        vs += v * vert[i3];
+                        hs += v * hori[i3];
+                    }
+                }
+            }
+            const float gv = vs / 16.0f, gh = hs / 16.0f;
+            grad[((long)pair * 2 + im) * N + g] = sqrtf(gv * gv + gh * gh);
+        }
+        b0[r * ld + x] = make_float2(Yin[g], Yref[g]);
+    }
+    __syncthreads();
+    float2* R = lds_fft<-1>(b0, b1, ld, nr, P);
+    float2* Ain = A + (long)pair * 2 * N;
+    float2* Aref = Ain + N;
+    for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
+        const int r = idx / wd, k = idx - r * wd;
+        const int m = (k == 0) ? 0 : wd - k;
+        const float2 zk = R[r * ld + k], zm = R[r * ld + m];
+        const long g = (long)(y0 + r) * wd + k;
+        Ain[g] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        Aref[g] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+    }
+}
+
+/* Column half of F5, then F9 and the column half of F11: a workgroup owns
+ * BX adjacent columns of one image and one orientation; the forward
+ * column FFT gives the spectrum, each scale's filter product is inverse
+ * transformed along the columns and written to plane (img*4 + o)*4 + s. */
+template <int BX>
+__global__ __launch_bounds__(kBlock) void fsim_colpass_kernel(const float2* __restrict__ A,
+                                                              float2* __restrict__ Tout,
+                                                              const float* __restrict__ lg,
+                                                              const float* __restrict__ ang,
+                                                              int wd, int hd, LinePlan P) {
+    extern __shared__ float2 lds[];
+    const int ld = lds_pitch(hd);
+    float2* b0 = lds;
+    float2* b1 = lds + BX * ld;
+    float2* b2 = lds + 2 * BX * ld;
+    const int io = blockIdx.y, img = io >> 2, o = io & 3;
+    const int pair = blockIdx.z;
+    const long N = (long)wd * hd;
+    const int x0 = blockIdx.x * BX;
+    const int nc = min(BX, wd - x0);
+    const float2* src = A + ((long)pair * 2 + img) * N;
+    for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
+        const int y = idx / nc, c = idx - y * nc;
+        b0[c * ld + y] = src[(long)y * wd + x0 + c];
+    }
+    __syncthreads();
+    float2* Fs = lds_fft<-1>(b0, b1, ld, nc, P);
+    float2* W = (Fs == b0) ? b1 : b0;
+    const float* angp = ang + (long)o * N;
+    for (int s = 0; s < 4; ++s) {
+        const float* lgp = lg + (long)s * N;
+        for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
+            const int y = idx / nc, c = idx - y * nc;
+            const long g = (long)y * wd + x0 + c;
+            const float hv = lgp[g] * angp[g];
+            W[c * ld + y] = cscale(Fs[c * ld + y], hv);
+        }
+        __syncthreads();
+        float2* R = lds_fft<1>(W, b2, ld, nc, P);
+        float2* dst = Tout + ((long)pair * 32 + io * 4 + s) * N;
+        for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
+            const int y = idx / nc, c = idx - y * nc;
+            dst[(long)y * wd + x0 + c] = R[c * ld + y];
+        }
+        __syncthreads();
+    }
+}
+
+/* Row half of F11 (normalised by 1/N) fused with F12
+ * (fsim_sum_filter_responses.glsl:33-83) and the F13 packing
+ * (fsim_pack_for_median.glsl:33-44). */
+template <int RB>
+__global__ __launch_bounds__(kBlock) void fsim_rowpass_kernel(const float2* __restrict__ Tin,
+                                                              float* __restrict__ an_e,
+                                                              float* __restrict__ keys,
+                                                              unsigned* __restrict__ hist,
+                                                              int wd, int hd, LinePlan P,
+                                                              float invN) {
+    extern __shared__ float2 lds[];
+    const int ld = lds_pitch(wd);
+    const int nl = 4 * RB;
+    float2* b0 = lds;
+    float2* b1 = lds + nl * ld;
+    const int io = blockIdx.y;
+    const int pair = blockIdx.z;
+    const long N = (long)wd * hd;
+    const int y0 = blockIdx.x * RB;
+    const int nr = min(RB, hd - y0);
+    for (int idx = threadIdx.x; idx < nl * wd; idx += blockDim.x) {
+        const int l = idx / wd, x = idx - l * wd;
+        const int s = l / RB, r = l - s * RB;
+        float2 v = make_float2(0.f, 0.f);
+        if (r < nr) v = Tin[((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * wd + x];
+        b0[l * ld + x] = v;
+    }
+    __syncthreads();
+    float2* R = lds_fft<1>(b0, b1, ld, nl, P);
+    const size_t bufbytes = (size_t)nl * ld * sizeof(float2);
+    unsigned* hl = (bufbytes >= kHistBins * sizeof(unsigned))
+                       ? (unsigned*)((R == b0) ? b1 : b0)
+                       : (unsigned*)(lds + 2 * nl * ld);
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
+    __syncthreads();
+    float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
+    float* enp = anp + N;
+    float* kp = keys + ((long)pair * 8 + io) * N;
+    for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
+        const int r = idx / wd, x = idx - r * wd;
+        float2 eo[4];
+        float amp = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            eo[s] = cscale(R[(s * RB + r) * ld + x], invN);
+            amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
+            sr += eo[s].x;
+            si += eo[s].y;
+        }
+        const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
+        const float me = sr / xe, mo = si / xe;
+        float energy = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+        const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+        const long g = (long)(y0 + r) * wd + x;
+        anp[g] = amp;
+        enp[g] = energy;
+        kp[g] = key;
+        atomicAdd(&hl[(__float_as_uint(key) >> 20) & 2047u], 1u);
+    }
+    __syncthreads();
+    unsigned* hg = hist + ((long)pair * 8 + io) * kHistBins;
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) {
+        const unsigned c = hl[i];
+        if (c) atomicAdd(&hg[i], c);
+    }
+}
+
+/* Finds the bucket holding the element of rank k (0-based, ascending) in
+ * counts[0..nbins); results in *bucket / *rest (shared).  All threads of
+ * the block must call it. */
+__device__ void rank_bucket(const unsigned* counts, int nbins, unsigned k, int* bucket,
+                            unsigned* rest, unsigned* wsum) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int per = (nbins + nthr - 1) / nthr;
+    const int b0 = tid * per;
+    unsigned local = 0;
+    for (int b = b0; b < b0 + per && b < nbins; ++b) local += counts[b];
+    /* wave64 inclusive scan */
+    const int lane = tid & 63, wid = tid >> 6;
+    unsigned inc = local;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned acc = 0;
+        const int nw = (nthr + 63) / 64;
+        for (int i = 0; i < nw; ++i) {
+            unsigned t = wsum[i];
+            wsum[i] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+    const unsigned excl = wsum[wid] + inc - local;
+    if (local > 0 && k >= excl && k < excl + local) {
+        unsigned cum = excl;
+        for (int b = b0; b < b0 + per && b < nbins; ++b) {
+            const unsigned c = counts[b];
+            if (k < cum + c) {
+                *bucket = b;
+                *rest = k - cum;
+                break;
+            }
+            cum += c;
+        }
+    }
+    __syncthreads();
+}
+
+struct NoiseArgs {
+    double em_n[4];
+    double est[8];
+};
+
+/* F13 + F15 thresholds: exact element N/2 of the ascending keys
+ * (fsim_noise_power.cpp:198-199) by a 3-level radix select on the float bit
+ * patterns (keys are >= 0, so the bits order like the values): level 1 uses
+ * the histogram built by fsim_rowpass (bits 30..20), level 2 bits 19..9,
+ * level 3 bits 8..0. */
+__global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restrict__ keys,
+                                                           const unsigned* __restrict__ hist,
+                                                           float* __restrict__ thr,
+                                                           double* __restrict__ stats, long N,
+                                                           NoiseArgs na) {
+    __shared__ unsigned h[kHistBins];
+    __shared__ unsigned wsum[16];
+    __shared__ int bucket;
+    __shared__ unsigned rest;
+    const int io = blockIdx.x, pair = blockIdx.y;
+    const long slot = (long)pair * 8 + io;
+    const unsigned* kb = (const unsigned*)(keys + slot * N);
+    const unsigned* hg = hist + slot * kHistBins;
+    const unsigned k = (unsigned)(N / 2);
+
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = hg[i];
+    if (threadIdx.x == 0) { bucket = 0; rest = 0; }
+    __syncthreads();
+    rank_bucket(h, kHistBins, k, &bucket, &rest, wsum);
+    const unsigned b1 = (unsigned)bucket, k1 = rest;
+
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    for (long i = threadIdx.x; i < N; i += blockDim.x) {
+        const unsigned u = kb[i];
+        if (((u >> 20) & 2047u) == b1) atomicAdd(&h[(u >> 9) & 2047u], 1u);
+    }
+    __syncthreads();
+    rank_bucket(h, 2048, k1, &bucket, &rest, wsum);
+    const unsigned b2 = (unsigned)bucket, k2 = rest;
+
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const unsigned pre = (b1 << 11) | b2;
+    for (long i = threadIdx.x; i < N; i += blockDim.x) {
+        const unsigned u = kb[i];
+        if (((u >> 9) & 0x3FFFFFu) == pre) atomicAdd(&h[u & 511u], 1u);
+    }
+    __syncthreads();
+    rank_bucket(h, 512, k2, &bucket, &rest, wsum);
+    if (threadIdx.x == 0) {
+        const unsigned bits = (b1 << 20) | (b2 << 9) | (unsigned)bucket;
+        const double median = (double)__uint_as_float(bits);
+        const int o = io & 3;
+        /* fsim_noise_power.cpp:199-201 */
+        const double mean = -median / log(0.5);
+        const double np = mean / na.em_n[o];
+        /* fsim_phase_congruency.glsl:45-53 */
+        const double pi = 3.141592653589;
+        const double e2 = 2.0 * np * na.est[2 * o] + 4.0 * np * na.est[2 * o + 1];
+        const double tau = sqrt(e2 / 2.0);
+        const double ene = tau * sqrt(pi / 2.0);
+        const double sig = sqrt((2.0 - pi / 2.0) * (tau * tau));
+        const double T = (ene + 2.0 * sig) / 1.7;
+        thr[slot] = (float)T;
+        double* st = stats + slot * 4;
+        st[0] = median;
+        st[1] = np;
+        st[2] = T;
+        st[3] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+/* F15 (fsim_phase_congruency.glsl:40-63) + F16 (fsim_final_multiply.glsl:
+ * 26-57) + per-block FP64 partials of F17. */
+__global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
+    const float* __restrict__ yiq, const float* __restrict__ grad, const float* __restrict__ an_e,
+    const float* __restrict__ thr, double* __restrict__ partial, long N, float* simc_map,
+    float* pc_map, float* pcm_map, float* sim_map) {
+    __shared__ double red[3][kBlock / 64];
+    const int pair = blockIdx.y;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v0 = 0, v1 = 0, v2 = 0;
+    if (idx < N) {
+        const float* ae = an_e + (long)pair * 16 * N;
+        const float* T = thr + (long)pair * 8;
+        float pc[2];
+        for (int im = 0; im < 2; ++im) {
+            float et = 0.f, at = 0.f;
+            for (int o = 0; o < 4; ++o) {
+                const int io = im * 4 + o;
+                const float amp = ae[(2 * io) * N + idx];
+                const float en = ae[(2 * io + 1) * N + idx];
+                et += fmaxf(en - T[io], 0.0f);
+                at += amp;
+            }
+            pc[im] = et / at; /* no epsilon: NaN where sum An = 0 */
+        }
+        const float* yq = yiq + (long)pair * 6 * N;
+        const float* gr = grad + (long)pair * 2 * N;
+        const float T1 = 0.85f, T2 = 160.0f, T3 = 200.0f, T4 = 200.0f, lambda = 0.03f;
+        const float p1 = pc[0], p2 = pc[1];
+        const float pcm = (p1 < p2) ? p2 : p1;
+        const float pcsim = (2.0f * p1 * p2 + T1) / (p1 * p1 + p2 * p2 + T1);
+        const float g1 = gr[idx], g2 = gr[N + idx];
+        const float gsim = (2.0f * g1 * g2 + T2) / (g1 * g1 + g2 * g2 + T2);
+        const float sim = gsim * pcsim * pcm;
+        const float i1 = yq[2 * N + idx], q1 = yq[3 * N + idx];
+        const float i2 = yq[4 * N + idx], q2 = yq[5 * N + idx];
+        const float isim = (2.0f * i1 * i2 + T3) / (i1 * i1 + i2 * i2 + T3);
+        const float qsim = (2.0f * q1 * q2 + T4) / (q1 * q1 + q2 * q2 + T4);
+        const float simc = gsim * pcsim * powf(fabsf(isim * qsim), lambda) * pcm;
+        v0 = pcm;
+        v1 = sim;
+        v2 = simc;
+        const long mo = (long)pair * N + idx;
+        if (simc_map) simc_map[mo] = simc;
+        if (pcm_map) pcm_map[mo] = pcm;
+        if (sim_map) sim_map[mo] = sim;
+        if (pc_map) {
+            pc_map[(long)pair * 2 * N + idx] = p1;
+            pc_map[(long)pair * 2 * N + N + idx] = p2;
+        }
+    }
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    v2 = wave_sum(v2);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = v0;
+        red[1][wid] = v1;
+        red[2][wid] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s[3] = {0, 0, 0};
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w)
+            for (int j = 0; j < 3; ++j) s[j] += red[j][w];
+        double* p = partial + ((long)pair * gridDim.x + blockIdx.x) * 4;
+        p[0] = s[0];
+        p[1] = s[1];
+        p[2] = s[2];
+        p[3] = 0.0;
+    }
+}
+
+/* F17 (fsim_final_multiply.cpp:294-300): fixed-order sum of the block
+ * partials; fsim = sum sim / sum pcm, fsimc = sum simc / sum pcm. */
+__global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __restrict__ partial,
+                                                               int nblk, double* __restrict__ res) {
+    __shared__ double sh[3][kBlock];
+    const int pair = blockIdx.x;
+    double s[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double* p = partial + ((long)pair * nblk + b) * 4;
+        for (int j = 0; j < 3; ++j) s[j] += p[j];
+    }
+    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = s[j];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* r = res + (long)pair * 8;
+        r[0] = sh[0][0];
+        r[1] = sh[1][0];
+        r[2] = sh[2][0];
+        r[3] = sh[1][0] / sh[0][0];
+        r[4] = sh[2][0] / sh[0][0];
+        r[5] = r[6] = r[7] = 0.0;
+    }
+}
+
+/* ---- host orchestration ------------------------------------------------- */
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+namespace {
+struct FsimRegions {
+    size_t yiq, grad, A, T, an_e, keys, hist, thr, stats, partial, res, total;
+};
+FsimRegions fsim_regions(long N, int n, int nblk) {
+    FsimRegions r{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += align256(bytes);
+        return o;
+    };
+    r.yiq = take(sizeof(float) * 6 * N * n);
+    r.grad = take(sizeof(float) * 2 * N * n);
+    r.A = take(sizeof(float2) * 2 * N * n);
+    r.T = take(sizeof(float2) * 32 * N * n);
+    r.an_e = take(sizeof(float) * 16 * N * n);
+    r.keys = take(sizeof(float) * 8 * N * n);
+    r.hist = take(sizeof(unsigned) * 8 * kHistBins * n);
+    r.thr = take(sizeof(float) * 8 * n);
+    r.stats = take(sizeof(double) * 8 * 4 * n);
+    r.partial = take(sizeof(double) * 4 * (size_t)nblk * n);
+    r.res = take(sizeof(double) * 8 * n);
+    r.total = off;
+    return r;
+}
+}  // namespace
+
+Ctx::FsimLayout Ctx::fsim_layout(long N, int n, char* base) const {
+    const int nblk = ceil_div(N, kBlock);
+    FsimRegions r = fsim_regions(N, n, nblk);
+    FsimLayout L;
+    L.N = N;
+    L.yiq = (float*)(base + r.yiq);
+    L.grad = (float*)(base + r.grad);
+    L.an_e = (float*)(base + r.an_e);
+    L.keys = (float*)(base + r.keys);
+    L.stats = (double*)(base + r.stats);
+    L.res = (double*)(base + r.res);
+    return L;
+}
+
+double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                        long pair_stride, int n, const FsimBatchOut& out) {
+    int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+    const int wd = (int)std::round((float)w / (float)F);
+    const int hd = (int)std::round((float)h / (float)F);
+    IQM_CHECK(wd >= 2 && hd >= 2, IQM_ERR_TOO_SMALL, "image too small for FSIM");
+    const FsimSizePlan& plan = fsim_plan(wd, hd);
+    const LinePlan& Px = line_plan(wd).plan;
+    const LinePlan& Py = line_plan(hd).plan;
+    const long N = (long)wd * hd;
+    const int nblk = ceil_div(N, kBlock);
+    FsimRegions r = fsim_regions(N, n, nblk);
+    char* base = (char*)workspace(WS_FSIM, r.total);
+    fsim_ws_ = base;
+    float* yiq = (float*)(base + r.yiq);
+    float* grad = (float*)(base + r.grad);
+    float2* A = (float2*)(base + r.A);
+    float2* T = (float2*)(base + r.T);
+    float* an_e = (float*)(base + r.an_e);
+    float* keys = (float*)(base + r.keys);
+    unsigned* hist = (unsigned*)(base + r.hist);
+    float* thr = (float*)(base + r.thr);
+    double* stats = (double*)(base + r.stats);
+    double* partial = (double*)(base + r.partial);
+    double* res = (double*)(base + r.res);
+
+    const int ldx = lds_pitch(wd), ldy = lds_pitch(hd);
+    /* F2 */
+    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, 16), ceil_div(hd, 16), n),
+           dim3(16, 16), 0, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
+    /* F3 + row half of F5 */
+    {
+        constexpr int RB = 4;
+        size_t sh = 2 * RB * ldx * sizeof(float2);
+        if (sh <= (size_t)kMaxLdsBytes) {
+            launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel<RB>, dim3(ceil_div(hd, RB), n),
+                   dim3(kBlock), sh, (const float*)yiq, grad, A, wd, hd, Px);
+        } else {
+            sh = 2 * ldx * sizeof(float2);
+            IQM_CHECK(sh <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "row too long for LDS");
+            launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel<1>, dim3(hd, n), dim3(kBlock), sh,
+                   (const float*)yiq, grad, A, wd, hd, Px);
+        }
+    }
+    /* column half of F5, F9, column half of F11 */
+    {
+        constexpr int BX = 8;
+        size_t sh = 3 * BX * ldy * sizeof(float2);
+        if (sh <= (size_t)kMaxLdsBytes) {
+            launch("fsim_colpass", fsim_colpass_kernel<BX>, dim3(ceil_div(wd, BX), 8, n),
+                   dim3(kBlock), sh, (const float2*)A, T, (const float*)plan.lg,
+                   (const float*)plan.ang, wd, hd, Py);
+        } else {
+            sh = 3 * ldy * sizeof(float2);
+            IQM_CHECK(sh <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "column too long for LDS");
+            launch("fsim_colpass", fsim_colpass_kernel<1>, dim3(wd, 8, n), dim3(kBlock), sh,
+                   (const float2*)A, T, (const float*)plan.lg, (const float*)plan.ang, wd, hd, Py);
+        }
+    }
+    /* row half of F11 + F12 + F13 packing */
+    {
+        const float invN = (float)(1.0 / (double)N);
+        auto shmem = [&](int rb) {
+            size_t buf = (size_t)4 * rb * ldx * sizeof(float2);
+            return 2 * buf + (buf >= kHistBins * sizeof(unsigned) ? 0 : kHistBins * sizeof(unsigned));
+        };
+        if (shmem(2) <= (size_t)kMaxLdsBytes) {
+            launch("fsim_rowpass", fsim_rowpass_kernel<2>, dim3(ceil_div(hd, 2), 8, n),
+                   dim3(kBlock), shmem(2), (const float2*)T, an_e, keys, hist, wd, hd, Px, invN);
+        } else {
+            IQM_CHECK(shmem(1) <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "row too long for LDS");
+            launch("fsim_rowpass", fsim_rowpass_kernel<1>, dim3(hd, 8, n), dim3(kBlock), shmem(1),
+                   (const float2*)T, an_e, keys, hist, wd, hd, Px, invN);
+        }
+    }
+    /* F13 + thresholds */
+    NoiseArgs na;
+    for (int i = 0; i < 4; ++i) na.em_n[i] = plan.em_n[i];
+    for (int i = 0; i < 8; ++i) na.est[i] = plan.est[i];
+    launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), 0, (const float*)keys,
+           (const unsigned*)hist, thr, stats, N, na);
+    /* F15 + F16 + F17 */
+    launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(nblk, n), dim3(kBlock), 0, (const float*)yiq,
+           (const float*)grad, (const float*)an_e, (const float*)thr, partial, N, out.d_simc,
+           out.d_pc, out.d_pcm, out.d_sim);
+    launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
+           nblk, res);
+    return res;
+}
+
+}  // namespace iqm

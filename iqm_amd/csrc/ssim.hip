@@ -1,0 +1,211 @@
+// Fused SSIM on MI355X.
+//
+// Replaces the reference's three dispatches (shaders/ssim/ssim_lumapack.glsl,
+// ssim_gaussinput.glsl, ssim.glsl) and the host-side MSSIM loop
+// (src/gpu/ssim/ssim.cpp:386-401) with one LDS-tiled kernel plus a small
+// fixed-order reduction:
+//   1. a (TX+2R) x (TY+2R) tile of both images is read once from HBM and
+//      turned into luma (0.299 R + 0.581 G + 0.114 B on UNORM values) minus a
+//      per-tile shift (the luma of the tile centre), which keeps the
+//      E[x^2] - mu^2 form of the variance free of cancellation;
+//   2. a horizontal 2R+1-tap pass writes the five weighted sums x, y, x^2,
+//      y^2, xy per row to LDS;
+//   3. a vertical pass finishes the 2-D sums, renormalises them by the
+//      in-bounds weight mass (the Gaussian window is separable, so the
+//      in-bounds set is a rectangle and its mass is Gx(x) * Gy(y)),
+//      evaluates the SSIM formula and stores the map;
+//   4. pixels inside the MSSIM crop [R, W-R] x [R, H-R] are summed into one
+//      FP64 partial per workgroup; ssim_finalize_kernel adds the partials in
+//      a fixed order.
+#include "ctx.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace iqm {
+
+constexpr int kMaxRadius = 15;
+
+struct SsimWeights {
+    float g[2 * kMaxRadius + 1]; /* g[d + R] = exp(-d^2 / (2 sigma^2)) */
+};
+
+__device__ __forceinline__ float luma_at(const uint8_t* p, int ch) {
+    float r, g, b;
+    if (ch == 1) {
+        r = g = b = (float)p[0] / 255.0f;
+    } else {
+        r = (float)p[0] / 255.0f;
+        g = (float)p[1] / 255.0f;
+        b = (float)p[2] / 255.0f;
+    }
+    return 0.299f * r + 0.581f * g + 0.114f * b;
+}
+
+template <int R, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void ssim_kernel(const uint8_t* __restrict__ img,
+                                                      const uint8_t* __restrict__ ref,
+                                                      long pair_stride, int w, int h, int ch,
+                                                      SsimWeights wt, float c1, float c2,
+                                                      float* __restrict__ map,
+                                                      double* __restrict__ partial) {
+    constexpr int LW = TX + 2 * R, LH = TY + 2 * R;
+    __shared__ float lx[LH][LW];
+    __shared__ float ly[LH][LW];
+    __shared__ float hp[5][LH][TX];
+    __shared__ double red[kBlock / 64];
+
+    const int pair = blockIdx.z;
+    const uint8_t* A = img + (long)pair * pair_stride;
+    const uint8_t* B = ref + (long)pair * pair_stride;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.x;
+
+    const int cx = min(x0 + TX / 2, w - 1), cy = min(y0 + TY / 2, h - 1);
+    const float sx = luma_at(A + ((long)cy * w + cx) * ch, ch);
+    const float sy = luma_at(B + ((long)cy * w + cx) * ch, ch);
+
+    for (int i = tid; i < LH * LW; i += blockDim.x) {
+        const int ry = i / LW, rx = i - ry * LW;
+        const int gx = x0 - R + rx, gy = y0 - R + ry;
+        float a = 0.f, b = 0.f;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+            const long o = ((long)gy * w + gx) * ch;
+            a = luma_at(A + o, ch) - sx;
+            b = luma_at(B + o, ch) - sy;
+        }
+        lx[ry][rx] = a;
+        ly[ry][rx] = b;
+    }
+    __syncthreads();
+
+    for (int i = tid; i < LH * TX; i += blockDim.x) {
+        const int ry = i / TX, c = i - ry * TX;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f;
+#pragma unroll
+        for (int d = 0; d <= 2 * R; ++d) {
+            const float g = wt.g[d];
+            const float a = lx[ry][c + d], b = ly[ry][c + d];
+            const float ga = g * a, gb = g * b;
+            s0 += ga;
+            s1 += gb;
+            s2 += ga * a;
+            s3 += gb * b;
+            s4 += ga * b;
+        }
+        hp[0][ry][c] = s0;
+        hp[1][ry][c] = s1;
+        hp[2][ry][c] = s2;
+        hp[3][ry][c] = s3;
+        hp[4][ry][c] = s4;
+    }
+    __syncthreads();
+
+    double crop = 0.0;
+    constexpr int ROWS_PER_PASS = kBlock / TX;
+    const int c = tid % TX;
+    const int x = x0 + c;
+    float gxs = 0.f;
+#pragma unroll
+    for (int d = -R; d <= R; ++d)
+        if (x + d >= 0 && x + d < w) gxs += wt.g[d + R];
+    for (int r = tid / TX; r < TY; r += ROWS_PER_PASS) {
+        const int y = y0 + r;
+        if (x >= w || y >= h) continue;
+        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, gys = 0.f;
+#pragma unroll
+        for (int d = 0; d <= 2 * R; ++d) {
+            const float g = wt.g[d];
+            m0 += g * hp[0][r + d][c];
+            m1 += g * hp[1][r + d][c];
+            m2 += g * hp[2][r + d][c];
+            m3 += g * hp[3][r + d][c];
+            m4 += g * hp[4][r + d][c];
+            const int yy = y + d - R;
+            if (yy >= 0 && yy < h) gys += g;
+        }
+        const float inv = 1.0f / (gxs * gys);
+        const float ma = m0 * inv, mb = m1 * inv;
+        const float va = m2 * inv - ma * ma;
+        const float vb = m3 * inv - mb * mb;
+        const float cov = m4 * inv - ma * mb;
+        const float mi = ma + sx, mr = mb + sy;
+        const float v = ((2.0f * mi * mr + c1) * (2.0f * cov + c2)) /
+                        ((mi * mi + mr * mr + c1) * (va + vb + c2));
+        if (map) map[(long)pair * w * h + (long)y * w + x] = v;
+        if (x >= R && x <= w - R && y >= R && y <= h - R) crop += (double)v;
+    }
+    for (int d = 32; d > 0; d >>= 1) crop += __shfl_xor(crop, d, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = crop;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) s += red[i];
+        partial[((long)pair * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __restrict__ partial,
+                                                               int nblk, double count,
+                                                               double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    const int pair = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(long)pair * nblk + b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[pair] = sh[0] / count;
+}
+
+template <int R>
+static void launch_ssim(Ctx& c, dim3 grid, const uint8_t* a, const uint8_t* b, long ps, int w,
+                        int h, int ch, const SsimWeights& wt, float c1, float c2, float* map,
+                        double* partial) {
+    if constexpr (R <= 7)
+        c.launch("ssim", ssim_kernel<R, 64, 16>, grid, dim3(kBlock), 0, a, b, ps, w, h, ch, wt, c1,
+                 c2, map, partial);
+    else
+        c.launch("ssim", ssim_kernel<R, 32, 8>, grid, dim3(kBlock), 0, a, b, ps, w, h, ch, wt, c1,
+                 c2, map, partial);
+}
+
+void Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                     long pair_stride, int n, int ks, float k1, float k2, float sigma,
+                     float* d_maps, double* mssim_host) {
+    IQM_CHECK(ks >= 3 && ks <= 2 * kMaxRadius + 1 && (ks % 2) == 1, IQM_ERR_INVALID,
+              "kernel_size must be odd and in [3, 31]");
+    const int R = (ks - 1) / 2;
+    /* ssim.cpp:390-400 needs a non-empty crop */
+    IQM_CHECK(w - R >= R && h - R >= R, IQM_ERR_TOO_SMALL, "image smaller than the SSIM window");
+    SsimWeights wt{};
+    const double sg = (double)sigma;
+    for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
+    const int TX = R <= 7 ? 64 : 32, TY = R <= 7 ? 16 : 8;
+    dim3 grid(ceil_div(w, TX), ceil_div(h, TY), n);
+    const int nblk = grid.x * grid.y;
+    double* partial = (double*)workspace(WS_SSIM, sizeof(double) * ((size_t)nblk * n + n));
+    double* out = partial + (size_t)nblk * n;
+    const float c1 = k1 * k1, c2 = k2 * k2;
+    switch (R) {
+#define IQM_SSIM_CASE(r) \
+    case r: launch_ssim<r>(*this, grid, d_img, d_ref, pair_stride, w, h, ch, wt, c1, c2, d_maps, partial); break;
+        IQM_SSIM_CASE(1) IQM_SSIM_CASE(2) IQM_SSIM_CASE(3) IQM_SSIM_CASE(4) IQM_SSIM_CASE(5)
+        IQM_SSIM_CASE(6) IQM_SSIM_CASE(7) IQM_SSIM_CASE(8) IQM_SSIM_CASE(9) IQM_SSIM_CASE(10)
+        IQM_SSIM_CASE(11) IQM_SSIM_CASE(12) IQM_SSIM_CASE(13) IQM_SSIM_CASE(14) IQM_SSIM_CASE(15)
+#undef IQM_SSIM_CASE
+        default: break;
+    }
+    const double count = (double)((long)(w - 2 * R + 1) * (long)(h - 2 * R + 1));
+    launch("ssim_finalize", ssim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
+           nblk, count, out);
+    if (mssim_host) {
+        IQM_HIP(hipMemcpyAsync(mssim_host, out, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+        IQM_HIP(hipStreamSynchronize(stream_));
+    }
+}
+
+}  // namespace iqm

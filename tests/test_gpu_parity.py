@@ -1,0 +1,171 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Tolerances (FP32 path): scalars within 1e-5 absolute, maps within 1e-4
+max-abs, against both the FP32 mirror and the FP64 oracle; the errors
+against FP64 are printed for every case.
+"""
+import numpy as np
+import pytest
+
+import iqm_amd
+from iqm_amd import synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SCALAR_TOL = 1e-5
+MAP_TOL = 1e-4
+
+
+def _maxabs(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)))
+
+
+def _report(tag, **errs):
+    print(f"[{tag}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
+
+
+@pytest.mark.parametrize("w,h", [(480, 270), (270, 480), (64, 48), (45, 36), (1, 8), (7, 1)])
+def test_fft2d_matches_numpy(ctx, w, h):
+    rng = np.random.default_rng(w * 1000 + h)
+    x = (rng.standard_normal((3, h, w)) + 1j * rng.standard_normal((3, h, w))).astype(np.complex64)
+    for inverse in (False, True):
+        y = ctx.fft2d(x, inverse=inverse, normalize=inverse)
+        ref = np.fft.ifft2(x.astype(np.complex128)) if inverse else np.fft.fft2(x.astype(np.complex128))
+        rel = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+        _report(f"fft {w}x{h} inv={inverse}", rel=rel)
+        assert rel < 2e-6
+
+
+def test_ssim_gray_512(ctx):
+    img, ref = synth.ssim_gray_pair(512, 512, seed=1)
+    r = iqm_amd.SSIM().compute_metric(ctx, img, ref)
+    m64, map64 = oracle.ssim(img, ref, prec=64)
+    m32, map32 = oracle.ssim(img, ref, prec=32)
+    e = dict(d_mssim64=abs(r.mssim - m64), d_mssim32=abs(r.mssim - m32),
+             d_map64=_maxabs(r.image_data, map64), d_map32=_maxabs(r.image_data, map32))
+    _report("ssim 512 gray", **e)
+    assert e["d_mssim64"] <= SCALAR_TOL and e["d_mssim32"] <= SCALAR_TOL
+    assert e["d_map64"] <= MAP_TOL and e["d_map32"] <= MAP_TOL
+
+
+@pytest.mark.parametrize("w,h,ch", [(1920, 1080, 3), (333, 211, 4), (100, 37, 1), (11, 11, 3)])
+def test_ssim_shapes(ctx, w, h, ch):
+    img, ref = synth.rgb_pair(w, h, seed=w + h)
+    if ch == 1:
+        img, ref = img[:, :, 1].copy(), ref[:, :, 1].copy()
+    elif ch == 4:
+        img, ref = synth.rgba(img), synth.rgba(ref)
+    r = iqm_amd.SSIM().compute_metric(ctx, img, ref)
+    m64, map64 = oracle.ssim(img, ref, prec=64)
+    e = dict(d_mssim64=abs(r.mssim - m64), d_map64=_maxabs(r.image_data, map64))
+    _report(f"ssim {w}x{h}x{ch}", **e)
+    assert e["d_mssim64"] <= SCALAR_TOL and e["d_map64"] <= MAP_TOL
+
+
+def test_ssim_identity_and_params(ctx):
+    img, ref = synth.rgb_pair(200, 150, seed=3)
+    r = iqm_amd.SSIM().compute_metric(ctx, ref, ref)
+    assert abs(r.mssim - 1.0) < 1e-6 and np.max(np.abs(r.image_data - 1.0)) < 1e-5
+    s = iqm_amd.SSIM()
+    s.kernel_size, s.sigma, s.k_1, s.k_2 = 7, 1.0, 0.02, 0.05
+    r = s.compute_metric(ctx, img, ref)
+    m64, map64 = oracle.ssim(img, ref, prec=64, kernel_size=7, k1=0.02, k2=0.05, sigma=1.0)
+    assert abs(r.mssim - m64) <= SCALAR_TOL and _maxabs(r.image_data, map64) <= MAP_TOL
+
+
+def _fsim_case(ctx, img, ref, tag, stages=False):
+    r = iqm_amd.FSIM().compute_metric(ctx, img, ref, maps=iqm_amd.FSIM.MAPS, detail=stages)
+    names = ("simc", "sim", "pcm", "pc_in", "pc_ref")
+    o64 = oracle.fsim(img, ref, prec=64, maps=names + (("yiq_in", "yiq_ref", "grad_in", "grad_ref",
+                                                         "an_e", "keys") if stages else ()))
+    o32 = oracle.fsim(img, ref, prec=32, maps=("simc",))
+    e = dict(d_fsim64=abs(r.fsim - o64.fsim), d_fsimc64=abs(r.fsimc - o64.fsimc),
+             d_fsim32=abs(r.fsim - o32.fsim), d_fsimc32=abs(r.fsimc - o32.fsimc),
+             d_simc64=_maxabs(r.maps["simc"], o64.maps["simc"]),
+             d_simc32=_maxabs(r.maps["simc"], o32.maps["simc"]))
+    for n in ("sim", "pcm", "pc_in", "pc_ref"):
+        e[f"d_{n}64"] = _maxabs(r.maps[n], o64.maps[n])
+    _report(tag, **e)
+    assert e["d_fsim64"] <= SCALAR_TOL and e["d_fsimc64"] <= SCALAR_TOL
+    assert e["d_fsim32"] <= SCALAR_TOL and e["d_fsimc32"] <= SCALAR_TOL
+    for k, v in e.items():
+        if k.startswith("d_") and not k.startswith("d_fsim"):
+            assert v <= MAP_TOL, (k, v)
+    return r, o64
+
+
+def test_fsim_stages_small(ctx):
+    img, ref = synth.rgb_pair(320, 240, seed=11)
+    r, o = _fsim_case(ctx, img, ref, "fsim 320x240 stages", stages=True)
+    d = r.detail
+    np.testing.assert_allclose(d["em_n"], o.em_n, rtol=1e-6)
+    np.testing.assert_allclose(d["est"], o.est, rtol=1e-5)
+    np.testing.assert_allclose(d["median"], o.median, rtol=1e-4)
+    np.testing.assert_allclose(d["T"], o.T, rtol=1e-4)
+    np.testing.assert_allclose(d["sums"], o.sums, rtol=1e-5)
+    for n in ("yiq_in", "yiq_ref", "grad_in", "grad_ref"):
+        scale = max(1.0, float(np.max(np.abs(o.maps[n]))))
+        assert _maxabs(d[n], o.maps[n]) / scale <= 1e-5, n
+    for n in ("an_e", "keys"):
+        scale = float(np.max(np.abs(o.maps[n])))
+        err = _maxabs(d[n], o.maps[n]) / scale
+        _report(f"stage {n}", rel=err)
+        assert err <= 1e-4, n
+
+
+def test_fsim_1080p(ctx):
+    img, ref = synth.rgb_pair(1920, 1080, seed=2)
+    _fsim_case(ctx, img, ref, "fsim 1920x1080 rgb")
+
+
+@pytest.mark.parametrize("w,h,ch", [(512, 512, 1), (640, 480, 4), (999, 601, 3)])
+def test_fsim_shapes(ctx, w, h, ch):
+    img, ref = synth.rgb_pair(w, h, seed=w)
+    if ch == 1:
+        img, ref = img[:, :, 0].copy(), ref[:, :, 0].copy()
+    elif ch == 4:
+        img, ref = synth.rgba(img), synth.rgba(ref)
+    _fsim_case(ctx, img, ref, f"fsim {w}x{h}x{ch}")
+
+
+def test_fsim_identity(ctx):
+    img, ref = synth.rgb_pair(400, 300, seed=5)
+    r = iqm_amd.FSIM().compute_metric(ctx, ref, ref)
+    assert abs(r.fsim - 1.0) < 1e-6 and abs(r.fsimc - 1.0) < 1e-6
+
+
+def test_fsim_batch_dev_matches_single(ctx):
+    w, h, n = 640, 360, 3
+    pairs = [synth.rgb_pair(w, h, seed=100 + i) for i in range(n)]
+    imgs = np.stack([p[0] for p in pairs])
+    refs = np.stack([p[1] for p in pairs])
+    di, dr = ctx.alloc(imgs.nbytes), ctx.alloc(refs.nbytes)
+    try:
+        ctx.upload(di, imgs)
+        ctx.upload(dr, refs)
+        fs, fc = iqm_amd.FSIM().compute_batch_dev(ctx, di, dr, w, h, 3, w * h * 3, n)
+        ss = iqm_amd.SSIM().compute_batch_dev(ctx, di, dr, w, h, 3, w * h * 3, n)
+    finally:
+        ctx.free(di)
+        ctx.free(dr)
+    for i in range(n):
+        r = iqm_amd.FSIM().compute_metric(ctx, pairs[i][0], pairs[i][1])
+        s = iqm_amd.SSIM().compute_metric(ctx, pairs[i][0], pairs[i][1], want_map=False)
+        assert fs[i] == r.fsim and fc[i] == r.fsimc
+        assert ss[i] == s.mssim
+
+
+def test_errors(ctx):
+    a = np.zeros((10, 12, 3), np.uint8)
+    b = np.zeros((10, 13, 3), np.uint8)
+    with pytest.raises(iqm_amd.IqmError, match="Compared images must have the same size"):
+        iqm_amd.SSIM().compute_metric(ctx, a, b)
+    with pytest.raises(iqm_amd.IqmError, match="Compared images must have the same size"):
+        iqm_amd.FSIM().compute_metric(ctx, a, b)
+    tiny = np.zeros((6, 6, 3), np.uint8)
+    with pytest.raises(iqm_amd.IqmError) as ei:
+        iqm_amd.SSIM().compute_metric(ctx, tiny, tiny)
+    assert ei.value.code == iqm_amd.IQM_ERR_TOO_SMALL
