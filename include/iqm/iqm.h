@@ -143,6 +143,11 @@ int iqm_memcpy_d2h(iqm_ctx* ctx, void* dst, const void* src, size_t bytes);
 void* iqm_stream(iqm_ctx* ctx);
 int iqm_sync(iqm_ctx* ctx);
 
+/* Batched device entry points replay a captured hipGraph when called again
+ * with the same arguments (default on; IQM_GRAPHS=0 in the environment turns
+ * it off at context creation). */
+int iqm_set_graphs(iqm_ctx* ctx, int32_t on);
+
 /* ---- stage timing ------------------------------------------------------- */
 
 /* When on, every kernel launch is bracketed by hipEvents on the context's

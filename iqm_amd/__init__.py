@@ -32,7 +32,8 @@ EXPORTS = (
     "iqm_create", "iqm_destroy", "iqm_last_error", "iqm_device_name", "iqm_ssim_default_params",
     "iqm_ssim", "iqm_ssim_batch_dev", "iqm_fsim_dims", "iqm_fsim", "iqm_fsim_batch_dev",
     "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
-    "iqm_stream", "iqm_sync", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
+    "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_set_stage_timing", "iqm_stage_times",
+    "iqm_reset_stage_times",
 )
 
 
@@ -98,6 +99,7 @@ def lib():
     L.iqm_stream.argtypes = [vp]
     L.iqm_stream.restype = vp
     L.iqm_sync.argtypes = [vp]
+    L.iqm_set_graphs.argtypes = [vp, i32]
     L.iqm_set_stage_timing.argtypes = [vp, i32]
     L.iqm_stage_times.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(i64)]
     L.iqm_stage_times.restype = i32
@@ -182,6 +184,9 @@ class Context:
     def download(self, src: int, out: np.ndarray) -> np.ndarray:
         _check(lib().iqm_memcpy_d2h(self.handle, out.ctypes.data, src, out.nbytes))
         return out
+
+    def set_graphs(self, on: bool) -> None:
+        _check(lib().iqm_set_graphs(self.handle, 1 if on else 0))
 
     def set_stage_timing(self, on: bool) -> None:
         _check(lib().iqm_set_stage_timing(self.handle, 1 if on else 0))

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -17,18 +18,25 @@ Ctx::Ctx(int device) : device_(device) {
     IQM_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     IQM_HIP(hipGetDeviceProperties(&prop, device));
-    name_ = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    name_ = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+            std::to_string(prop.multiProcessorCount) + " CUs, " +
+            std::to_string(prop.sharedMemPerBlock / 1024) + " KiB LDS/block)";
+    lds_per_block_ = prop.sharedMemPerBlock;
+    if (const char* g = std::getenv("IQM_GRAPHS")) graphs_ = std::atoi(g) != 0;
     IQM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
 Ctx::~Ctx() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graphs();
     for (auto& kv : ws_) (void)hipFree(kv.second.first);
-    for (auto& kv : line_plans_) (void)hipFree(kv.second.dev_tw);
+    for (auto& kv : line_plans_)
+        for (float2* p : kv.second.owned) (void)hipFree(p);
     for (auto& kv : fsim_plans_) {
         (void)hipFree(kv.second.lg);
         (void)hipFree(kv.second.ang);
+        (void)hipFree(kv.second.bank);
     }
     for (auto& p : pending_) {
         (void)hipEventDestroy(p.second.first);
@@ -41,6 +49,8 @@ Ctx::~Ctx() {
 void* Ctx::workspace(int slot, size_t bytes) {
     auto& e = ws_[slot];
     if (e.second < bytes) {
+        IQM_CHECK(!capturing_, IQM_ERR_DEVICE, "workspace growth during graph capture");
+        drop_graphs(); /* captured graphs hold the old addresses */
         if (e.first) {
             IQM_HIP(hipStreamSynchronize(stream_));
             IQM_HIP(hipFree(e.first));
@@ -56,6 +66,46 @@ void* Ctx::workspace(int slot, size_t bytes) {
         e.second = want;
     }
     return e.first;
+}
+
+void Ctx::drop_graphs() {
+    for (auto& kv : graphs_cache_) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs_cache_.clear();
+}
+
+double* Ctx::fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                              long pair_stride, int n, const FsimBatchOut& out) {
+    if (!graphs_ || timing_) return fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+    GraphKey key{d_img, d_ref, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, pair_stride, w, h, ch, n};
+    auto it = graphs_cache_.find(key);
+    if (it == graphs_cache_.end()) {
+        /* one eager run sizes the workspaces and builds the per-size plans
+         * (both allocate and synchronise, which a capture may not) */
+        fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+        hipGraph_t g = nullptr;
+        IQM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+        double* res = nullptr;
+        try {
+            res = fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+        } catch (...) {
+            capturing_ = false;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        capturing_ = false;
+        IQM_HIP(hipStreamEndCapture(stream_, &g));
+        GraphEntry e;
+        e.res = res;
+        const hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        IQM_HIP(ie);
+        it = graphs_cache_.emplace(key, e).first;
+        return res; /* the capture ran nothing: the eager run above produced it */
+    }
+    IQM_HIP(hipGraphLaunch(it->second.exec, stream_));
+    return it->second.res;
 }
 
 void Ctx::sync() {
@@ -342,7 +392,7 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
         iqm::Ctx& c = *ctx->c;
         iqm::FsimBatchOut out;
         out.d_simc = d_simc_maps;
-        double* dres = c.fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+        double* dres = c.fsim_batch_graph(d_img, d_ref, w, h, ch, pair_stride, n, out);
         if (fsim_out || fsimc_out) {
             std::vector<double> res((size_t)8 * n);
             IQM_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
@@ -411,6 +461,11 @@ void* iqm_stream(iqm_ctx* ctx) { return ctx ? (void*)ctx->c->stream() : nullptr;
 int iqm_sync(iqm_ctx* ctx) {
     if (!ctx) return fail(IQM_ERR_INVALID, "null context");
     return guarded([&] { ctx->c->sync(); });
+}
+
+int iqm_set_graphs(iqm_ctx* ctx, int32_t on) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] { ctx->c->set_graphs(on != 0); });
 }
 
 int iqm_set_stage_timing(iqm_ctx* ctx, int32_t on) {

@@ -50,6 +50,15 @@ __host__ __device__ __forceinline__ float2 cmul_si(float2 a) {
 }
 
 constexpr int kBlock = 256;          // default workgroup size (4 wavefronts)
+constexpr int kBlockFft = 512;       // workgroup size of the LDS line-transform kernels
+
+/* Workgroup barrier that orders LDS only: waits for this wave's LDS
+ * operations, then joins the barrier.  Global stores issued before it stay in
+ * flight (a __syncthreads() would wait for them).  Use only where the
+ * barrier protects LDS data. */
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 constexpr int kMaxLdsBytes = 65536;  // per-workgroup LDS budget used by every kernel
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

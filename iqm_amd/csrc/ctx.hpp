@@ -7,6 +7,7 @@
 #include "fft.hpp"
 
 #include <map>
+#include <tuple>
 #include <string>
 #include <utility>
 #include <vector>
@@ -15,7 +16,7 @@ namespace iqm {
 
 struct LinePlanHost {
     LinePlan plan{};
-    float2* dev_tw = nullptr;
+    std::vector<float2*> owned; /* device tables of the plan */
 };
 
 /* Per-(wd, hd) constants of FSIM: the filter planes and the image-independent
@@ -25,6 +26,7 @@ struct FsimSizePlan {
     long N = 0;
     float* lg = nullptr;   /* 4 planes: log-Gabor x lowpass per scale (F6-F7) */
     float* ang = nullptr;  /* 4 planes: angular spread per orientation (F8)  */
+    float* bank = nullptr; /* 16 planes H_z = LG_s * A_o, z = 4 o + s (F9)  */
     double em_n[4] = {0, 0, 0, 0};
     double est[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -50,6 +52,9 @@ class Ctx {
     Ctx& operator=(const Ctx&) = delete;
 
     int device() const { return device_; }
+    /* LDS bytes one workgroup may allocate on this device */
+    size_t lds_per_block() const { return lds_per_block_; }
+    int xcds() const { return xcds_; }
     hipStream_t stream() const { return stream_; }
     const std::string& name() const { return name_; }
 
@@ -103,6 +108,13 @@ class Ctx {
         }
     }
 
+    /* Replays the launch sequence of a repeated fsim_batch call (same
+     * arguments) as a captured hipGraph.  Returns the result array. */
+    double* fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                             long pair_stride, int n, const FsimBatchOut& out);
+    void set_graphs(bool on) { graphs_ = on; }
+    bool graphs() const { return graphs_; }
+
     void set_timing(bool on);
     int stage_times(int max, const char** names, double* ms, int64_t* launches);
     void reset_times();
@@ -112,12 +124,33 @@ class Ctx {
     void collect_timing();
 
     int device_ = 0;
+    size_t lds_per_block_ = 65536;
+    int xcds_ = 8;
     hipStream_t stream_ = nullptr;
     std::string name_;
     std::map<int, std::pair<void*, size_t>> ws_;
     std::map<int, LinePlanHost> line_plans_;
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
     char* fsim_ws_ = nullptr;
+    int last_chunk_ = 1;
+
+    struct GraphKey {
+        const void *a, *b, *o1, *o2, *o3, *o4;
+        long stride;
+        int w, h, ch, n;
+        bool operator<(const GraphKey& k) const {
+            return std::tie(a, b, o1, o2, o3, o4, stride, w, h, ch, n) <
+                   std::tie(k.a, k.b, k.o1, k.o2, k.o3, k.o4, k.stride, k.w, k.h, k.ch, k.n);
+        }
+    };
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        double* res = nullptr;
+    };
+    std::map<GraphKey, GraphEntry> graphs_cache_;
+    void drop_graphs();
+    bool graphs_ = true;
+    bool capturing_ = false;
 
     bool timing_ = false;
     std::vector<hipEvent_t> free_events_;
@@ -129,6 +162,7 @@ class Ctx {
 /* workspace slots */
 enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP };
 
-int rows_per_block(int n);
+size_t line_lds_bytes(const LinePlan& P, int lines, int bufs);
+int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap);
 
 }  // namespace iqm

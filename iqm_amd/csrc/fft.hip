@@ -5,32 +5,147 @@
 #include "ctx.hpp"
 
 #include <cmath>
+#include <complex>
 
 namespace iqm {
 
-bool plan_radices(int n, std::vector<int>& rad) {
-    rad.clear();
-    if (n < 1) return false;
-    int r = n;
-    while (r % 8 == 0) { rad.push_back(8); r /= 8; }
-    while (r % 4 == 0) { rad.push_back(4); r /= 4; }
-    while (r % 2 == 0) { rad.push_back(2); r /= 2; }
-    for (int p : {3, 5, 7, 11, 13})
-        while (r % p == 0) { rad.push_back(p); r /= p; }
-    if (r != 1 || (int)rad.size() > kMaxStages) return false;
-    return true;
+namespace {
+
+/* iterative radix-2 FFT in double (plan-time only), sign -1 */
+void host_fft_pow2(std::vector<std::complex<double>>& a) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t k = 0; k < len / 2; ++k) {
+                const double ang = -two_pi * (double)k / (double)len;
+                const std::complex<double> w(std::cos(ang), std::sin(ang));
+                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                a[i + k] = u + v;
+                a[i + k + len / 2] = u - v;
+            }
+        }
+    }
 }
 
-/* In-place transform of rows (length w, contiguous) of `planes` planes;
- * each workgroup handles `lines` consecutive rows of one plane. */
+float2* upload_table(const std::vector<float2>& v) {
+    float2* d = nullptr;
+    IQM_HIP(hipMalloc(&d, sizeof(float2) * v.size()));
+    IQM_HIP(hipMemcpy(d, v.data(), sizeof(float2) * v.size(), hipMemcpyHostToDevice));
+    return d;
+}
+
+std::vector<float2> twiddle_table(int n) {
+    std::vector<float2> tw(n);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int j = 0; j < n; ++j) {
+        const double a = two_pi * (double)j / (double)n;
+        tw[j] = make_float2((float)std::cos(a), (float)(-std::sin(a)));
+    }
+    return tw;
+}
+
+}  // namespace
+
+const LinePlanHost& Ctx::line_plan(int n) {
+    auto it = line_plans_.find(n);
+    if (it != line_plans_.end()) return it->second;
+    IQM_CHECK(n >= 1 && n <= 4096, IQM_ERR_UNSUPPORTED,
+              "FFT length " + std::to_string(n) + " outside [1, 4096]");
+    LinePlanHost lp;
+    LinePlan& P = lp.plan;
+    P = LinePlan{};
+    P.n = n;
+    std::vector<int> rad;
+    if (plan_radices(n, rad)) {
+        P.nst = (int)rad.size();
+        for (int i = 0; i < kMaxStages; ++i) P.rad[i] = i < P.nst ? rad[i] : 1;
+        float2* tw = upload_table(twiddle_table(n));
+        P.tw = tw;
+        P.lds_len = n;
+        lp.owned = {tw};
+    } else {
+        int m = 1;
+        while (m < 2 * n - 1) m <<= 1;
+        std::vector<int> mr;
+        IQM_CHECK(plan_radices(m, mr), IQM_ERR_UNSUPPORTED, "Bluestein length not plannable");
+        P.blue = 1;
+        P.m = m;
+        P.mst = (int)mr.size();
+        for (int i = 0; i < kMaxStages; ++i) P.mrad[i] = i < P.mst ? mr[i] : 1;
+        const double pi = 3.14159265358979323846;
+        std::vector<float2> chirp(n);
+        std::vector<std::complex<double>> cd(n);
+        for (int j = 0; j < n; ++j) {
+            const long long q = ((long long)j * j) % (2LL * n);
+            const double a = pi * (double)q / (double)n;
+            cd[j] = std::complex<double>(std::cos(a), -std::sin(a));
+            chirp[j] = make_float2((float)cd[j].real(), (float)cd[j].imag());
+        }
+        /* forward kernel g_d = conj(c_d), inverse kernel g_d = c_d, d in (-n, n) */
+        std::vector<std::complex<double>> gf(m, 0.0), gi(m, 0.0);
+        for (int d = 0; d < n; ++d) {
+            gf[d] = std::conj(cd[d]);
+            gi[d] = cd[d];
+            if (d) {
+                gf[m - d] = gf[d];
+                gi[m - d] = gi[d];
+            }
+        }
+        host_fft_pow2(gf);
+        host_fft_pow2(gi);
+        std::vector<float2> gff(m), gif(m);
+        for (int j = 0; j < m; ++j) {
+            gff[j] = make_float2((float)gf[j].real(), (float)gf[j].imag());
+            gif[j] = make_float2((float)gi[j].real(), (float)gi[j].imag());
+        }
+        float2* mtw = upload_table(twiddle_table(m));
+        float2* ch = upload_table(chirp);
+        float2* g1 = upload_table(gff);
+        float2* g2 = upload_table(gif);
+        P.mtw = mtw;
+        P.chirp = ch;
+        P.gf = g1;
+        P.gi = g2;
+        P.lds_len = m;
+        lp.owned = {mtw, ch, g1, g2};
+    }
+    return line_plans_.emplace(n, lp).first->second;
+}
+
+/* LDS bytes of a kernel holding `bufs` buffers of `lines` lines of plan P,
+ * plus the staged twiddles. */
+size_t line_lds_bytes(const LinePlan& P, int lines, int bufs) {
+    return (size_t)bufs * lines * lds_pitch(P.lds_len) * sizeof(float2) +
+           (size_t)lds_twiddles(P) * sizeof(float2);
+}
+
+/* Largest line count in [1, cap] whose LDS footprint fits the budget; 0 if
+ * even one line does not fit. */
+int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap) {
+    int best = 0;
+    for (int l = 1; l <= cap; ++l)
+        if (line_lds_bytes(P, l, bufs) <= budget) best = l;
+    return best;
+}
+
+/* In-place transform of rows (length w, contiguous) of every plane; each
+ * workgroup handles `lines` consecutive rows of one plane. */
 template <int S>
 __global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2* data, int w, int h,
                                                           long plane_stride, int lines,
                                                           LinePlan P, float scale) {
     extern __shared__ float2 lds[];
-    const int ld = lds_pitch(w);
+    const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
     float2* b1 = lds + lines * ld;
+    const float2* tw = stage_twiddles(P, lds + 2 * lines * ld);
     float2* base = data + (long)blockIdx.y * plane_stride;
     const int y0 = blockIdx.x * lines;
     const int nl = min(lines, h - y0);
@@ -39,23 +154,24 @@ __global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2* data, int w, i
         b0[l * ld + x] = base[(long)(y0 + l) * w + x];
     }
     __syncthreads();
-    float2* r = lds_fft<S>(b0, b1, ld, nl, P);
+    float2* r = line_fft<S>(b0, b1, ld, nl, P, tw);
     for (int idx = threadIdx.x; idx < nl * w; idx += blockDim.x) {
         int l = idx / w, x = idx - l * w;
         base[(long)(y0 + l) * w + x] = cscale(r[l * ld + x], scale);
     }
 }
 
-/* In-place transform of columns (length h, stride w) of `planes` planes;
- * each workgroup handles `cols` adjacent columns. */
+/* In-place transform of columns (length h, stride w) of every plane; each
+ * workgroup handles `cols` adjacent columns. */
 template <int S>
 __global__ __launch_bounds__(kBlock) void fft_cols_kernel(float2* data, int w, int h,
                                                           long plane_stride, int cols,
                                                           LinePlan P, float scale) {
     extern __shared__ float2 lds[];
-    const int ld = lds_pitch(h);
+    const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
     float2* b1 = lds + cols * ld;
+    const float2* tw = stage_twiddles(P, lds + 2 * cols * ld);
     float2* base = data + (long)blockIdx.y * plane_stride;
     const int x0 = blockIdx.x * cols;
     const int nc = min(cols, w - x0);
@@ -64,52 +180,23 @@ __global__ __launch_bounds__(kBlock) void fft_cols_kernel(float2* data, int w, i
         b0[c * ld + y] = base[(long)y * w + x0 + c];
     }
     __syncthreads();
-    float2* r = lds_fft<S>(b0, b1, ld, nc, P);
+    float2* r = line_fft<S>(b0, b1, ld, nc, P, tw);
     for (int idx = threadIdx.x; idx < nc * h; idx += blockDim.x) {
         int y = idx / nc, c = idx - y * nc;
         base[(long)y * w + x0 + c] = cscale(r[c * ld + y], scale);
     }
 }
 
-const LinePlanHost& Ctx::line_plan(int n) {
-    auto it = line_plans_.find(n);
-    if (it != line_plans_.end()) return it->second;
-    LinePlanHost lp;
-    std::vector<int> rad;
-    IQM_CHECK(plan_radices(n, rad), IQM_ERR_UNSUPPORTED,
-              "FFT length " + std::to_string(n) + " has a prime factor above 13");
-    std::vector<float2> tw(n);
-    const double two_pi = 6.283185307179586476925286766559;
-    for (int j = 0; j < n; ++j) {
-        double a = two_pi * (double)j / (double)n;
-        tw[j] = make_float2((float)std::cos(a), (float)(-std::sin(a)));
-    }
-    float2* d = nullptr;
-    IQM_HIP(hipMalloc(&d, sizeof(float2) * n));
-    IQM_HIP(hipMemcpy(d, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    lp.dev_tw = d;
-    lp.plan.n = n;
-    lp.plan.nst = (int)rad.size();
-    for (int i = 0; i < kMaxStages; ++i) lp.plan.rad[i] = i < (int)rad.size() ? rad[i] : 1;
-    lp.plan.tw = d;
-    return line_plans_.emplace(n, lp).first->second;
-}
-
-int rows_per_block(int n) {
-    int per = kMaxLdsBytes / (2 * lds_pitch(n) * (int)sizeof(float2));
-    return per < 1 ? 0 : (per > 8 ? 8 : per);
-}
-
 void Ctx::fft2d_dev(float2* d, int w, int h, int planes, int dir, bool normalize) {
     const LinePlan& px = line_plan(w).plan;
     const LinePlan& py = line_plan(h).plan;
-    const int lr = rows_per_block(w), lc = rows_per_block(h);
+    const size_t budget = 65536;
+    const int lr = lines_that_fit(px, 2, budget, 8), lc = lines_that_fit(py, 2, budget, 8);
     IQM_CHECK(lr > 0 && lc > 0, IQM_ERR_UNSUPPORTED, "FFT line too long for LDS");
     const long ps = (long)w * h;
     const float s = normalize ? (float)(1.0 / ((double)w * (double)h)) : 1.0f;
     dim3 gr(ceil_div(h, lr), planes), gc(ceil_div(w, lc), planes);
-    size_t shr = 2 * lr * lds_pitch(w) * sizeof(float2);
-    size_t shc = 2 * lc * lds_pitch(h) * sizeof(float2);
+    const size_t shr = line_lds_bytes(px, lr, 2), shc = line_lds_bytes(py, lc, 2);
     if (dir < 0) {
         launch("fft_rows_fwd", fft_rows_kernel<-1>, gr, dim3(kBlock), shr, d, w, h, ps, lr, px, 1.0f);
         launch("fft_cols_fwd", fft_cols_kernel<-1>, gc, dim3(kBlock), shc, d, w, h, ps, lc, py, s);

@@ -1,16 +1,18 @@
 // Line FFTs in LDS: the project's replacement of VkFFT
 // (reference call sites src/gpu/fsim/fsim.cpp:370-437 and :527-550).
 //
-// A workgroup transforms `nlines` lines of length n held in LDS with a
-// Stockham autosort schedule: stage j of radix r reads element i + t*(n/r)
-// of the source buffer, applies the twiddle exp(dir*2*pi*i*t*k/(r*p)) with
-// k = i mod p (p = product of the earlier radices), runs a radix-r DFT and
-// writes element (i/p)*r*p + k + t*p of the destination buffer.  Buffers
-// ping-pong between stages and the result ends in natural order.
+// A workgroup transforms `nlines` lines held in LDS with a Stockham autosort
+// schedule: stage j of radix r reads element i + t*(n/r) of the source
+// buffer, applies the twiddle exp(S*2*pi*i*t*k/(r*p)) with k = i mod p
+// (p = product of the earlier radices), runs a radix-r DFT and writes
+// element (i/p)*r*p + k + t*p of the destination buffer.  Buffers ping-pong
+// between stages and the result ends in natural order.
 //
-// Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies; a length
-// whose prime factors are all <= 13 is planned directly.  Twiddles are
-// computed in double on the host and stored as float.
+// Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies.  A
+// length with a larger prime factor goes through Bluestein's chirp-z
+// identity X_k = c_k * sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(S*pi*i*j^2/n),
+// evaluated as a circular convolution of power-of-two length m >= 2n-1.
+// Twiddles and chirps are computed in double on the host and stored as float.
 #pragma once
 
 #include "common.hpp"
@@ -20,13 +22,23 @@
 namespace iqm {
 
 constexpr int kMaxStages = 16;
+constexpr int kMaxLdsTwiddles = 4096;
 
 /* Kernel-argument view of a line plan (passed by value). */
 struct LinePlan {
-    int n;
-    int nst;
-    int rad[kMaxStages];
-    const float2* tw; /* exp(-2 pi i j / n), j in [0, n) */
+    int n;                 /* transform length                                  */
+    int nst;               /* direct plan: number of stages                     */
+    int rad[kMaxStages];   /* direct plan: radices in application order         */
+    const float2* tw;      /* direct plan: exp(-2 pi i j / n), j < n            */
+    int blue;              /* 1: Bluestein                                      */
+    int m;                 /* Bluestein convolution length (power of two)       */
+    int mst;               /* stages of the length-m plan                       */
+    int mrad[kMaxStages];
+    const float2* mtw;     /* exp(-2 pi i j / m), j < m                         */
+    const float2* chirp;   /* exp(-pi i (j^2 mod 2n) / n), j < n                */
+    const float2* gf;      /* FFT_m of the forward convolution kernel           */
+    const float2* gi;      /* FFT_m of the inverse convolution kernel           */
+    int lds_len;           /* elements a line occupies in LDS (n or m)          */
 };
 
 /* ---- small-radix DFTs (y_t = sum_q x_q exp(S*2*pi*i*q*t/R)) -------------- */
@@ -137,14 +149,14 @@ __host__ __device__ __forceinline__ void dft4(float2* x) {
 
 template <int S>
 __host__ __device__ __forceinline__ void dft8(float2* x) {
-    /* two radix-4 DFTs on the even and odd samples, then a radix-2 combine */
+    /* radix-4 DFTs of the even and odd samples, then a radix-2 combine with
+     * w8^k = exp(S*2*pi*i*k/8) */
     float2 e[4] = {x[0], x[2], x[4], x[6]};
     float2 o[4] = {x[1], x[3], x[5], x[7]};
     dft4<S>(e);
     dft4<S>(o);
     const float h = 0.70710678118654752440f;
-    /* w8^k = exp(S*2*pi*i*k/8) */
-    float2 w1 = make_float2(h, S * h), w3 = make_float2(-h, S * h);
+    const float2 w1 = make_float2(h, S * h), w3 = make_float2(-h, S * h);
     o[1] = cmul(o[1], w1);
     o[2] = cmul_si<S>(o[2]);
     o[3] = cmul(o[3], w3);
@@ -163,7 +175,17 @@ __host__ __device__ __forceinline__ void dft_r(float2* x) {
     else dft_odd<R, S>(x);
 }
 
-/* One Stockham stage over nlines lines (line l at src + l*ld). */
+/* q = a / d for 0 <= a < 2^22 via a float reciprocal and one correction. */
+__host__ __device__ __forceinline__ int fast_div(int a, int d, float inv) {
+    int q = (int)((float)a * inv);
+    const int r = a - q * d;
+    if (r >= d) ++q;
+    else if (r < 0) --q;
+    return q;
+}
+
+/* One Stockham stage over nlines lines (line l at src + l*ld); tw holds
+ * exp(-2 pi i j / n). */
 template <int R, int S>
 __host__ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int ld,
                                                         int nlines, int n, int p,
@@ -171,10 +193,11 @@ __host__ __device__ __forceinline__ void stockham_stage(const float2* src, float
     const int T = n / R;
     const int total = nlines * T;
     const int tws = n / (R * p);
+    const float invT = 1.0f / (float)T, invp = 1.0f / (float)p;
     for (int idx = tid; idx < total; idx += nthr) {
-        const int line = idx / T;
+        const int line = fast_div(idx, T, invT);
         const int i = idx - line * T;
-        const int k = i % p;
+        const int k = i - fast_div(i, p, invp) * p;
         const float2* s = src + line * ld;
         float2 x[R];
 #pragma unroll
@@ -211,18 +234,19 @@ __host__ __device__ __forceinline__ void stockham_dispatch(int r, const float2* 
     }
 }
 
-/* Transforms nlines lines starting in buf0; returns the buffer holding the
- * result (buf0 or buf1).  Ends with a workgroup barrier. */
+/* Direct mixed-radix transform; returns the buffer holding the result.
+ * Ends with a workgroup barrier. */
 template <int S>
-__device__ __forceinline__ float2* lds_fft(float2* buf0, float2* buf1, int ld, int nlines,
-                                           const LinePlan& P) {
+__device__ __forceinline__ float2* lds_fft_direct(float2* buf0, float2* buf1, int ld, int nlines,
+                                                  int n, int nst, const int* rad,
+                                                  const float2* tw) {
     float2* src = buf0;
     float2* dst = buf1;
     int p = 1;
-    for (int st = 0; st < P.nst; ++st) {
-        const int r = P.rad[st];
-        stockham_dispatch<S>(r, src, dst, ld, nlines, P.n, p, P.tw, threadIdx.x, blockDim.x);
-        __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int r = rad[st];
+        stockham_dispatch<S>(r, src, dst, ld, nlines, n, p, tw, threadIdx.x, blockDim.x);
+        lds_barrier();
         float2* t = src;
         src = dst;
         dst = t;
@@ -231,12 +255,75 @@ __device__ __forceinline__ float2* lds_fft(float2* buf0, float2* buf1, int ld, i
     return src;
 }
 
+/* Number of float2 twiddles a kernel stages in LDS for this plan (0 when the
+ * plan reads them from global memory). */
+__host__ __device__ __forceinline__ int lds_twiddles(const LinePlan& P) {
+    return (!P.blue && P.n <= kMaxLdsTwiddles) ? P.n : 0;
+}
+
+/* Copies the direct plan's twiddles to LDS (when it fits) and returns the
+ * table the transforms should read.  The caller must barrier before use. */
+__device__ __forceinline__ const float2* stage_twiddles(const LinePlan& P, float2* tw_lds) {
+    if (lds_twiddles(P) == 0) return P.blue ? P.mtw : P.tw;
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) tw_lds[i] = P.tw[i];
+    return tw_lds;
+}
+
+/* Full line transform of nlines lines whose first n elements hold the input;
+ * lines are ld >= P.lds_len apart.  Returns the buffer holding the result
+ * (first n elements of each line).  Starts and ends at a barrier. */
+template <int S>
+__device__ float2* line_fft(float2* b0, float2* b1, int ld, int nlines, const LinePlan& P,
+                            const float2* tw) {
+    if (!P.blue) return lds_fft_direct<S>(b0, b1, ld, nlines, P.n, P.nst, P.rad, tw);
+    const int n = P.n, m = P.m;
+    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
+        const int l = idx / m, j = idx - l * m;
+        float2 v = make_float2(0.f, 0.f);
+        if (j < n) {
+            float2 c = P.chirp[j];
+            if (S > 0) c.y = -c.y;
+            v = cmul(b0[l * ld + j], c);
+        }
+        b0[l * ld + j] = v;
+    }
+    lds_barrier();
+    float2* r = lds_fft_direct<-1>(b0, b1, ld, nlines, m, P.mst, P.mrad, P.mtw);
+    float2* other = (r == b0) ? b1 : b0;
+    const float2* g = (S < 0) ? P.gf : P.gi;
+    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
+        const int l = idx / m, j = idx - l * m;
+        r[l * ld + j] = cmul(r[l * ld + j], g[j]);
+    }
+    lds_barrier();
+    float2* q = lds_fft_direct<1>(r, other, ld, nlines, m, P.mst, P.mrad, P.mtw);
+    const float inv_m = 1.0f / (float)m;
+    for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
+        const int l = idx / n, k = idx - l * n;
+        float2 c = P.chirp[k];
+        if (S > 0) c.y = -c.y;
+        q[l * ld + k] = cscale(cmul(q[l * ld + k], c), inv_m);
+    }
+    lds_barrier();
+    return q;
+}
+
 /* Host-side planner: radices for n (largest power-of-two radices first,
  * then the odd primes); false when n has a prime factor above 13. */
-bool plan_radices(int n, std::vector<int>& rad);
+inline bool plan_radices(int n, std::vector<int>& rad) {
+    rad.clear();
+    if (n < 1) return false;
+    int r = n;
+    while (r % 8 == 0) { rad.push_back(8); r /= 8; }
+    while (r % 4 == 0) { rad.push_back(4); r /= 4; }
+    while (r % 2 == 0) { rad.push_back(2); r /= 2; }
+    for (int p : {3, 5, 7, 11, 13})
+        while (r % p == 0) { rad.push_back(p); r /= p; }
+    return r == 1 && (int)rad.size() <= kMaxStages;
+}
 
-/* LDS row pitch (in float2) used for a line of length n: odd, so that the
- * transposed loads of the column kernels spread over the LDS banks. */
-__host__ __device__ __forceinline__ int lds_pitch(int n) { return (n % 2 == 0) ? n + 1 : n; }
+/* LDS row pitch (in float2) for a line occupying len elements: odd, so that
+ * the transposed accesses of the column kernels spread over the LDS banks. */
+__host__ __device__ __forceinline__ int lds_pitch(int len) { return (len % 2 == 0) ? len + 1 : len; }
 
 }  // namespace iqm

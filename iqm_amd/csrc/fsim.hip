@@ -18,8 +18,11 @@
 // once per (wd, hd) by Ctx::fsim_plan and cached.
 #include "ctx.hpp"
 #include "fft.hpp"
+#include "fsim_tl.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace iqm {
@@ -33,8 +36,8 @@ __device__ __forceinline__ float freq_coord(int x, int n) {
     return (float)((int)shifted - n / 2) / (float)n;
 }
 
-__global__ __launch_bounds__(kBlock) void fsim_tables_kernel(float* lg, float* ang, int wd,
-                                                             int hd) {
+__global__ __launch_bounds__(kBlock) void fsim_tables_kernel(float* lg, float* ang, float* bank,
+                                                             int wd, int hd) {
     const long N = (long)wd * hd;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= N) return;
@@ -65,6 +68,8 @@ __global__ __launch_bounds__(kBlock) void fsim_tables_kernel(float* lg, float* a
         const float dtheta = fabsf(atan2f(ds, dc));
         ang[o * N + idx] = expf(-(dtheta * dtheta) / (2.0f * (thetaSigma * thetaSigma)));
     }
+    /* fsim_filter_combinations.glsl:51: H_z = gabor * angular, z = 4 o + s */
+    for (int z = 0; z < 16; ++z) bank[z * N + idx] = lg[(z & 3) * N + idx] * ang[(z >> 2) * N + idx];
 }
 
 /* Partial sums for EM_n[o] = sum_k (LG_0 * A_o)^2 over every bin (F10). */
@@ -143,8 +148,9 @@ const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
     const long N = P.N;
     IQM_HIP(hipMalloc(&P.lg, sizeof(float) * 4 * N));
     IQM_HIP(hipMalloc(&P.ang, sizeof(float) * 4 * N));
+    IQM_HIP(hipMalloc(&P.bank, sizeof(float) * 16 * N));
     launch("fsim_tables", fsim_tables_kernel, dim3(ceil_div(N, kBlock)), dim3(kBlock), 0, P.lg,
-           P.ang, wd, hd);
+           P.ang, P.bank, wd, hd);
 
     const int nb = 64;
     double* partial = (double*)workspace(WS_PLAN_TMP, sizeof(double) * 8 * nb + sizeof(float2) * 16 * N + 256);
@@ -178,36 +184,69 @@ const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
 
 constexpr int kHistBins = 2048;
 
-/* F2: fsim_downsample.glsl:38-54.  yiq planes per pair: Y_in, Y_ref, I_in,
- * Q_in, I_ref, Q_ref.  Block (0,0) of each pair also clears the pair's
- * median histograms. */
+/* F2: fsim_downsample.glsl:38-54.  A workgroup produces kPrepCols
+ * downscaled pixels of one row for both images: the bytes of the F
+ * full-resolution rows they cover are staged in LDS as 16-byte aligned
+ * chunks (an aligned 16-byte block that holds a byte of the image lies in
+ * that byte's page, so these loads never leave mapped memory), then each
+ * thread sums its F x F box (zero outside the image, divided by F^2).  yiq
+ * planes per pair: Y_in, Y_ref, I_in, Q_in, I_ref, Q_ref.  Workgroup (0, 0)
+ * of each pair also clears the pair's median histograms. */
+constexpr int kPrepCols = 256;
+
+/* 16-byte chunks staged per full-resolution row for a segment of seg bytes */
+__host__ __device__ __forceinline__ int prep_row_chunks(int seg) { return seg / 16 + 2; }
+
 __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
                                                            const uint8_t* __restrict__ ref,
                                                            long pair_stride, int w, int h,
                                                            int ch, int F, int wd, int hd,
                                                            float* __restrict__ yiq,
                                                            unsigned* __restrict__ hist) {
+    extern __shared__ uint4 stage4[];
+    const uint8_t* stage = (const uint8_t*)stage4;
     const int pair = blockIdx.z;
+    const int tid = threadIdx.x;
     if (blockIdx.x == 0 && blockIdx.y == 0) {
         unsigned* hp = hist + (long)pair * 8 * kHistBins;
-        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-        for (int i = tid; i < 8 * kHistBins; i += blockDim.x * blockDim.y) hp[i] = 0u;
+        for (int i = tid; i < 8 * kHistBins; i += blockDim.x) hp[i] = 0u;
     }
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= wd || y >= hd) return;
+    const int y = blockIdx.y;
+    const int xd0 = blockIdx.x * kPrepCols;
+    const int ncols = min(kPrepCols, wd - xd0);
+    /* full-resolution window of this block, clipped to the image */
+    const int c_lo = xd0 * F - F / 2, c_hi = (xd0 + ncols) * F - F / 2;
+    const int r_lo = y * F - F / 2;
+    const int cc0 = max(c_lo, 0), cc1 = min(c_hi, w);
+    const int rr0 = max(r_lo, 0), rr1 = min(r_lo + F, h);
+    const int seg = max(cc1 - cc0, 0) * ch; /* bytes of each staged row */
+    const int nrows = max(rr1 - rr0, 0);
+    const int rc = prep_row_chunks(seg);     /* chunks per staged row */
     const long N = (long)wd * hd;
-    const long idx = (long)y * wd + x;
+    float* base = yiq + (long)pair * 6 * N;
     const float scaler = (float)F * (float)F;
-    const int xs = x * F - F / 2, ys = y * F - F / 2;
-    float out[2][3];
+    const int x = xd0 + tid;
     for (int im = 0; im < 2; ++im) {
         const uint8_t* src = (im ? ref : img) + (long)pair * pair_stride;
-        float si = 0.f, sq = 0.f, sy = 0.f;
-        for (int j = xs; j < xs + F; ++j) {
-            for (int k = ys; k < ys + F; ++k) {
-                if (j >= 0 && j < w && k >= 0 && k < h) {
-                    const uint8_t* p = src + ((long)k * w + j) * ch;
+        for (int i = tid; i < nrows * rc; i += blockDim.x) {
+            const int r = i / rc, k = i - r * rc;
+            const uintptr_t a0 = (uintptr_t)(src + ((long)(rr0 + r) * w + cc0) * ch);
+            const uintptr_t first = a0 & ~(uintptr_t)15;
+            const uintptr_t last = (a0 + seg - 1) & ~(uintptr_t)15;
+            const uintptr_t a = first + (uintptr_t)k * 16;
+            if (a <= last) stage4[i] = *(const uint4*)a;
+        }
+        __syncthreads();
+        if (tid < ncols) {
+            float si = 0.f, sq = 0.f, sy = 0.f;
+            const int xs = x * F - F / 2;
+            for (int j = xs; j < xs + F; ++j) {
+                if (j < cc0 || j >= cc1) continue;
+                for (int k = r_lo; k < r_lo + F; ++k) {
+                    if (k < rr0 || k >= rr1) continue;
+                    const int r = k - rr0;
+                    const uintptr_t a0 = (uintptr_t)(src + ((long)k * w + cc0) * ch);
+                    const uint8_t* p = stage + r * rc * 16 + (int)(a0 & 15) + (j - cc0) * ch;
                     float R, G, B;
                     if (ch == 1) { R = G = B = (float)p[0]; }
                     else { R = (float)p[0]; G = (float)p[1]; B = (float)p[2]; }
@@ -216,36 +255,32 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
                     sy += R * 0.299f + G * 0.587f + B * 0.114f;
                 }
             }
+            const long idx = (long)y * wd + x;
+            base[(im ? 1 : 0) * N + idx] = sy / scaler;
+            base[(im ? 4 : 2) * N + idx] = si / scaler;
+            base[(im ? 5 : 3) * N + idx] = sq / scaler;
         }
-        out[im][0] = si / scaler;
-        out[im][1] = sq / scaler;
-        out[im][2] = sy / scaler;
+        __syncthreads();
     }
-    float* base = yiq + (long)pair * 6 * N;
-    base[0 * N + idx] = out[0][2];
-    base[1 * N + idx] = out[1][2];
-    base[2 * N + idx] = out[0][0];
-    base[3 * N + idx] = out[0][1];
-    base[4 * N + idx] = out[1][0];
-    base[5 * N + idx] = out[1][1];
 }
 
 /* F3 (fsim_gradientmap.glsl:38-56) and the row half of F5 on the packed
- * rows Y_in + i*Y_ref; the two real-input row spectra are split with
- * X(k) = (Z(k) + conj Z(-k)) / 2, Y(k) = (Z(k) - conj Z(-k)) / (2i). */
-template <int RB>
+ * rows Y_in + i*Y_ref of `rb` rows; the two real-input row spectra are split
+ * with X(k) = (Z(k) + conj Z(-k)) / 2, Y(k) = (Z(k) - conj Z(-k)) / (2i). */
 __global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* __restrict__ yiq,
                                                                   float* __restrict__ grad,
                                                                   float2* __restrict__ A,
-                                                                  int wd, int hd, LinePlan P) {
+                                                                  int wd, int hd, int rb,
+                                                                  LinePlan P) {
     extern __shared__ float2 lds[];
-    const int ld = lds_pitch(wd);
+    const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
-    float2* b1 = lds + RB * ld;
+    float2* b1 = lds + rb * ld;
+    const float2* tw = stage_twiddles(P, lds + 2 * rb * ld);
     const int pair = blockIdx.y;
     const long N = (long)wd * hd;
-    const int y0 = blockIdx.x * RB;
-    const int nr = min(RB, hd - y0);
+    const int y0 = blockIdx.x * rb;
+    const int nr = min(rb, hd - y0);
     const float* Yin = yiq + (long)pair * 6 * N;
     const float* Yref = Yin + N;
     const float vert[9] = {3.f, 0.f, -3.f, 10.f, 0.f, -10.f, 3.f, 0.f, -3.f};
@@ -273,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* _
         b0[r * ld + x] = make_float2(Yin[g], Yref[g]);
     }
     __syncthreads();
-    float2* R = lds_fft<-1>(b0, b1, ld, nr, P);
+    float2* R = line_fft<-1>(b0, b1, ld, nr, P, tw);
     float2* Ain = A + (long)pair * 2 * N;
     float2* Aref = Ain + N;
     for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
@@ -286,87 +321,91 @@ __global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* _
     }
 }
 
-/* Column half of F5, then F9 and the column half of F11: a workgroup owns
- * BX adjacent columns of one image and one orientation; the forward
- * column FFT gives the spectrum, each scale's filter product is inverse
- * transformed along the columns and written to plane (img*4 + o)*4 + s. */
-template <int BX>
-__global__ __launch_bounds__(kBlock) void fsim_colpass_kernel(const float2* __restrict__ A,
+/* Column half of F5, then F9 and the column half of F11.  A workgroup owns
+ * `bx` adjacent columns of one image of one pair: the forward column FFT
+ * gives the spectrum; each of the 16 filter products H_z * F is inverse
+ * transformed along the columns and written to plane img*16 + z.
+ * The flat grid is laid out (pair, img, padded column block) with the
+ * column-block count padded to a multiple of 8, so column block cb always
+ * runs on XCD cb mod 8 and each XCD's L2 holds only its slice of the bank. */
+__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* __restrict__ A,
                                                               float2* __restrict__ Tout,
-                                                              const float* __restrict__ lg,
-                                                              const float* __restrict__ ang,
-                                                              int wd, int hd, LinePlan P) {
+                                                              const float* __restrict__ bank,
+                                                              int wd, int hd, int bx, int nbx_pad,
+                                                              LinePlan P) {
     extern __shared__ float2 lds[];
-    const int ld = lds_pitch(hd);
+    const int cb = blockIdx.x % nbx_pad;
+    const int rest = blockIdx.x / nbx_pad;
+    const int img = rest & 1, pair = rest >> 1;
+    const int x0 = cb * bx;
+    if (x0 >= wd) return;
+    const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
-    float2* b1 = lds + BX * ld;
-    float2* b2 = lds + 2 * BX * ld;
-    const int io = blockIdx.y, img = io >> 2, o = io & 3;
-    const int pair = blockIdx.z;
+    float2* b1 = lds + bx * ld;
+    float2* b2 = lds + 2 * bx * ld;
+    const float2* tw = stage_twiddles(P, lds + 3 * bx * ld);
     const long N = (long)wd * hd;
-    const int x0 = blockIdx.x * BX;
-    const int nc = min(BX, wd - x0);
+    const int nc = min(bx, wd - x0);
     const float2* src = A + ((long)pair * 2 + img) * N;
     for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
         const int y = idx / nc, c = idx - y * nc;
         b0[c * ld + y] = src[(long)y * wd + x0 + c];
     }
     __syncthreads();
-    float2* Fs = lds_fft<-1>(b0, b1, ld, nc, P);
+    float2* Fs = line_fft<-1>(b0, b1, ld, nc, P, tw);
     float2* W = (Fs == b0) ? b1 : b0;
-    const float* angp = ang + (long)o * N;
-    for (int s = 0; s < 4; ++s) {
-        const float* lgp = lg + (long)s * N;
+    for (int z = 0; z < 16; ++z) {
+        const float* hz = bank + (long)z * N;
         for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
             const int y = idx / nc, c = idx - y * nc;
-            const long g = (long)y * wd + x0 + c;
-            const float hv = lgp[g] * angp[g];
-            W[c * ld + y] = cscale(Fs[c * ld + y], hv);
+            W[c * ld + y] = cscale(Fs[c * ld + y], hz[(long)y * wd + x0 + c]);
         }
-        __syncthreads();
-        float2* R = lds_fft<1>(W, b2, ld, nc, P);
-        float2* dst = Tout + ((long)pair * 32 + io * 4 + s) * N;
+        lds_barrier();
+        float2* R = line_fft<1>(W, b2, ld, nc, P, tw);
+        float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N;
         for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
             const int y = idx / nc, c = idx - y * nc;
             dst[(long)y * wd + x0 + c] = R[c * ld + y];
         }
-        __syncthreads();
+        lds_barrier(); /* the next filter overwrites W and b2 */
     }
 }
 
 /* Row half of F11 (normalised by 1/N) fused with F12
  * (fsim_sum_filter_responses.glsl:33-83) and the F13 packing
- * (fsim_pack_for_median.glsl:33-44). */
-template <int RB>
-__global__ __launch_bounds__(kBlock) void fsim_rowpass_kernel(const float2* __restrict__ Tin,
+ * (fsim_pack_for_median.glsl:33-44) for `rb` rows of one image and
+ * orientation; T planes img*16 + 4*o + s. */
+__global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* __restrict__ Tin,
                                                               float* __restrict__ an_e,
                                                               float* __restrict__ keys,
                                                               unsigned* __restrict__ hist,
-                                                              int wd, int hd, LinePlan P,
+                                                              int wd, int hd, int rb, LinePlan P,
                                                               float invN) {
     extern __shared__ float2 lds[];
-    const int ld = lds_pitch(wd);
-    const int nl = 4 * RB;
+    const int ld = lds_pitch(P.lds_len);
+    const int nl = 4 * rb;
     float2* b0 = lds;
     float2* b1 = lds + nl * ld;
+    const int ntw = lds_twiddles(P);
+    const float2* tw = stage_twiddles(P, lds + 2 * nl * ld);
     const int io = blockIdx.y;
     const int pair = blockIdx.z;
     const long N = (long)wd * hd;
-    const int y0 = blockIdx.x * RB;
-    const int nr = min(RB, hd - y0);
+    const int y0 = blockIdx.x * rb;
+    const int nr = min(rb, hd - y0);
     for (int idx = threadIdx.x; idx < nl * wd; idx += blockDim.x) {
         const int l = idx / wd, x = idx - l * wd;
-        const int s = l / RB, r = l - s * RB;
+        const int s = l / rb, r = l - s * rb;
         float2 v = make_float2(0.f, 0.f);
         if (r < nr) v = Tin[((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * wd + x];
         b0[l * ld + x] = v;
     }
     __syncthreads();
-    float2* R = lds_fft<1>(b0, b1, ld, nl, P);
+    float2* R = line_fft<1>(b0, b1, ld, nl, P, tw);
     const size_t bufbytes = (size_t)nl * ld * sizeof(float2);
     unsigned* hl = (bufbytes >= kHistBins * sizeof(unsigned))
                        ? (unsigned*)((R == b0) ? b1 : b0)
-                       : (unsigned*)(lds + 2 * nl * ld);
+                       : (unsigned*)(lds + 2 * nl * ld + ntw);
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
     __syncthreads();
     float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
@@ -378,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void fsim_rowpass_kernel(const float2* __re
         float amp = 0.f, sr = 0.f, si = 0.f;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            eo[s] = cscale(R[(s * RB + r) * ld + x], invN);
+            eo[s] = cscale(R[(s * rb + r) * ld + x], invN);
             amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
             sr += eo[s].x;
             si += eo[s].y;
@@ -637,7 +676,7 @@ namespace {
 struct FsimRegions {
     size_t yiq, grad, A, T, an_e, keys, hist, thr, stats, partial, res, total;
 };
-FsimRegions fsim_regions(long N, int n, int nblk) {
+FsimRegions fsim_regions(long N, int n, int chunk, int nblk) {
     FsimRegions r{};
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -648,7 +687,7 @@ FsimRegions fsim_regions(long N, int n, int nblk) {
     r.yiq = take(sizeof(float) * 6 * N * n);
     r.grad = take(sizeof(float) * 2 * N * n);
     r.A = take(sizeof(float2) * 2 * N * n);
-    r.T = take(sizeof(float2) * 32 * N * n);
+    r.T = take(sizeof(float2) * 32 * N * chunk);
     r.an_e = take(sizeof(float) * 16 * N * n);
     r.keys = take(sizeof(float) * 8 * N * n);
     r.hist = take(sizeof(unsigned) * 8 * kHistBins * n);
@@ -659,11 +698,35 @@ FsimRegions fsim_regions(long N, int n, int nblk) {
     r.total = off;
     return r;
 }
+
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
+/* Tile and chunk choices of one FSIM launch sequence. */
+struct FsimTiles {
+    int grad_rb, col_bx, row_rb, chunk;
+};
+
+FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long N) {
+    FsimTiles t;
+    const size_t budget = std::min<size_t>(c.lds_per_block(), 65536);
+    t.grad_rb = lines_that_fit(Px, 2, budget, 4);
+    t.col_bx = lines_that_fit(Py, 3, budget, env_int("IQM_FSIM_BX", 8));
+    t.row_rb = lines_that_fit(Px, 2, budget, 4 * env_int("IQM_FSIM_RB", 1)) / 4;
+    /* chunk of pairs whose 32 response planes stay within ~128 MB (MALL) */
+    const long chunk_mb = env_int("IQM_FSIM_CHUNK_MB", 128);
+    t.chunk = (int)std::max<long>(1, (chunk_mb << 20) / (long)(sizeof(float2) * 32 * N));
+    IQM_CHECK(t.grad_rb > 0 && t.col_bx > 0 && t.row_rb > 0, IQM_ERR_UNSUPPORTED,
+              "FSIM grid too large for the LDS line transforms");
+    return t;
+}
 }  // namespace
 
 Ctx::FsimLayout Ctx::fsim_layout(long N, int n, char* base) const {
     const int nblk = ceil_div(N, kBlock);
-    FsimRegions r = fsim_regions(N, n, nblk);
+    FsimRegions r = fsim_regions(N, n, last_chunk_, nblk);
     FsimLayout L;
     L.N = N;
     L.yiq = (float*)(base + r.yiq);
@@ -686,7 +749,10 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const LinePlan& Py = line_plan(hd).plan;
     const long N = (long)wd * hd;
     const int nblk = ceil_div(N, kBlock);
-    FsimRegions r = fsim_regions(N, n, nblk);
+    const FsimTiles t = pick_tiles(*this, Px, Py, N);
+    const int chunk = std::min(t.chunk, n);
+    last_chunk_ = chunk;
+    FsimRegions r = fsim_regions(N, n, chunk, nblk);
     char* base = (char*)workspace(WS_FSIM, r.total);
     fsim_ws_ = base;
     float* yiq = (float*)(base + r.yiq);
@@ -701,53 +767,48 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     double* partial = (double*)(base + r.partial);
     double* res = (double*)(base + r.res);
 
-    const int ldx = lds_pitch(wd), ldy = lds_pitch(hd);
     /* F2 */
-    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, 16), ceil_div(hd, 16), n),
-           dim3(16, 16), 0, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
+    {
+        const size_t stage_bytes = (size_t)F * prep_row_chunks(kPrepCols * F * ch) * 16;
+        IQM_CHECK(stage_bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+        launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, kPrepCols), hd, n), dim3(kBlock),
+               stage_bytes, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
+    }
     /* F3 + row half of F5 */
-    {
-        constexpr int RB = 4;
-        size_t sh = 2 * RB * ldx * sizeof(float2);
-        if (sh <= (size_t)kMaxLdsBytes) {
-            launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel<RB>, dim3(ceil_div(hd, RB), n),
-                   dim3(kBlock), sh, (const float*)yiq, grad, A, wd, hd, Px);
+    launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel, dim3(ceil_div(hd, t.grad_rb), n),
+           dim3(kBlock), line_lds_bytes(Px, t.grad_rb, 2), (const float*)yiq, grad, A, wd, hd,
+           t.grad_rb, Px);
+    /* per chunk: column half of F5, F9, F11, then the row half of F11 + F12 */
+    const int nbx = ceil_div(wd, t.col_bx);
+    const int nbx_pad = ceil_div(nbx, xcds_) * xcds_;
+    const float invN = (float)(1.0 / (double)N);
+    size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);
+    if ((size_t)4 * t.row_rb * lds_pitch(Px.lds_len) * sizeof(float2) < kHistBins * sizeof(unsigned))
+        row_sh += kHistBins * sizeof(unsigned);
+    const bool tl_col = tl_supported(hd) && env_int("IQM_FSIM_GENERIC", 0) == 0;
+    const bool tl_row = tl_supported(wd) && env_int("IQM_FSIM_GENERIC", 0) == 0;
+    const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 80 * 1024));
+    for (int p0 = 0; p0 < n; p0 += chunk) {
+        const int np = std::min(chunk, n - p0);
+        const float2* Ach = A + (long)p0 * 2 * N;
+        if (tl_col) {
+            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", 8), xcds_, tl_budget};
+            tl_colpass(*this, hd, a);
         } else {
-            sh = 2 * ldx * sizeof(float2);
-            IQM_CHECK(sh <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "row too long for LDS");
-            launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel<1>, dim3(hd, n), dim3(kBlock), sh,
-                   (const float*)yiq, grad, A, wd, hd, Px);
+            launch("fsim_colpass", fsim_colpass_kernel, dim3(nbx_pad * 2 * np), dim3(kBlockFft),
+                   line_lds_bytes(Py, t.col_bx, 3), Ach, T, (const float*)plan.bank, wd, hd,
+                   t.col_bx, nbx_pad, Py);
         }
-    }
-    /* column half of F5, F9, column half of F11 */
-    {
-        constexpr int BX = 8;
-        size_t sh = 3 * BX * ldy * sizeof(float2);
-        if (sh <= (size_t)kMaxLdsBytes) {
-            launch("fsim_colpass", fsim_colpass_kernel<BX>, dim3(ceil_div(wd, BX), 8, n),
-                   dim3(kBlock), sh, (const float2*)A, T, (const float*)plan.lg,
-                   (const float*)plan.ang, wd, hd, Py);
+        if (tl_row) {
+            TlRowArgs a{T, an_e + (long)p0 * 16 * N, keys + (long)p0 * 8 * N,
+                        hist + (long)p0 * 8 * kHistBins, Px.tw, hd, np, env_int("IQM_FSIM_TL_RB", 2),
+                        invN, tl_budget};
+            tl_rowpass(*this, wd, a);
         } else {
-            sh = 3 * ldy * sizeof(float2);
-            IQM_CHECK(sh <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "column too long for LDS");
-            launch("fsim_colpass", fsim_colpass_kernel<1>, dim3(wd, 8, n), dim3(kBlock), sh,
-                   (const float2*)A, T, (const float*)plan.lg, (const float*)plan.ang, wd, hd, Py);
-        }
-    }
-    /* row half of F11 + F12 + F13 packing */
-    {
-        const float invN = (float)(1.0 / (double)N);
-        auto shmem = [&](int rb) {
-            size_t buf = (size_t)4 * rb * ldx * sizeof(float2);
-            return 2 * buf + (buf >= kHistBins * sizeof(unsigned) ? 0 : kHistBins * sizeof(unsigned));
-        };
-        if (shmem(2) <= (size_t)kMaxLdsBytes) {
-            launch("fsim_rowpass", fsim_rowpass_kernel<2>, dim3(ceil_div(hd, 2), 8, n),
-                   dim3(kBlock), shmem(2), (const float2*)T, an_e, keys, hist, wd, hd, Px, invN);
-        } else {
-            IQM_CHECK(shmem(1) <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "row too long for LDS");
-            launch("fsim_rowpass", fsim_rowpass_kernel<1>, dim3(hd, 8, n), dim3(kBlock), shmem(1),
-                   (const float2*)T, an_e, keys, hist, wd, hd, Px, invN);
+            launch("fsim_rowpass", fsim_rowpass_kernel, dim3(ceil_div(hd, t.row_rb), 8, np),
+                   dim3(kBlockFft), row_sh, (const float2*)T, an_e + (long)p0 * 16 * N,
+                   keys + (long)p0 * 8 * N, hist + (long)p0 * 8 * kHistBins, wd, hd, t.row_rb, Px,
+                   invN);
         }
     }
     /* F13 + thresholds */

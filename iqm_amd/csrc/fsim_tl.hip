@@ -1,0 +1,242 @@
+// Two-level (register) versions of the FSIM column and row passes for line
+// lengths with a compile-time split n = N1 * N2 (fft_reg.hpp).  Each line
+// transform is two in-register DFTs with one LDS exchange, instead of one
+// LDS round trip per radix stage.  Same inputs, outputs and plane layout as
+// fsim_colpass_kernel / fsim_rowpass_kernel in fsim.hip.
+#include "ctx.hpp"
+#include "fft_reg.hpp"
+#include "fsim_tl.hpp"
+
+namespace iqm {
+
+constexpr int kHistBinsTl = 2048;
+
+/* Column pass: forward column FFT of `bx` columns of one image, then for
+ * each of the 16 filters z the product H_z * F inverse transformed along the
+ * columns and stored to plane img*16 + z.  Grid as fsim_colpass_kernel
+ * (column blocks padded to a multiple of 8 for the XCD mapping). */
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void fsim_colpass_tl_kernel(const float2* __restrict__ A,
+                                                               float2* __restrict__ Tout,
+                                                               const float* __restrict__ bank,
+                                                               const float2* __restrict__ twg,
+                                                               int wd, int bx, int nbx_pad) {
+    constexpr int n = N1 * N2;
+    constexpr int ZL = Exch<N1, N2>::line;
+    constexpr int LS = n + 1; /* spectrum line pitch (odd) */
+    extern __shared__ float2 lds[];
+    const int cb = blockIdx.x % nbx_pad;
+    const int rest = blockIdx.x / nbx_pad;
+    const int img = rest & 1, pair = rest >> 1;
+    const int x0 = cb * bx;
+    if (x0 >= wd) return;
+    const int nc = min(bx, wd - x0);
+    float2* S = lds;
+    float2* Z = S + bx * LS;
+    float2* tw = Z + bx * ZL;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    const long N = (long)wd * n;
+    const float2* src = A + ((long)pair * 2 + img) * N + x0;
+    __syncthreads();
+    /* forward: step 1, items (c, j2), c fastest */
+    for (int it = threadIdx.x; it < bx * N2; it += blockDim.x) {
+        const int c = it % bx, j2 = it / bx;
+        float2 x[N1];
+        if (c < nc) {
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = src[(long)(N2 * j1 + j2) * wd + c];
+        } else {
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
+        }
+        two_level_step1<N1, N2, -1>(x, j2, tw, Z + c * ZL);
+    }
+    lds_barrier();
+    for (int it = threadIdx.x; it < bx * N1; it += blockDim.x) {
+        const int c = it % bx, k1 = it / bx;
+        float2 y[N2];
+        two_level_step2<N1, N2, -1>(Z + c * ZL, k1, y);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) S[c * LS + k1 + N1 * k2] = y[k2];
+    }
+    lds_barrier();
+    for (int z = 0; z < 16; ++z) {
+        const float* hz = bank + (long)z * N + x0;
+        for (int it = threadIdx.x; it < bx * N2; it += blockDim.x) {
+            const int c = it % bx, j2 = it / bx;
+            float2 x[N1];
+            if (c < nc) {
+#pragma unroll
+                for (int j1 = 0; j1 < N1; ++j1) {
+                    const int row = N2 * j1 + j2;
+                    x[j1] = cscale(S[c * LS + row], hz[(long)row * wd + c]);
+                }
+            } else {
+#pragma unroll
+                for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
+            }
+            two_level_step1<N1, N2, 1>(x, j2, tw, Z + c * ZL);
+        }
+        lds_barrier();
+        float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
+        for (int it = threadIdx.x; it < bx * N1; it += blockDim.x) {
+            const int c = it % bx, k1 = it / bx;
+            float2 y[N2];
+            two_level_step2<N1, N2, 1>(Z + c * ZL, k1, y);
+            if (c < nc) {
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) dst[(long)(k1 + N1 * k2) * wd + c] = y[k2];
+            }
+        }
+        lds_barrier(); /* the next filter overwrites Z */
+    }
+}
+
+/* Row pass: inverse row FFT of the 4 scales of `rb` rows of one image and
+ * orientation, normalised by 1/N, then the response epilogue, the median
+ * keys and their top-bit histogram (as fsim_rowpass_kernel). */
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
+                                                               float* __restrict__ an_e,
+                                                               float* __restrict__ keys,
+                                                               unsigned* __restrict__ hist,
+                                                               const float2* __restrict__ twg,
+                                                               int hd, int rb, float invN) {
+    constexpr int n = N1 * N2; /* = wd */
+    constexpr int ZL = Exch<N1, N2>::line;
+    constexpr int LE = n + 1;
+    extern __shared__ float2 lds[];
+    const int nl = 4 * rb;
+    float2* Z = lds;
+    float2* E = Z + nl * ZL;
+    float2* tw = E + nl * LE;
+    unsigned* hl = (unsigned*)(tw + n);
+    const int io = blockIdx.y, pair = blockIdx.z;
+    const long N = (long)n * hd;
+    const int y0 = blockIdx.x * rb;
+    const int nr = min(rb, hd - y0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) hl[i] = 0u;
+    __syncthreads();
+    /* step 1, items (line, j2), j2 fastest; line = s * rb + r */
+    for (int it = threadIdx.x; it < nl * N2; it += blockDim.x) {
+        const int l = it / N2, j2 = it - l * N2;
+        const int s = l / rb, r = l - s * rb;
+        float2 x[N1];
+        if (r < nr) {
+            const float2* src = Tin + ((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * n;
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = src[N2 * j1 + j2];
+        } else {
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
+        }
+        two_level_step1<N1, N2, 1>(x, j2, tw, Z + l * ZL);
+    }
+    lds_barrier();
+    for (int it = threadIdx.x; it < nl * N1; it += blockDim.x) {
+        const int l = it / N1, k1 = it - l * N1;
+        float2 y[N2];
+        two_level_step2<N1, N2, 1>(Z + l * ZL, k1, y);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) E[l * LE + k1 + N1 * k2] = y[k2];
+    }
+    lds_barrier();
+    float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
+    float* enp = anp + N;
+    float* kp = keys + ((long)pair * 8 + io) * N;
+    for (int idx = threadIdx.x; idx < nr * n; idx += blockDim.x) {
+        const int r = idx / n, x = idx - r * n;
+        float2 eo[4];
+        float amp = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            eo[s] = cscale(E[(s * rb + r) * LE + x], invN);
+            amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
+            sr += eo[s].x;
+            si += eo[s].y;
+        }
+        const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
+        const float me = sr / xe, mo = si / xe;
+        float energy = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+        const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+        const long g = (long)(y0 + r) * n + x;
+        anp[g] = amp;
+        enp[g] = energy;
+        kp[g] = key;
+        atomicAdd(&hl[(__float_as_uint(key) >> 20) & 2047u], 1u);
+    }
+    __syncthreads();
+    unsigned* hg = hist + ((long)pair * 8 + io) * kHistBinsTl;
+    for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) {
+        const unsigned c = hl[i];
+        if (c) atomicAdd(&hg[i], c);
+    }
+}
+
+namespace {
+
+template <int N1, int N2>
+size_t col_lds(int bx) {
+    return sizeof(float2) * ((size_t)bx * (N1 * N2 + 1) + (size_t)bx * Exch<N1, N2>::line + N1 * N2);
+}
+template <int N1, int N2>
+size_t row_lds(int rb) {
+    return sizeof(float2) * ((size_t)4 * rb * Exch<N1, N2>::line + (size_t)4 * rb * (N1 * N2 + 1) + N1 * N2) +
+           sizeof(unsigned) * kHistBinsTl;
+}
+
+int threads_for(int items) {
+    int t = ((items + 63) / 64) * 64;
+    return t < 64 ? 64 : (t > 256 ? 256 : t);
+}
+
+template <int N1, int N2>
+void col_launch(Ctx& c, const TlColArgs& a) {
+    const int n = N1 * N2;
+    int bx = a.bx;
+    while (bx > 1 && col_lds<N1, N2>(bx) > a.lds_budget) --bx;
+    const int nbx = ceil_div(a.wd, bx);
+    const int nbx_pad = ceil_div(nbx, a.xcds) * a.xcds;
+    const int thr = threads_for(bx * (N1 > N2 ? N1 : N2));
+    c.launch("fsim_colpass", fsim_colpass_tl_kernel<N1, N2>, dim3(nbx_pad * 2 * a.npairs), dim3(thr),
+             col_lds<N1, N2>(bx), a.A, a.T, a.bank, a.tw, a.wd, bx, nbx_pad);
+    (void)n;
+}
+
+template <int N1, int N2>
+void row_launch(Ctx& c, const TlRowArgs& a) {
+    int rb = a.rb;
+    while (rb > 1 && row_lds<N1, N2>(rb) > a.lds_budget) --rb;
+    const int thr = threads_for(4 * rb * (N1 > N2 ? N1 : N2));
+    c.launch("fsim_rowpass", fsim_rowpass_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), 8, a.npairs),
+             dim3(thr), row_lds<N1, N2>(rb), a.T, a.an_e, a.keys, a.hist, a.tw, a.hd, rb, a.invN);
+}
+
+}  // namespace
+
+/* Compile-time splits available to the two-level kernels. */
+bool tl_supported(int n) { return n == 270 || n == 480 || n == 256; }
+
+void tl_colpass(Ctx& c, int hd, const TlColArgs& a) {
+    switch (hd) {
+        case 270: col_launch<15, 18>(c, a); break;
+        case 480: col_launch<20, 24>(c, a); break;
+        case 256: col_launch<16, 16>(c, a); break;
+        default: throw Error(IQM_ERR_UNSUPPORTED, "no two-level column split");
+    }
+}
+
+void tl_rowpass(Ctx& c, int wd, const TlRowArgs& a) {
+    switch (wd) {
+        case 270: row_launch<15, 18>(c, a); break;
+        case 480: row_launch<20, 24>(c, a); break;
+        case 256: row_launch<16, 16>(c, a); break;
+        default: throw Error(IQM_ERR_UNSUPPORTED, "no two-level row split");
+    }
+}
+
+}  // namespace iqm

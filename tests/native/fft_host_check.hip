@@ -1,0 +1,128 @@
+// Host-side check of the FFT index math (no GPU needed): runs the same
+// __host__ __device__ building blocks the kernels use -- Stockham stages,
+// register DFTs and the two-level line split -- sequentially on the CPU and
+// prints the largest error against a direct double-precision DFT.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "fft_reg.hpp"
+
+using namespace iqm;
+using cd = std::complex<double>;
+
+static std::vector<cd> dft(const std::vector<cd>& x, int S) {
+    const int n = (int)x.size();
+    std::vector<cd> y(n);
+    for (int k = 0; k < n; ++k) {
+        cd acc = 0;
+        for (int j = 0; j < n; ++j) {
+            const double a = 2.0 * M_PI * (double)((long)j * k % n) / n;
+            acc += x[j] * cd(std::cos(a), S * std::sin(a));
+        }
+        y[k] = acc;
+    }
+    return y;
+}
+
+static std::vector<cd> rnd(int n, int seed) {
+    std::mt19937 g(seed);
+    std::normal_distribution<double> d;
+    std::vector<cd> x(n);
+    for (auto& v : x) v = cd(d(g), d(g));
+    return x;
+}
+
+static double relerr(const std::vector<float2>& got, const std::vector<cd>& ref) {
+    double e = 0, m = 0;
+    for (size_t i = 0; i < ref.size(); ++i) {
+        e = std::max(e, std::abs(cd(got[i].x, got[i].y) - ref[i]));
+        m = std::max(m, std::abs(ref[i]));
+    }
+    return e / m;
+}
+
+static std::vector<float2> twiddles(int n) {
+    std::vector<float2> tw(n);
+    for (int j = 0; j < n; ++j) {
+        double a = 2.0 * M_PI * j / n;
+        tw[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    return tw;
+}
+
+template <int S>
+static double check_stockham(int n) {
+    std::vector<int> rad;
+    if (!plan_radices(n, rad)) return -1;
+    auto x = rnd(n, n);
+    std::vector<float2> a(n), b(n);
+    for (int i = 0; i < n; ++i) a[i] = make_float2((float)x[i].real(), (float)x[i].imag());
+    auto tw = twiddles(n);
+    float2 *src = a.data(), *dst = b.data();
+    int p = 1;
+    for (int r : rad) {
+        for (int tid = 0; tid < 7; ++tid) stockham_dispatch<S>(r, src, dst, n, 1, n, p, tw.data(), tid, 7);
+        std::swap(src, dst);
+        p *= r;
+    }
+    return relerr(std::vector<float2>(src, src + n), dft(x, S));
+}
+
+template <int M, int S>
+static double check_reg() {
+    auto x = rnd(M, 1000 + M);
+    float2 v[M];
+    for (int i = 0; i < M; ++i) v[i] = make_float2((float)x[i].real(), (float)x[i].imag());
+    RegDft<M, S>::run(v);
+    return relerr(std::vector<float2>(v, v + M), dft(x, S));
+}
+
+template <int N1, int N2, int S>
+static double check_two_level() {
+    constexpr int n = N1 * N2;
+    auto x = rnd(n, 7 * n);
+    auto tw = twiddles(n);
+    std::vector<float2> z(Exch<N1, N2>::line), out(n);
+    for (int j2 = 0; j2 < N2; ++j2) {
+        float2 v[N1];
+        for (int j1 = 0; j1 < N1; ++j1) {
+            const cd e = x[N2 * j1 + j2];
+            v[j1] = make_float2((float)e.real(), (float)e.imag());
+        }
+        two_level_step1<N1, N2, S>(v, j2, tw.data(), z.data());
+    }
+    for (int k1 = 0; k1 < N1; ++k1) {
+        float2 y[N2];
+        two_level_step2<N1, N2, S>(z.data(), k1, y);
+        for (int k2 = 0; k2 < N2; ++k2) out[k1 + N1 * k2] = y[k2];
+    }
+    return relerr(out, dft(x, S));
+}
+
+int main() {
+    double worst = 0;
+    auto rep = [&](const char* what, int n, double e) {
+        std::printf("%s %d %.3e\n", what, n, e);
+        if (e < 0 || e > worst) worst = e < 0 ? 1.0 : e;
+    };
+    for (int n : {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 27, 45, 64, 96, 270, 480, 1024, 4095}) {
+        rep("stockham_fwd", n, check_stockham<-1>(n));
+        rep("stockham_inv", n, check_stockham<1>(n));
+    }
+    rep("reg_fwd", 15, check_reg<15, -1>());
+    rep("reg_fwd", 16, check_reg<16, -1>());
+    rep("reg_fwd", 18, check_reg<18, -1>());
+    rep("reg_fwd", 20, check_reg<20, -1>());
+    rep("reg_inv", 24, check_reg<24, 1>());
+    rep("reg_inv", 30, check_reg<30, 1>());
+    rep("reg_fwd", 32, check_reg<32, -1>());
+    rep("two_level_fwd", 270, check_two_level<15, 18, -1>());
+    rep("two_level_inv", 270, check_two_level<15, 18, 1>());
+    rep("two_level_fwd", 480, check_two_level<20, 24, -1>());
+    rep("two_level_inv", 480, check_two_level<20, 24, 1>());
+    std::printf("worst %.3e\n", worst);
+    return worst < 2e-6 ? 0 : 1;
+}

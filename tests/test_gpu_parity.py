@@ -169,3 +169,35 @@ def test_errors(ctx):
     with pytest.raises(iqm_amd.IqmError) as ei:
         iqm_amd.SSIM().compute_metric(ctx, tiny, tiny)
     assert ei.value.code == iqm_amd.IQM_ERR_TOO_SMALL
+
+
+def test_graph_replay_matches_eager(ctx):
+    """A repeated batch call replays a captured hipGraph; every replay, with
+    eager work and synchronisations in between, equals the eager result."""
+    w, h, n = 960, 540, 4
+    _, wd, hd = iqm_amd.fsim_dims(w, h)  # F = 2 here: 480 x 270
+    pairs = [synth.rgb_pair(w, h, seed=200 + i) for i in range(2)]
+    imgs = np.stack([pairs[i % 2][0] for i in range(n)])
+    refs = np.stack([pairs[i % 2][1] for i in range(n)])
+    maps = np.empty((n, hd, wd), np.float32)
+    di, dr, dm = ctx.alloc(imgs.nbytes), ctx.alloc(refs.nbytes), ctx.alloc(maps.nbytes)
+    try:
+        ctx.upload(di, imgs)
+        ctx.upload(dr, refs)
+        f = iqm_amd.FSIM()
+        ctx.set_graphs(False)
+        fs0, fc0 = f.compute_batch_dev(ctx, di, dr, w, h, 3, w * h * 3, n, dm)
+        m0 = ctx.download(dm, maps).copy()
+        ctx.set_graphs(True)
+        for rep in range(4):
+            fs, fc = f.compute_batch_dev(ctx, di, dr, w, h, 3, w * h * 3, n, dm)
+            m = ctx.download(dm, maps)
+            assert np.array_equal(fs, fs0) and np.array_equal(fc, fc0), rep
+            assert np.array_equal(m, m0), rep
+            iqm_amd.SSIM().compute_metric(ctx, pairs[0][0], pairs[0][1], want_map=False)
+            ctx.sync()
+    finally:
+        ctx.set_graphs(True)
+        ctx.free(di)
+        ctx.free(dr)
+        ctx.free(dm)
