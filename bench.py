@@ -94,6 +94,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-stages", action="store_true", help="skip the per-kernel timing pass")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,6 +118,14 @@ def main() -> int:
     ctx.upload(d_img, imgs)
     ctx.upload(d_ref, refs)
     fsim = iqm_amd.FSIM()
+    if args.copy_calib_mb:
+        nb = args.copy_calib_mb << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        hip = _hip()
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), nb, 3) == 0  # device to device
+        ctx.free(a)
+        ctx.free(b)
 
     def step():
         return fsim.compute_batch_dev(ctx, d_img, d_ref, W, H, CH, pair_bytes, B)

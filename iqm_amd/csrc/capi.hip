@@ -324,6 +324,7 @@ int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* f
         iqm_fsim_dims(img->width, img->height, &F, &wd, &hd);
         const long N = (long)wd * hd;
         iqm::FsimBatchOut out;
+        out.want_grad = detail != nullptr;
         float* dm = nullptr;
         if (maps) {
             dm = (float*)c.workspace(iqm::WS_MAPS, sizeof(float) * 5 * N);

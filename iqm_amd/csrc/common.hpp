@@ -59,6 +59,17 @@ constexpr int kBlockFft = 512;       // workgroup size of the LDS line-transform
 __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
+/* Streaming (non-temporal) 8-byte accesses for data touched once. */
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void nt_store(float2* p, float2 v) {
+    v2f_t t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (v2f_t*)p);
+}
+__device__ __forceinline__ float2 nt_load(const float2* p) {
+    const v2f_t t = __builtin_nontemporal_load((const v2f_t*)p);
+    return make_float2(t.x, t.y);
+}
+
 constexpr int kMaxLdsBytes = 65536;  // per-workgroup LDS budget used by every kernel
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

@@ -42,6 +42,7 @@ struct FsimBatchOut {
     float* d_pc = nullptr;    /* nullable device, n * 2 * N (pc_in, pc_ref) */
     float* d_pcm = nullptr;   /* nullable device, n * N */
     float* d_sim = nullptr;   /* nullable device, n * N */
+    bool want_grad = false;   /* keep the F3 gradient planes in the workspace */
 };
 
 class Ctx {
@@ -133,6 +134,7 @@ class Ctx {
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
     char* fsim_ws_ = nullptr;
     int last_chunk_ = 1;
+    int last_nblk_ = 1;
 
     struct GraphKey {
         const void *a, *b, *o1, *o2, *o3, *o4;

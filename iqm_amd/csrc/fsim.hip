@@ -3,16 +3,16 @@
 // Pipeline per batch of same-size pairs (reference: FSIM::computeMetric,
 // src/gpu/fsim/fsim.cpp:80-163), 7 launches:
 //   fsim_prep          F2: YIQ of 255-scaled RGB, F x F box average
-//   fsim_grad_rowfft   F3: Scharr magnitude; forward row FFT of Y_in + i*Y_ref,
-//                      split into the two row spectra (F4-F5, first half)
+//   fsim_rowfft        forward row FFT of Y_in + i*Y_ref, split into the two
+//                      row spectra (F4-F5, first half)
 //   fsim_colpass       forward column FFT -> spectrum, times H_{s,o} (F9),
 //                      inverse column FFT (F11, first half); 16 planes/image
 //   fsim_rowpass       inverse row FFT (F11, second half) + response epilogue
 //                      (F12) + |EO_{s=0}|^2 keys and their top-bit histogram
 //   fsim_select        exact upper median by radix select (F13) -> noise
 //                      power and threshold T (F15) in double
-//   fsim_pcfinal       phase congruency (F15), final multiply (F16), FP64
-//                      block partial sums (F17)
+//   fsim_pcfinal       Scharr gradient (F3), phase congruency (F15), final
+//                      multiply (F16), FP64 block partial sums (F17)
 //   fsim_finalize      fixed-order sum of the partials, FSIM and FSIMc
 // Per-size constants (filter planes, EM_n of F10, est of F14) are computed
 // once per (wd, hd) by Ctx::fsim_plan and cached.
@@ -182,7 +182,7 @@ const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
 
 /* ---- hot path kernels -------------------------------------------------- */
 
-constexpr int kHistBins = 2048;
+constexpr int kHistBins = 1024; /* level-1 radix-select bins: key bits 30..21 */
 
 /* F2: fsim_downsample.glsl:38-54.  A workgroup produces kPrepCols
  * downscaled pixels of one row for both images: the bytes of the F
@@ -264,14 +264,12 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
     }
 }
 
-/* F3 (fsim_gradientmap.glsl:38-56) and the row half of F5 on the packed
- * rows Y_in + i*Y_ref of `rb` rows; the two real-input row spectra are split
- * with X(k) = (Z(k) + conj Z(-k)) / 2, Y(k) = (Z(k) - conj Z(-k)) / (2i). */
-__global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* __restrict__ yiq,
-                                                                  float* __restrict__ grad,
-                                                                  float2* __restrict__ A,
-                                                                  int wd, int hd, int rb,
-                                                                  LinePlan P) {
+/* Row half of F5 on the packed rows Y_in + i*Y_ref of `rb` rows; the two
+ * real-input row spectra are split with X(k) = (Z(k) + conj Z(-k)) / 2,
+ * Y(k) = (Z(k) - conj Z(-k)) / (2i). */
+__global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __restrict__ yiq,
+                                                             float2* __restrict__ A, int wd, int hd,
+                                                             int rb, LinePlan P) {
     extern __shared__ float2 lds[];
     const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
@@ -283,28 +281,9 @@ __global__ __launch_bounds__(kBlock) void fsim_grad_rowfft_kernel(const float* _
     const int nr = min(rb, hd - y0);
     const float* Yin = yiq + (long)pair * 6 * N;
     const float* Yref = Yin + N;
-    const float vert[9] = {3.f, 0.f, -3.f, 10.f, 0.f, -10.f, 3.f, 0.f, -3.f};
-    const float hori[9] = {3.f, 10.f, 3.f, 0.f, 0.f, 0.f, -3.f, -10.f, -3.f};
     for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
-        const int r = idx / wd, x = idx - r * wd, y = y0 + r;
-        const long g = (long)y * wd + x;
-        for (int im = 0; im < 2; ++im) {
-            const float* Y = im ? Yref : Yin;
-            float vs = 0.f, hs = 0.f;
-            for (int j = -1; j < 2; ++j) {
-                for (int k = -1; k < 2; ++k) {
-                    const int px = x + j, py = y + k;
-                    if (px >= 0 && px < wd && py >= 0 && py < hd) {
-                        const float v = Y[(long)py * wd + px];
-                        const int i3 = (j + 1) + (k + 1) * 3;
-                        vs += v * vert[i3];
-                        hs += v * hori[i3];
-                    }
-                }
-            }
-            const float gv = vs / 16.0f, gh = hs / 16.0f;
-            grad[((long)pair * 2 + im) * N + g] = sqrtf(gv * gv + gh * gh);
-        }
+        const int r = idx / wd, x = idx - r * wd;
+        const long g = (long)(y0 + r) * wd + x;
         b0[r * ld + x] = make_float2(Yin[g], Yref[g]);
     }
     __syncthreads();
@@ -433,7 +412,7 @@ __global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* _
         anp[g] = amp;
         enp[g] = energy;
         kp[g] = key;
-        atomicAdd(&hl[(__float_as_uint(key) >> 20) & 2047u], 1u);
+        atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
     }
     __syncthreads();
     unsigned* hg = hist + ((long)pair * 8 + io) * kHistBins;
@@ -495,15 +474,15 @@ struct NoiseArgs {
 
 /* F13 + F15 thresholds: exact element N/2 of the ascending keys
  * (fsim_noise_power.cpp:198-199) by a 3-level radix select on the float bit
- * patterns (keys are >= 0, so the bits order like the values): level 1 uses
- * the histogram built by fsim_rowpass (bits 30..20), level 2 bits 19..9,
- * level 3 bits 8..0. */
+ * patterns (keys are >= 0, so the bits order like the values and bit 31 is
+ * clear): level 1 uses the histogram built by fsim_rowpass (bits 30..21),
+ * level 2 bits 20..10, level 3 bits 9..0. */
 __global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restrict__ keys,
                                                            const unsigned* __restrict__ hist,
                                                            float* __restrict__ thr,
                                                            double* __restrict__ stats, long N,
                                                            NoiseArgs na) {
-    __shared__ unsigned h[kHistBins];
+    __shared__ unsigned h[2048];
     __shared__ unsigned wsum[16];
     __shared__ int bucket;
     __shared__ unsigned rest;
@@ -519,27 +498,51 @@ __global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restri
     rank_bucket(h, kHistBins, k, &bucket, &rest, wsum);
     const unsigned b1 = (unsigned)bucket, k1 = rest;
 
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = 0u;
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) h[i] = 0u;
     __syncthreads();
-    for (long i = threadIdx.x; i < N; i += blockDim.x) {
-        const unsigned u = kb[i];
-        if (((u >> 20) & 2047u) == b1) atomicAdd(&h[(u >> 9) & 2047u], 1u);
+    /* keys are read as uint4 when every plane starts 16-byte aligned */
+    const bool vec = (N % 4) == 0;
+    if (vec) {
+        const uint4* k4 = (const uint4*)kb;
+        for (long i = threadIdx.x; i < N / 4; i += blockDim.x) {
+            const uint4 q = k4[i];
+            const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (((u[j] >> 21) & 1023u) == b1) atomicAdd(&h[(u[j] >> 10) & 2047u], 1u);
+        }
+    } else {
+        for (long i = threadIdx.x; i < N; i += blockDim.x) {
+            const unsigned u = kb[i];
+            if (((u >> 21) & 1023u) == b1) atomicAdd(&h[(u >> 10) & 2047u], 1u);
+        }
     }
     __syncthreads();
     rank_bucket(h, 2048, k1, &bucket, &rest, wsum);
     const unsigned b2 = (unsigned)bucket, k2 = rest;
 
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0u;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) h[i] = 0u;
     __syncthreads();
     const unsigned pre = (b1 << 11) | b2;
-    for (long i = threadIdx.x; i < N; i += blockDim.x) {
-        const unsigned u = kb[i];
-        if (((u >> 9) & 0x3FFFFFu) == pre) atomicAdd(&h[u & 511u], 1u);
+    if (vec) {
+        const uint4* k4 = (const uint4*)kb;
+        for (long i = threadIdx.x; i < N / 4; i += blockDim.x) {
+            const uint4 q = k4[i];
+            const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (((u[j] >> 10) & 0x1FFFFFu) == pre) atomicAdd(&h[u[j] & 1023u], 1u);
+        }
+    } else {
+        for (long i = threadIdx.x; i < N; i += blockDim.x) {
+            const unsigned u = kb[i];
+            if (((u >> 10) & 0x1FFFFFu) == pre) atomicAdd(&h[u & 1023u], 1u);
+        }
     }
     __syncthreads();
-    rank_bucket(h, 512, k2, &bucket, &rest, wsum);
+    rank_bucket(h, 1024, k2, &bucket, &rest, wsum);
     if (threadIdx.x == 0) {
-        const unsigned bits = (b1 << 20) | (b2 << 9) | (unsigned)bucket;
+        const unsigned bits = (b1 << 21) | (b2 << 10) | (unsigned)bucket;
         const double median = (double)__uint_as_float(bits);
         const int o = io & 3;
         /* fsim_noise_power.cpp:199-201 */
@@ -566,17 +569,54 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-/* F15 (fsim_phase_congruency.glsl:40-63) + F16 (fsim_final_multiply.glsl:
- * 26-57) + per-block FP64 partials of F17. */
+/* F3 (fsim_gradientmap.glsl:38-56) + F15 (fsim_phase_congruency.glsl:40-63)
+ * + F16 (fsim_final_multiply.glsl:26-57) + per-block FP64 partials of F17.
+ * A workgroup covers a kPcTx x kPcTy tile; the Y planes of both images are
+ * staged in LDS with a one-pixel halo for the Scharr stencil. */
+constexpr int kPcTx = 64, kPcTy = 4;
+
 __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
-    const float* __restrict__ yiq, const float* __restrict__ grad, const float* __restrict__ an_e,
-    const float* __restrict__ thr, double* __restrict__ partial, long N, float* simc_map,
+    const float* __restrict__ yiq, float* __restrict__ grad, const float* __restrict__ an_e,
+    const float* __restrict__ thr, double* __restrict__ partial, int wd, int hd, float* simc_map,
     float* pc_map, float* pcm_map, float* sim_map) {
+    constexpr int HX = kPcTx + 2, HY = kPcTy + 2;
+    __shared__ float ys[2][HY][HX];
     __shared__ double red[3][kBlock / 64];
-    const int pair = blockIdx.y;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = (long)wd * hd;
+    const int pair = blockIdx.z;
+    const int x0 = blockIdx.x * kPcTx, y0 = blockIdx.y * kPcTy;
+    const float* yq = yiq + (long)pair * 6 * N;
+    for (int i = threadIdx.x; i < 2 * HY * HX; i += blockDim.x) {
+        const int im = i / (HY * HX), rem = i - im * HY * HX;
+        const int ly = rem / HX, lx = rem - ly * HX;
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        ys[im][ly][lx] = (gx >= 0 && gx < wd && gy >= 0 && gy < hd) ? yq[(long)im * N + (long)gy * wd + gx] : 0.f;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kPcTx, ty = threadIdx.x / kPcTx;
+    const int x = x0 + tx, y = y0 + ty;
     double v0 = 0, v1 = 0, v2 = 0;
-    if (idx < N) {
+    if (x < wd && y < hd) {
+        const long idx = (long)y * wd + x;
+        /* Scharr pair as a correlation, x offset j outer, y offset k inner;
+         * the staged zeros outside the image are the shader's skipped taps */
+        float gm[2];
+        for (int im = 0; im < 2; ++im) {
+            float vs = 0.f, hs = 0.f;
+            for (int j = -1; j < 2; ++j) {
+                for (int k = -1; k < 2; ++k) {
+                    const float v = ys[im][ty + 1 + k][tx + 1 + j];
+                    vs += v * ((float)(-j) * (k == 0 ? 10.f : 3.f));  /* verticalArray   */
+                    hs += v * ((float)(-k) * (j == 0 ? 10.f : 3.f));  /* horizontalArray */
+                }
+            }
+            const float gv = vs / 16.0f, gh = hs / 16.0f;
+            gm[im] = sqrtf(gv * gv + gh * gh);
+        }
+        if (grad) {
+            grad[(long)pair * 2 * N + idx] = gm[0];
+            grad[(long)pair * 2 * N + N + idx] = gm[1];
+        }
         const float* ae = an_e + (long)pair * 16 * N;
         const float* T = thr + (long)pair * 8;
         float pc[2];
@@ -591,13 +631,11 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
             }
             pc[im] = et / at; /* no epsilon: NaN where sum An = 0 */
         }
-        const float* yq = yiq + (long)pair * 6 * N;
-        const float* gr = grad + (long)pair * 2 * N;
         const float T1 = 0.85f, T2 = 160.0f, T3 = 200.0f, T4 = 200.0f, lambda = 0.03f;
         const float p1 = pc[0], p2 = pc[1];
         const float pcm = (p1 < p2) ? p2 : p1;
         const float pcsim = (2.0f * p1 * p2 + T1) / (p1 * p1 + p2 * p2 + T1);
-        const float g1 = gr[idx], g2 = gr[N + idx];
+        const float g1 = gm[0], g2 = gm[1];
         const float gsim = (2.0f * g1 * g2 + T2) / (g1 * g1 + g2 * g2 + T2);
         const float sim = gsim * pcsim * pcm;
         const float i1 = yq[2 * N + idx], q1 = yq[3 * N + idx];
@@ -628,13 +666,14 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s[3] = {0, 0, 0};
+        double s3[3] = {0, 0, 0};
         for (int w = 0; w < (int)(blockDim.x / 64); ++w)
-            for (int j = 0; j < 3; ++j) s[j] += red[j][w];
-        double* p = partial + ((long)pair * gridDim.x + blockIdx.x) * 4;
-        p[0] = s[0];
-        p[1] = s[1];
-        p[2] = s[2];
+            for (int j = 0; j < 3; ++j) s3[j] += red[j][w];
+        const int nblk = gridDim.x * gridDim.y;
+        double* p = partial + ((long)pair * nblk + blockIdx.y * gridDim.x + blockIdx.x) * 4;
+        p[0] = s3[0];
+        p[1] = s3[1];
+        p[2] = s3[2];
         p[3] = 0.0;
     }
 }
@@ -715,8 +754,10 @@ FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long 
     t.grad_rb = lines_that_fit(Px, 2, budget, 4);
     t.col_bx = lines_that_fit(Py, 3, budget, env_int("IQM_FSIM_BX", 8));
     t.row_rb = lines_that_fit(Px, 2, budget, 4 * env_int("IQM_FSIM_RB", 1)) / 4;
-    /* chunk of pairs whose 32 response planes stay within ~128 MB (MALL) */
-    const long chunk_mb = env_int("IQM_FSIM_CHUNK_MB", 128);
+    /* pairs per column/row-pass launch: one launch over the whole batch
+     * measured fastest on MI355X (larger grids beat cache residency of the
+     * response planes); the cap bounds the response-plane workspace */
+    const long chunk_mb = env_int("IQM_FSIM_CHUNK_MB", 16384);
     t.chunk = (int)std::max<long>(1, (chunk_mb << 20) / (long)(sizeof(float2) * 32 * N));
     IQM_CHECK(t.grad_rb > 0 && t.col_bx > 0 && t.row_rb > 0, IQM_ERR_UNSUPPORTED,
               "FSIM grid too large for the LDS line transforms");
@@ -725,8 +766,7 @@ FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long 
 }  // namespace
 
 Ctx::FsimLayout Ctx::fsim_layout(long N, int n, char* base) const {
-    const int nblk = ceil_div(N, kBlock);
-    FsimRegions r = fsim_regions(N, n, last_chunk_, nblk);
+    FsimRegions r = fsim_regions(N, n, last_chunk_, last_nblk_);
     FsimLayout L;
     L.N = N;
     L.yiq = (float*)(base + r.yiq);
@@ -748,10 +788,11 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const LinePlan& Px = line_plan(wd).plan;
     const LinePlan& Py = line_plan(hd).plan;
     const long N = (long)wd * hd;
-    const int nblk = ceil_div(N, kBlock);
+    const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);
     const FsimTiles t = pick_tiles(*this, Px, Py, N);
     const int chunk = std::min(t.chunk, n);
     last_chunk_ = chunk;
+    last_nblk_ = nblk;
     FsimRegions r = fsim_regions(N, n, chunk, nblk);
     char* base = (char*)workspace(WS_FSIM, r.total);
     fsim_ws_ = base;
@@ -774,10 +815,16 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
         launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, kPrepCols), hd, n), dim3(kBlock),
                stage_bytes, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
     }
-    /* F3 + row half of F5 */
-    launch("fsim_grad_rowfft", fsim_grad_rowfft_kernel, dim3(ceil_div(hd, t.grad_rb), n),
-           dim3(kBlock), line_lds_bytes(Px, t.grad_rb, 2), (const float*)yiq, grad, A, wd, hd,
-           t.grad_rb, Px);
+    /* row half of F5 (F3 is evaluated in fsim_pcfinal) */
+    const bool tl = env_int("IQM_FSIM_GENERIC", 0) == 0;
+    const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 160 * 1024));
+    if (tl && tl_supported(wd)) {
+        TlGradArgs a{yiq, A, Px.tw, hd, n, env_int("IQM_FSIM_TL_GRB", 4), tl_budget};
+        tl_rowfft(*this, wd, a);
+    } else {
+        launch("fsim_rowfft", fsim_rowfft_kernel, dim3(ceil_div(hd, t.grad_rb), n), dim3(kBlock),
+               line_lds_bytes(Px, t.grad_rb, 2), (const float*)yiq, A, wd, hd, t.grad_rb, Px);
+    }
     /* per chunk: column half of F5, F9, F11, then the row half of F11 + F12 */
     const int nbx = ceil_div(wd, t.col_bx);
     const int nbx_pad = ceil_div(nbx, xcds_) * xcds_;
@@ -785,14 +832,13 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);
     if ((size_t)4 * t.row_rb * lds_pitch(Px.lds_len) * sizeof(float2) < kHistBins * sizeof(unsigned))
         row_sh += kHistBins * sizeof(unsigned);
-    const bool tl_col = tl_supported(hd) && env_int("IQM_FSIM_GENERIC", 0) == 0;
-    const bool tl_row = tl_supported(wd) && env_int("IQM_FSIM_GENERIC", 0) == 0;
-    const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 80 * 1024));
+    const bool tl_col = tl && tl_supported(hd);
+    const bool tl_row = tl && tl_supported(wd);
     for (int p0 = 0; p0 < n; p0 += chunk) {
         const int np = std::min(chunk, n - p0);
         const float2* Ach = A + (long)p0 * 2 * N;
         if (tl_col) {
-            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", 8), xcds_, tl_budget};
+            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", 16), xcds_, tl_budget};
             tl_colpass(*this, hd, a);
         } else {
             launch("fsim_colpass", fsim_colpass_kernel, dim3(nbx_pad * 2 * np), dim3(kBlockFft),
@@ -818,9 +864,10 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), 0, (const float*)keys,
            (const unsigned*)hist, thr, stats, N, na);
     /* F15 + F16 + F17 */
-    launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(nblk, n), dim3(kBlock), 0, (const float*)yiq,
-           (const float*)grad, (const float*)an_e, (const float*)thr, partial, N, out.d_simc,
-           out.d_pc, out.d_pcm, out.d_sim);
+    launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
+           dim3(kBlock), 0, (const float*)yiq,
+           out.want_grad ? grad : (float*)nullptr, (const float*)an_e, (const float*)thr, partial, wd,
+           hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim);
     launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
            nblk, res);
     return res;

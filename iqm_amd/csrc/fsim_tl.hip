@@ -9,14 +9,14 @@
 
 namespace iqm {
 
-constexpr int kHistBinsTl = 2048;
+constexpr int kHistBinsTl = 1024; /* as kHistBins in fsim.hip */
 
 /* Column pass: forward column FFT of `bx` columns of one image, then for
  * each of the 16 filters z the product H_z * F inverse transformed along the
  * columns and stored to plane img*16 + z.  Grid as fsim_colpass_kernel
  * (column blocks padded to a multiple of 8 for the XCD mapping). */
 template <int N1, int N2>
-__global__ __launch_bounds__(256) void fsim_colpass_tl_kernel(const float2* __restrict__ A,
+__global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __restrict__ A,
                                                                float2* __restrict__ Tout,
                                                                const float* __restrict__ bank,
                                                                const float2* __restrict__ twg,
@@ -84,8 +84,11 @@ __global__ __launch_bounds__(256) void fsim_colpass_tl_kernel(const float2* __re
             float2 y[N2];
             two_level_step2<N1, N2, 1>(Z + c * ZL, k1, y);
             if (c < nc) {
+                /* streamed once to the row pass: keep it out of the L2 so the
+                 * filter bank stays resident */
 #pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) dst[(long)(k1 + N1 * k2) * wd + c] = y[k2];
+                for (int k2 = 0; k2 < N2; ++k2)
+                    nt_store(&dst[(long)(k1 + N1 * k2) * wd + c], y[k2]);
             }
         }
         lds_barrier(); /* the next filter overwrites Z */
@@ -96,20 +99,20 @@ __global__ __launch_bounds__(256) void fsim_colpass_tl_kernel(const float2* __re
  * orientation, normalised by 1/N, then the response epilogue, the median
  * keys and their top-bit histogram (as fsim_rowpass_kernel). */
 template <int N1, int N2>
-__global__ __launch_bounds__(256) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
+__global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
                                                                float* __restrict__ an_e,
                                                                float* __restrict__ keys,
                                                                unsigned* __restrict__ hist,
                                                                const float2* __restrict__ twg,
                                                                int hd, int rb, float invN) {
     constexpr int n = N1 * N2; /* = wd */
-    constexpr int ZL = Exch<N1, N2>::line;
-    constexpr int LE = n + 1;
+    constexpr int ZL = Exch<N1, N2>::line > n + 1 ? Exch<N1, N2>::line : n + 1;
+    constexpr int LE = ZL; /* E shares Z's line pitch */
     extern __shared__ float2 lds[];
     const int nl = 4 * rb;
     float2* Z = lds;
-    float2* E = Z + nl * ZL;
-    float2* tw = E + nl * LE;
+    float2* E = lds; /* aliases Z, see step 2 */
+    float2* tw = Z + nl * ZL;
     unsigned* hl = (unsigned*)(tw + n);
     const int io = blockIdx.y, pair = blockIdx.z;
     const long N = (long)n * hd;
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(256) void fsim_rowpass_tl_kernel(const float2* __re
         if (r < nr) {
             const float2* src = Tin + ((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * n;
 #pragma unroll
-            for (int j1 = 0; j1 < N1; ++j1) x[j1] = src[N2 * j1 + j2];
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = nt_load(&src[N2 * j1 + j2]);
         } else {
 #pragma unroll
             for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
@@ -134,14 +137,27 @@ __global__ __launch_bounds__(256) void fsim_rowpass_tl_kernel(const float2* __re
         two_level_step1<N1, N2, 1>(x, j2, tw, Z + l * ZL);
     }
     lds_barrier();
-    for (int it = threadIdx.x; it < nl * N1; it += blockDim.x) {
-        const int l = it / N1, k1 = it - l * N1;
+    /* step 2 in rounds of blockDim items: every item of a round reads its
+     * exchange row into registers before any item writes outputs over the
+     * same LDS (E aliases Z) */
+    const int items2 = nl * N1;
+    for (int base = 0; base < items2; base += blockDim.x) {
+        const int it = base + threadIdx.x;
         float2 y[N2];
-        two_level_step2<N1, N2, 1>(Z + l * ZL, k1, y);
+        const bool act = it < items2;
+        const int l = act ? it / N1 : 0, k1 = act ? it - l * N1 : 0;
+        if (act) {
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) E[l * LE + k1 + N1 * k2] = y[k2];
+            for (int j2 = 0; j2 < N2; ++j2) y[j2] = Z[l * ZL + k1 * Exch<N1, N2>::pitch + j2];
+        }
+        lds_barrier();
+        if (act) {
+            RegDft<N2, 1>::run(y);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) E[l * LE + k1 + N1 * k2] = y[k2];
+        }
+        lds_barrier();
     }
-    lds_barrier();
     float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
     float* enp = anp + N;
     float* kp = keys + ((long)pair * 8 + io) * N;
@@ -167,13 +183,78 @@ __global__ __launch_bounds__(256) void fsim_rowpass_tl_kernel(const float2* __re
         anp[g] = amp;
         enp[g] = energy;
         kp[g] = key;
-        atomicAdd(&hl[(__float_as_uint(key) >> 20) & 2047u], 1u);
+        atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
     }
     __syncthreads();
     unsigned* hg = hist + ((long)pair * 8 + io) * kHistBinsTl;
     for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) {
         const unsigned c = hl[i];
         if (c) atomicAdd(&hg[i], c);
+    }
+}
+
+/* Two-level version of fsim_rowfft_kernel (fsim.hip): the forward row FFT
+ * of the packed rows Y_in + i*Y_ref and the split into the two row spectra. */
+template <int N1, int N2>
+__global__ __launch_bounds__(512) void fsim_rowfft_tl_kernel(const float* __restrict__ yiq,
+                                                             float2* __restrict__ A,
+                                                                  const float2* __restrict__ twg,
+                                                                  int hd, int rb) {
+    constexpr int n = N1 * N2; /* = wd */
+    constexpr int ZL = Exch<N1, N2>::line > n + 1 ? Exch<N1, N2>::line : n + 1;
+    extern __shared__ float2 lds[];
+    float2* Z = lds;
+    float2* E = lds; /* aliases Z */
+    float2* tw = Z + rb * ZL;
+    const int pair = blockIdx.y;
+    const long N = (long)n * hd;
+    const int y0 = blockIdx.x * rb;
+    const int nr = min(rb, hd - y0);
+    const float* Yin = yiq + (long)pair * 6 * N;
+    const float* Yref = Yin + N;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    __syncthreads();
+    for (int it = threadIdx.x; it < rb * N2; it += blockDim.x) {
+        const int r = it / N2, j2 = it - r * N2;
+        float2 x[N1];
+        if (r < nr) {
+            const long o = (long)(y0 + r) * n + j2;
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(Yin[o + N2 * j1], Yref[o + N2 * j1]);
+        } else {
+#pragma unroll
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
+        }
+        two_level_step1<N1, N2, -1>(x, j2, tw, Z + r * ZL);
+    }
+    lds_barrier();
+    const int items2 = rb * N1;
+    for (int base = 0; base < items2; base += blockDim.x) {
+        const int it = base + threadIdx.x;
+        float2 y[N2];
+        const bool act = it < items2;
+        const int r = act ? it / N1 : 0, k1 = act ? it - r * N1 : 0;
+        if (act) {
+#pragma unroll
+            for (int j2 = 0; j2 < N2; ++j2) y[j2] = Z[r * ZL + k1 * Exch<N1, N2>::pitch + j2];
+        }
+        lds_barrier();
+        if (act) {
+            RegDft<N2, -1>::run(y);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) E[r * ZL + k1 + N1 * k2] = y[k2];
+        }
+        lds_barrier();
+    }
+    float2* Ain = A + (long)pair * 2 * N;
+    float2* Aref = Ain + N;
+    for (int idx = threadIdx.x; idx < nr * n; idx += blockDim.x) {
+        const int r = idx / n, k = idx - r * n;
+        const int m = (k == 0) ? 0 : n - k;
+        const float2 zk = E[r * ZL + k], zm = E[r * ZL + m];
+        const long g = (long)(y0 + r) * n + k;
+        Ain[g] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        Aref[g] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
     }
 }
 
@@ -185,13 +266,13 @@ size_t col_lds(int bx) {
 }
 template <int N1, int N2>
 size_t row_lds(int rb) {
-    return sizeof(float2) * ((size_t)4 * rb * Exch<N1, N2>::line + (size_t)4 * rb * (N1 * N2 + 1) + N1 * N2) +
-           sizeof(unsigned) * kHistBinsTl;
+    const size_t line = Exch<N1, N2>::line > N1 * N2 + 1 ? Exch<N1, N2>::line : N1 * N2 + 1;
+    return sizeof(float2) * ((size_t)4 * rb * line + N1 * N2) + sizeof(unsigned) * kHistBinsTl;
 }
 
 int threads_for(int items) {
     int t = ((items + 63) / 64) * 64;
-    return t < 64 ? 64 : (t > 256 ? 256 : t);
+    return t < 64 ? 64 : (t > 512 ? 512 : t);
 }
 
 template <int N1, int N2>
@@ -216,6 +297,18 @@ void row_launch(Ctx& c, const TlRowArgs& a) {
              dim3(thr), row_lds<N1, N2>(rb), a.T, a.an_e, a.keys, a.hist, a.tw, a.hd, rb, a.invN);
 }
 
+template <int N1, int N2>
+void grad_launch(Ctx& c, const TlGradArgs& a) {
+    constexpr int n = N1 * N2;
+    constexpr size_t line = Exch<N1, N2>::line > n + 1 ? Exch<N1, N2>::line : n + 1;
+    int rb = a.rb;
+    auto lds = [&](int r) { return sizeof(float2) * ((size_t)r * line + n); };
+    while (rb > 1 && lds(rb) > a.lds_budget) --rb;
+    const int thr = threads_for(rb * (N1 > N2 ? N1 : N2));
+    c.launch("fsim_rowfft", fsim_rowfft_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), a.npairs),
+             dim3(thr), lds(rb), a.yiq, a.A, a.tw, a.hd, rb);
+}
+
 }  // namespace
 
 /* Compile-time splits available to the two-level kernels. */
@@ -227,6 +320,15 @@ void tl_colpass(Ctx& c, int hd, const TlColArgs& a) {
         case 480: col_launch<20, 24>(c, a); break;
         case 256: col_launch<16, 16>(c, a); break;
         default: throw Error(IQM_ERR_UNSUPPORTED, "no two-level column split");
+    }
+}
+
+void tl_rowfft(Ctx& c, int wd, const TlGradArgs& a) {
+    switch (wd) {
+        case 270: grad_launch<15, 18>(c, a); break;
+        case 480: grad_launch<20, 24>(c, a); break;
+        case 256: grad_launch<16, 16>(c, a); break;
+        default: throw Error(IQM_ERR_UNSUPPORTED, "no two-level row split");
     }
 }
 

@@ -25,7 +25,16 @@ struct TlRowArgs {
     size_t lds_budget;
 };
 
+struct TlGradArgs {
+    const float* yiq;
+    float2* A;
+    const float2* tw;     /* exp(-2 pi i j / wd) */
+    int hd, npairs, rb;
+    size_t lds_budget;
+};
+
 bool tl_supported(int n);
+void tl_rowfft(Ctx& c, int wd, const TlGradArgs& a);
 void tl_colpass(Ctx& c, int hd, const TlColArgs& a);
 void tl_rowpass(Ctx& c, int wd, const TlRowArgs& a);
 
