@@ -94,6 +94,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-stages", action="store_true", help="skip the per-kernel timing pass")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -129,6 +130,10 @@ def main() -> int:
 
     def step():
         return fsim.compute_batch_dev(ctx, d_img, d_ref, W, H, CH, pair_bytes, B)
+
+    tune_us = None
+    if not args.no_tune:  # outside the timed region: picks the tiles for this size
+        tune_us = ctx.autotune_fsim(W, H, CH, 8)
 
     for _ in range(args.warmup):
         fs, fc = step()
@@ -184,6 +189,7 @@ def main() -> int:
                      "alg_bytes_per_pair": bytes_pair},
         "wall_pairs_per_s": B * args.steps / wall_s,
         "stages_us": stages,
+        "tiles": json.loads(ctx.export_tunes()).get("fsim", []),
         "dominant_kernel": dominant,
         "device": ctx.device_name,
     }

@@ -148,6 +148,20 @@ int iqm_sync(iqm_ctx* ctx);
  * it off at context creation). */
 int iqm_set_graphs(iqm_ctx* ctx, int32_t on);
 
+/* ---- autotuner (tile shapes of the FSIM launch sequence) ---------------- */
+
+/* Times candidate tile shapes for images of w x h x ch on n synthetic pairs
+ * and keeps the fastest for that downscaled size in the context (used by every
+ * later FSIM call of that size).  us_per_pair (nullable) gets its time. */
+int iqm_autotune_fsim(iqm_ctx* ctx, int32_t w, int32_t h, int32_t ch, int32_t n,
+                      double* us_per_pair);
+/* The context's tile table as JSON (a plan file); returns its length without
+ * the terminating NUL and writes at most cap bytes (NUL-terminated when it
+ * fits).  Returns -1 on error. */
+int64_t iqm_tune_export(iqm_ctx* ctx, char* buf, int64_t cap);
+/* Loads a plan file written by iqm_tune_export into the context. */
+int iqm_tune_import(iqm_ctx* ctx, const char* json);
+
 /* ---- stage timing ------------------------------------------------------- */
 
 /* When on, every kernel launch is bracketed by hipEvents on the context's

@@ -32,8 +32,8 @@ EXPORTS = (
     "iqm_create", "iqm_destroy", "iqm_last_error", "iqm_device_name", "iqm_ssim_default_params",
     "iqm_ssim", "iqm_ssim_batch_dev", "iqm_fsim_dims", "iqm_fsim", "iqm_fsim_batch_dev",
     "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
-    "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_set_stage_timing", "iqm_stage_times",
-    "iqm_reset_stage_times",
+    "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_tune_export",
+    "iqm_tune_import", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
 )
 
 
@@ -100,6 +100,10 @@ def lib():
     L.iqm_stream.restype = vp
     L.iqm_sync.argtypes = [vp]
     L.iqm_set_graphs.argtypes = [vp, i32]
+    L.iqm_autotune_fsim.argtypes = [vp, i32, i32, i32, i32, dp]
+    L.iqm_tune_export.argtypes = [vp, ctypes.c_char_p, i64]
+    L.iqm_tune_export.restype = i64
+    L.iqm_tune_import.argtypes = [vp, ctypes.c_char_p]
     L.iqm_set_stage_timing.argtypes = [vp, i32]
     L.iqm_stage_times.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(i64)]
     L.iqm_stage_times.restype = i32
@@ -187,6 +191,23 @@ class Context:
 
     def set_graphs(self, on: bool) -> None:
         _check(lib().iqm_set_graphs(self.handle, 1 if on else 0))
+
+    def autotune_fsim(self, w: int, h: int, ch: int = 3, n: int = 8) -> float:
+        """Tunes the FSIM tiles for w x h inputs; returns the best us/pair."""
+        us = np.zeros(1, np.float64)
+        _check(lib().iqm_autotune_fsim(self.handle, w, h, ch, n, _dptr(us)))
+        return float(us[0])
+
+    def export_tunes(self) -> str:
+        n = lib().iqm_tune_export(self.handle, None, 0)
+        if n < 0:
+            raise IqmError(IQM_ERR_INVALID, lib().iqm_last_error().decode())
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().iqm_tune_export(self.handle, buf, n + 1)
+        return buf.value.decode()
+
+    def import_tunes(self, plan: str) -> None:
+        _check(lib().iqm_tune_import(self.handle, plan.encode()))
 
     def set_stage_timing(self, on: bool) -> None:
         _check(lib().iqm_set_stage_timing(self.handle, 1 if on else 0))

@@ -464,6 +464,36 @@ int iqm_sync(iqm_ctx* ctx) {
     return guarded([&] { ctx->c->sync(); });
 }
 
+int iqm_autotune_fsim(iqm_ctx* ctx, int32_t w, int32_t h, int32_t ch, int32_t n,
+                      double* us_per_pair) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        iqm::FsimTune t = ctx->c->autotune_fsim(w, h, ch, n);
+        if (us_per_pair) *us_per_pair = t.us_per_pair;
+    });
+}
+
+int64_t iqm_tune_export(iqm_ctx* ctx, char* buf, int64_t cap) {
+    if (!ctx) return -1;
+    int64_t len = -1;
+    if (guarded([&] {
+            const std::string s = ctx->c->export_tunes();
+            len = (int64_t)s.size();
+            if (buf && cap > 0) {
+                const int64_t n = std::min<int64_t>(len, cap - 1);
+                std::memcpy(buf, s.data(), (size_t)n);
+                buf[n] = '\0';
+            }
+        }) != IQM_OK)
+        return -1;
+    return len;
+}
+
+int iqm_tune_import(iqm_ctx* ctx, const char* json) {
+    if (!ctx || !json) return fail(IQM_ERR_INVALID, "null argument");
+    return guarded([&] { ctx->c->import_tunes(json); });
+}
+
 int iqm_set_graphs(iqm_ctx* ctx, int32_t on) {
     if (!ctx) return fail(IQM_ERR_INVALID, "null context");
     return guarded([&] { ctx->c->set_graphs(on != 0); });

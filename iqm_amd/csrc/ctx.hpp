@@ -37,6 +37,15 @@ struct FsimNoiseArgs {
     double est[8];
 };
 
+/* Tile choices of the FSIM launch sequence for one (wd, hd). */
+struct FsimTune {
+    int two_level = 1; /* use the register two-level passes when the size has a split */
+    int col_bx = 16;   /* columns per column-pass workgroup */
+    int row_rb = 2;    /* rows per row-pass workgroup */
+    int fft_rb = 4;    /* rows per row-FFT workgroup */
+    double us_per_pair = 0; /* measured by the autotuner (0 = default, not measured) */
+};
+
 struct FsimBatchOut {
     float* d_simc = nullptr;  /* nullable device maps, n * N */
     float* d_pc = nullptr;    /* nullable device, n * 2 * N (pc_in, pc_ref) */
@@ -114,6 +123,18 @@ class Ctx {
     double* fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                              long pair_stride, int n, const FsimBatchOut& out);
     void set_graphs(bool on) { graphs_ = on; }
+
+    /* Tile table: tuned entries per (wd, hd); defaults otherwise. */
+    FsimTune fsim_tune(int wd, int hd) const;
+    void set_fsim_tune(int wd, int hd, const FsimTune& t) {
+        tunes_[{wd, hd}] = t;
+        drop_graphs();
+    }
+    /* Times candidate tiles on n synthetic pairs of w x h x ch and keeps the
+     * fastest for the downscaled size; returns it. */
+    FsimTune autotune_fsim(int w, int h, int ch, int n);
+    std::string export_tunes() const;
+    void import_tunes(const std::string& json);
     bool graphs() const { return graphs_; }
 
     void set_timing(bool on);
@@ -132,6 +153,7 @@ class Ctx {
     std::map<int, std::pair<void*, size_t>> ws_;
     std::map<int, LinePlanHost> line_plans_;
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
+    std::map<std::pair<int, int>, FsimTune> tunes_;
     char* fsim_ws_ = nullptr;
     int last_chunk_ = 1;
     int last_nblk_ = 1;

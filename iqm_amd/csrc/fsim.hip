@@ -748,12 +748,12 @@ struct FsimTiles {
     int grad_rb, col_bx, row_rb, chunk;
 };
 
-FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long N) {
+FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long N, const FsimTune& tune) {
     FsimTiles t;
     const size_t budget = std::min<size_t>(c.lds_per_block(), 65536);
-    t.grad_rb = lines_that_fit(Px, 2, budget, 4);
-    t.col_bx = lines_that_fit(Py, 3, budget, env_int("IQM_FSIM_BX", 8));
-    t.row_rb = lines_that_fit(Px, 2, budget, 4 * env_int("IQM_FSIM_RB", 1)) / 4;
+    t.grad_rb = lines_that_fit(Px, 2, budget, env_int("IQM_FSIM_GRB", tune.fft_rb));
+    t.col_bx = lines_that_fit(Py, 3, budget, env_int("IQM_FSIM_BX", tune.col_bx));
+    t.row_rb = lines_that_fit(Px, 2, budget, 4 * env_int("IQM_FSIM_RB", tune.row_rb)) / 4;
     /* pairs per column/row-pass launch: one launch over the whole batch
      * measured fastest on MI355X (larger grids beat cache residency of the
      * response planes); the cap bounds the response-plane workspace */
@@ -789,7 +789,8 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const LinePlan& Py = line_plan(hd).plan;
     const long N = (long)wd * hd;
     const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);
-    const FsimTiles t = pick_tiles(*this, Px, Py, N);
+    const FsimTune tune = fsim_tune(wd, hd);
+    const FsimTiles t = pick_tiles(*this, Px, Py, N, tune);
     const int chunk = std::min(t.chunk, n);
     last_chunk_ = chunk;
     last_nblk_ = nblk;
@@ -816,10 +817,10 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
                stage_bytes, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
     }
     /* row half of F5 (F3 is evaluated in fsim_pcfinal) */
-    const bool tl = env_int("IQM_FSIM_GENERIC", 0) == 0;
+    const bool tl = env_int("IQM_FSIM_GENERIC", tune.two_level ? 0 : 1) == 0;
     const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 160 * 1024));
     if (tl && tl_supported(wd)) {
-        TlGradArgs a{yiq, A, Px.tw, hd, n, env_int("IQM_FSIM_TL_GRB", 4), tl_budget};
+        TlGradArgs a{yiq, A, Px.tw, hd, n, env_int("IQM_FSIM_TL_GRB", tune.fft_rb), tl_budget};
         tl_rowfft(*this, wd, a);
     } else {
         launch("fsim_rowfft", fsim_rowfft_kernel, dim3(ceil_div(hd, t.grad_rb), n), dim3(kBlock),
@@ -838,7 +839,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
         const int np = std::min(chunk, n - p0);
         const float2* Ach = A + (long)p0 * 2 * N;
         if (tl_col) {
-            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", 16), xcds_, tl_budget};
+            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", tune.col_bx), xcds_, tl_budget};
             tl_colpass(*this, hd, a);
         } else {
             launch("fsim_colpass", fsim_colpass_kernel, dim3(nbx_pad * 2 * np), dim3(kBlockFft),
@@ -847,7 +848,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
         }
         if (tl_row) {
             TlRowArgs a{T, an_e + (long)p0 * 16 * N, keys + (long)p0 * 8 * N,
-                        hist + (long)p0 * 8 * kHistBins, Px.tw, hd, np, env_int("IQM_FSIM_TL_RB", 2),
+                        hist + (long)p0 * 8 * kHistBins, Px.tw, hd, np, env_int("IQM_FSIM_TL_RB", tune.row_rb),
                         invN, tl_budget};
             tl_rowpass(*this, wd, a);
         } else {

@@ -201,3 +201,25 @@ def test_graph_replay_matches_eager(ctx):
         ctx.free(di)
         ctx.free(dr)
         ctx.free(dm)
+
+
+def test_autotune_and_plan_file(ctx):
+    """The autotuner picks tiles for a size, the plan file round-trips, and
+    the tuned launch sequence gives the same scores."""
+    import json
+
+    img, ref = synth.rgb_pair(1920, 1080, seed=77)
+    before = iqm_amd.FSIM().compute_metric(ctx, img, ref)
+    us = ctx.autotune_fsim(1920, 1080, 3, 2)
+    assert us > 0
+    plan = json.loads(ctx.export_tunes())
+    entry = [e for e in plan["fsim"] if (e["wd"], e["hd"]) == (480, 270)][0]
+    assert entry["col_bx"] in (8, 16) and entry["row_rb"] in (1, 2, 4)
+    after = iqm_amd.FSIM().compute_metric(ctx, img, ref)
+    assert abs(after.fsim - before.fsim) < 1e-7 and abs(after.fsimc - before.fsimc) < 1e-7
+    ctx.import_tunes(json.dumps({"fsim": [dict(entry, col_bx=8, row_rb=1, fft_rb=2)]}))
+    again = json.loads(ctx.export_tunes())["fsim"]
+    assert [e for e in again if (e["wd"], e["hd"]) == (480, 270)][0]["col_bx"] == 8
+    r = iqm_amd.FSIM().compute_metric(ctx, img, ref)
+    assert abs(r.fsim - before.fsim) < 1e-7
+    ctx.import_tunes(json.dumps({"fsim": [entry]}))
