@@ -125,6 +125,52 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
                        int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                        float* d_simc_maps, double* fsim_out, double* fsimc_out);
 
+/* ---- learned phase-congruency surrogate -------------------------------- */
+/* No counterpart in the reference: a 6-layer 3x3 convolutional network
+ * (32 channels, ReLU, BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) from the downscaled luminance Y/255 to the PC map of F15
+ * (fsim_phase_congruency.glsl:26-64), trained on this device against exact
+ * PC.  A surrogate belongs to one context and must be destroyed before it. */
+typedef struct iqm_sur iqm_sur;
+
+/* Number of FP32 parameters (OIHW weights and biases of the 6 layers). */
+int32_t iqm_sur_param_count(void);
+/* New network with He-uniform weights from `seed` (SplitMix64) and zero
+ * biases and Adam moments. */
+int iqm_sur_create(iqm_ctx* ctx, uint64_t seed, iqm_sur** out);
+void iqm_sur_destroy(iqm_sur* s);
+int iqm_sur_get_params(iqm_sur* s, float* host_params);
+int iqm_sur_set_params(iqm_sur* s, const float* host_params);
+/* Gradient of the last iqm_sur_loss_grad / iqm_sur_train_step. */
+int iqm_sur_get_grad(iqm_sur* s, float* host_grad);
+/* Adam steps taken and training images consumed (the resume cursor). */
+int iqm_sur_state(iqm_sur* s, int64_t* step, uint64_t* data_cursor);
+/* n seeded size x size training images (even n: images 2i and 2i+1 form an
+ * FSIM pair; 1/f^2 fields, half of them distorted): d_x gets Y/255 and
+ * d_pc the exact PC, n*size*size device floats each. */
+int iqm_sur_make_patches(iqm_sur* s, int32_t n, int32_t size, uint64_t seed, float* d_x, float* d_pc);
+/* Mean squared error over n images of h x w (device) and its gradient. */
+int iqm_sur_loss_grad(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
+                      int32_t w, double* loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr. */
+int iqm_sur_train_step(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
+                       int32_t w, float lr, double* loss);
+/* Network output for n images of h x w (device in, device out). */
+int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w);
+/* Checkpoint (weights, Adam moments, step, seed, cursor) as a byte blob;
+ * returns its length and writes it when cap allows.  -1 on error. */
+int64_t iqm_sur_save(iqm_sur* s, uint8_t* buf, int64_t cap);
+int iqm_sur_load(iqm_sur* s, const uint8_t* buf, int64_t len);
+/* FSIM/FSIMc as iqm_fsim_batch_dev with PC from the network.  Pairs with
+ * i % spot_every == 0 (spot_every > 0) are also run exactly; spot_err gets the
+ * largest |PC_net - PC_exact| over 4 seeded 32x32 tiles of their images, and
+ * when it exceeds spot_tol the batch is recomputed exactly (*fell_back = 1).
+ * spot_err and fell_back are nullable. */
+int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img, const uint8_t* d_ref,
+                                 int32_t w, int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                                 int32_t spot_every, float spot_tol, double* fsim_out,
+                                 double* fsimc_out, float* spot_err, int32_t* fell_back);
+
 /* ---- FFT (replaces VkFFT: fsim.cpp:370-437 plan, :527-550 append) ------- */
 
 /* In-place batched 2-D C2C FFT of `batch` interleaved complex float planes

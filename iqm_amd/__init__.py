@@ -34,6 +34,10 @@ EXPORTS = (
     "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
     "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_tune_export",
     "iqm_tune_import", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
+    "iqm_sur_param_count", "iqm_sur_create", "iqm_sur_destroy", "iqm_sur_get_params",
+    "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches",
+    "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
+    "iqm_fsim_surrogate_batch_dev",
 )
 
 
@@ -109,6 +113,26 @@ def lib():
     L.iqm_stage_times.restype = i32
     L.iqm_reset_stage_times.argtypes = [vp]
     L.iqm_reset_stage_times.restype = None
+    u64 = ctypes.c_uint64
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.iqm_sur_param_count.argtypes = []
+    L.iqm_sur_param_count.restype = i32
+    L.iqm_sur_create.argtypes = [vp, u64, ctypes.POINTER(vp)]
+    L.iqm_sur_destroy.argtypes = [vp]
+    L.iqm_sur_destroy.restype = None
+    L.iqm_sur_get_params.argtypes = [vp, fp]
+    L.iqm_sur_set_params.argtypes = [vp, fp]
+    L.iqm_sur_get_grad.argtypes = [vp, fp]
+    L.iqm_sur_state.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(u64)]
+    L.iqm_sur_make_patches.argtypes = [vp, i32, i32, u64, vp, vp]
+    L.iqm_sur_loss_grad.argtypes = [vp, vp, vp, i32, i32, i32, dp]
+    L.iqm_sur_train_step.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, dp]
+    L.iqm_sur_infer.argtypes = [vp, vp, vp, i32, i32, i32]
+    L.iqm_sur_save.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
+    L.iqm_sur_save.restype = i64
+    L.iqm_sur_load.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
+    L.iqm_fsim_surrogate_batch_dev.argtypes = [vp, vp, vp, vp, i32, i32, i32, i64, i32, i32,
+                                               ctypes.c_float, dp, dp, fp, ctypes.POINTER(i32)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -333,3 +357,100 @@ class FSIM:
         _check(lib().iqm_fsim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
                                         d_simc_maps, _dptr(fs), _dptr(fc)))
         return fs, fc
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Surrogate:
+    """Learned phase-congruency network (no counterpart in the reference).
+
+    A 6-layer 3x3 convolutional network, 32 channels, from Y/255 on FSIM's
+    downscaled grid to the PC map; BF16 on the matrix cores, trained on the
+    device with Adam.  Parameters are one flat FP32 vector in PyTorch OIHW
+    order: w1[32,1,3,3], b1[32], 4 x (w[32,32,3,3], b[32]), w6[1,32,3,3], b6[1].
+    """
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        h = ctypes.c_void_p()
+        _check(lib().iqm_sur_create(ctx.handle, int(seed), ctypes.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().iqm_sur_param_count())
+
+    def close(self) -> None:
+        if self.handle:
+            lib().iqm_sur_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.param_count(), np.float32)
+        _check(lib().iqm_sur_get_params(self.handle, _fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.param_count():
+            raise ValueError("parameter vector has the wrong length")
+        _check(lib().iqm_sur_set_params(self.handle, _fptr(p)))
+
+    def grad(self) -> np.ndarray:
+        g = np.zeros(self.param_count(), np.float32)
+        _check(lib().iqm_sur_get_grad(self.handle, _fptr(g)))
+        return g
+
+    def state(self) -> tuple[int, int]:
+        st, cur = ctypes.c_int64(), ctypes.c_uint64()
+        _check(lib().iqm_sur_state(self.handle, ctypes.byref(st), ctypes.byref(cur)))
+        return st.value, cur.value
+
+    def make_patches(self, n: int, size: int, seed: int, d_x: int, d_pc: int) -> None:
+        _check(lib().iqm_sur_make_patches(self.handle, n, size, int(seed), d_x, d_pc))
+
+    def loss_grad(self, d_x: int, d_t: int, n: int, h: int, w: int) -> float:
+        loss = np.zeros(1, np.float64)
+        _check(lib().iqm_sur_loss_grad(self.handle, d_x, d_t, n, h, w, _dptr(loss)))
+        return float(loss[0])
+
+    def train_step(self, d_x: int, d_t: int, n: int, h: int, w: int, lr: float) -> float:
+        loss = np.zeros(1, np.float64)
+        _check(lib().iqm_sur_train_step(self.handle, d_x, d_t, n, h, w, float(lr), _dptr(loss)))
+        return float(loss[0])
+
+    def infer(self, d_x: int, d_out: int, n: int, h: int, w: int) -> None:
+        _check(lib().iqm_sur_infer(self.handle, d_x, d_out, n, h, w))
+
+    def save(self) -> bytes:
+        n = int(lib().iqm_sur_save(self.handle, None, 0))
+        if n < 0:
+            raise IqmError(IQM_ERR_INVALID, lib().iqm_last_error().decode())
+        buf = (ctypes.c_uint8 * n)()
+        if lib().iqm_sur_save(self.handle, buf, n) != n:
+            raise IqmError(IQM_ERR_INVALID, lib().iqm_last_error().decode())
+        return bytes(buf)
+
+    def load(self, blob: bytes) -> None:
+        buf = (ctypes.c_uint8 * len(blob)).from_buffer_copy(blob)
+        _check(lib().iqm_sur_load(self.handle, buf, len(blob)))
+
+    def fsim_batch_dev(self, d_img: int, d_ref: int, w: int, h: int, ch: int, pair_stride: int,
+                       n: int, spot_every: int = 0, spot_tol: float = 0.02):
+        """FSIM/FSIMc with PC from the network; returns (fsim, fsimc, spot_err, fell_back)."""
+        fs = np.zeros(n, np.float64)
+        fc = np.zeros(n, np.float64)
+        err = np.zeros(1, np.float32)
+        back = ctypes.c_int32()
+        _check(lib().iqm_fsim_surrogate_batch_dev(self.ctx.handle, self.handle, d_img, d_ref, w, h, ch,
+                                                  pair_stride, n, spot_every, float(spot_tol),
+                                                  _dptr(fs), _dptr(fc), _fptr(err), ctypes.byref(back)))
+        return fs, fc, float(err[0]), bool(back.value)

@@ -1,5 +1,6 @@
 // Context implementation and the extern "C" entry points of include/iqm/iqm.h.
 #include "ctx.hpp"
+#include "surrogate.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -402,6 +403,162 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
                 if (fsim_out) fsim_out[i] = res[(size_t)8 * i + 3];
                 if (fsimc_out) fsimc_out[i] = res[(size_t)8 * i + 4];
             }
+        }
+    });
+}
+
+struct iqm_sur {
+    iqm_ctx* ctx;
+    iqm::Surrogate* s;
+};
+
+int32_t iqm_sur_param_count(void) { return iqm::SurLayout::total; }
+
+int iqm_sur_create(iqm_ctx* ctx, uint64_t seed, iqm_sur** out) {
+    if (!ctx || !out) return fail(IQM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        IQM_HIP(hipSetDevice(ctx->c->device()));
+        auto* h = new iqm_sur{ctx, nullptr};
+        try {
+            h->s = new iqm::Surrogate(*ctx->c, seed);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void iqm_sur_destroy(iqm_sur* s) {
+    if (!s) return;
+    guarded([&] { delete s->s; });
+    delete s;
+}
+
+#define SUR_GUARD(s) \
+    if (!(s)) return fail(IQM_ERR_INVALID, "null surrogate")
+
+int iqm_sur_get_params(iqm_sur* s, float* host_params) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(host_params, IQM_ERR_INVALID, "null buffer");
+        s->s->get_params(host_params);
+    });
+}
+
+int iqm_sur_set_params(iqm_sur* s, const float* host_params) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(host_params, IQM_ERR_INVALID, "null buffer");
+        s->ctx->c->sync();
+        s->s->set_params(host_params);
+        s->ctx->c->sync();
+    });
+}
+
+int iqm_sur_get_grad(iqm_sur* s, float* host_grad) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(host_grad, IQM_ERR_INVALID, "null buffer");
+        s->s->get_grad(host_grad);
+    });
+}
+
+int iqm_sur_state(iqm_sur* s, int64_t* step, uint64_t* data_cursor) {
+    SUR_GUARD(s);
+    if (step) *step = s->s->m.step;
+    if (data_cursor) *data_cursor = s->s->m.data_cursor;
+    return IQM_OK;
+}
+
+int iqm_sur_make_patches(iqm_sur* s, int32_t n, int32_t size, uint64_t seed, float* d_x, float* d_pc) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(d_x && d_pc, IQM_ERR_INVALID, "null buffer");
+        IQM_CHECK(size <= 4096, IQM_ERR_INVALID, "patch size above 4096");
+        s->s->make_patches(n, size, seed, d_x, d_pc);
+    });
+}
+
+static void sur_check_shape(const float* a, const float* b, int n, int h, int w) {
+    IQM_CHECK(a && b, IQM_ERR_INVALID, "null buffer");
+    IQM_CHECK(n > 0 && h > 0 && w > 0, IQM_ERR_INVALID, "invalid image count or size");
+}
+
+int iqm_sur_loss_grad(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
+                      int32_t w, double* loss) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        sur_check_shape(d_x, d_target, n, h, w);
+        const double l = s->s->loss_and_grad(d_x, d_target, n, h, w);
+        if (loss) *loss = l;
+    });
+}
+
+int iqm_sur_train_step(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
+                       int32_t w, float lr, double* loss) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        sur_check_shape(d_x, d_target, n, h, w);
+        IQM_CHECK(lr > 0.f && std::isfinite(lr), IQM_ERR_INVALID, "learning rate must be positive");
+        const double l = s->s->train_step(d_x, d_target, n, h, w, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        sur_check_shape(d_x, d_out, n, h, w);
+        s->s->infer(d_x, d_out, n, h, w);
+        s->ctx->c->sync();
+    });
+}
+
+int64_t iqm_sur_save(iqm_sur* s, uint8_t* buf, int64_t cap) {
+    if (!s) return -1;
+    int64_t len = -1;
+    if (guarded([&] {
+            const std::vector<uint8_t> b = s->s->save();
+            len = (int64_t)b.size();
+            if (buf && cap >= len) std::memcpy(buf, b.data(), b.size());
+        }) != IQM_OK)
+        return -1;
+    return len;
+}
+
+int iqm_sur_load(iqm_sur* s, const uint8_t* buf, int64_t len) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(buf && len > 0, IQM_ERR_INVALID, "null checkpoint");
+        s->s->load(std::vector<uint8_t>(buf, buf + len));
+    });
+}
+
+int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img, const uint8_t* d_ref,
+                                 int32_t w, int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                                 int32_t spot_every, float spot_tol, double* fsim_out,
+                                 double* fsimc_out, float* spot_err, int32_t* fell_back) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(s->ctx == ctx, IQM_ERR_INVALID, "surrogate belongs to another context");
+        IQM_CHECK(d_img && d_ref && w > 0 && h > 0 && n > 0 && spot_every >= 0, IQM_ERR_INVALID,
+                  "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        iqm::Ctx& c = *ctx->c;
+        int back = 0;
+        double* dres = s->s->fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, spot_tol,
+                                        spot_err, &back);
+        if (fell_back) *fell_back = back;
+        std::vector<double> res((size_t)8 * n);
+        IQM_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
+        c.sync();
+        for (int i = 0; i < n; ++i) {
+            if (fsim_out) fsim_out[i] = res[(size_t)8 * i + 3];
+            if (fsimc_out) fsimc_out[i] = res[(size_t)8 * i + 4];
         }
     });
 }

@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 
+#include <functional>
 #include <map>
 #include <tuple>
 #include <string>
@@ -83,6 +84,12 @@ class Ctx {
      * [sum pcm, sum sim, sum simc, fsim, fsimc, -, -, -]. */
     double* fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                        long pair_stride, int n, const FsimBatchOut& out);
+    /* FSIM with phase congruency supplied by pc_fn instead of the FFT stages:
+     * pc_fn(yiq, N, n, pc) fills pc[pair][2][N] from the Y planes (plane 0 and
+     * 1 of every 6-plane pair block of yiq).  Results as fsim_batch. */
+    using PcFn = std::function<void(const float* yiq, long N, int n, float* pc)>;
+    double* fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                            long pair_stride, int n, const FsimBatchOut& out, const PcFn& pc_fn);
     /* pointer to the device workspace of the last fsim_batch (stage buffers) */
     struct FsimLayout {
         long N;
@@ -184,7 +191,7 @@ class Ctx {
 };
 
 /* workspace slots */
-enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP };
+enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR };
 
 size_t line_lds_bytes(const LinePlan& P, int lines, int bufs);
 int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap);

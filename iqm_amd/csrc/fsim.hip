@@ -578,7 +578,7 @@ constexpr int kPcTx = 64, kPcTy = 4;
 __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
     const float* __restrict__ yiq, float* __restrict__ grad, const float* __restrict__ an_e,
     const float* __restrict__ thr, double* __restrict__ partial, int wd, int hd, float* simc_map,
-    float* pc_map, float* pcm_map, float* sim_map) {
+    float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src) {
     constexpr int HX = kPcTx + 2, HY = kPcTy + 2;
     __shared__ float ys[2][HY][HX];
     __shared__ double red[3][kBlock / 64];
@@ -617,10 +617,11 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
             grad[(long)pair * 2 * N + idx] = gm[0];
             grad[(long)pair * 2 * N + N + idx] = gm[1];
         }
-        const float* ae = an_e + (long)pair * 16 * N;
-        const float* T = thr + (long)pair * 8;
         float pc[2];
-        for (int im = 0; im < 2; ++im) {
+        for (int im = 0; im < 2 && pc_src; ++im) pc[im] = pc_src[((long)pair * 2 + im) * N + idx];
+        for (int im = 0; im < 2 && !pc_src; ++im) {
+            const float* ae = an_e + (long)pair * 16 * N;
+            const float* T = thr + (long)pair * 8;
             float et = 0.f, at = 0.f;
             for (int o = 0; o < 4; ++o) {
                 const int io = im * 4 + o;
@@ -868,10 +869,43 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
            dim3(kBlock), 0, (const float*)yiq,
            out.want_grad ? grad : (float*)nullptr, (const float*)an_e, (const float*)thr, partial, wd,
-           hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim);
+           hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)nullptr);
     launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
            nblk, res);
     return res;
 }
 
+double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                             long pair_stride, int n, const FsimBatchOut& out, const PcFn& pc_fn) {
+    int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+    const int wd = (int)std::round((float)w / (float)F);
+    const int hd = (int)std::round((float)h / (float)F);
+    IQM_CHECK(wd >= 2 && hd >= 2, IQM_ERR_TOO_SMALL, "image too small for FSIM");
+    const long N = (long)wd * hd;
+    const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += align256(b); return o; };
+    const size_t o_yiq = take(sizeof(float) * 6 * N * n), o_pc = take(sizeof(float) * 2 * N * n),
+                 o_hist = take(sizeof(unsigned) * 8 * kHistBins * n), o_part = take(sizeof(double) * 4 * (size_t)nblk * n),
+                 o_res = take(sizeof(double) * 8 * n);
+    char* base = (char*)workspace(WS_SUR, off);
+    float* yiq = (float*)(base + o_yiq);
+    float* pc = (float*)(base + o_pc);
+    double* partial = (double*)(base + o_part);
+    double* res = (double*)(base + o_res);
+    {
+        const size_t stage_bytes = (size_t)F * prep_row_chunks(kPrepCols * F * ch) * 16;
+        IQM_CHECK(stage_bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+        launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, kPrepCols), hd, n), dim3(kBlock), stage_bytes,
+               d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, (unsigned*)(base + o_hist));
+    }
+    pc_fn(yiq, N, n, pc);
+    launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
+           dim3(kBlock), 0, (const float*)yiq, (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
+           partial, wd, hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)pc);
+    launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial, nblk, res);
+    return res;
+}
+
 }  // namespace iqm
+

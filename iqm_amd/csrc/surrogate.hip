@@ -1,0 +1,873 @@
+// Learned phase-congruency surrogate (see surrogate.hpp).
+//
+// Layers 2-5 (32 -> 32 channels, 3x3) run as implicit GEMMs on the gfx950
+// BF16 matrix cores (v_mfma_f32_16x16x32_bf16): a wavefront computes 16
+// output pixels x 16 output channels per instruction chain, one instruction
+// per tap covering the tap's 32 input channels (K = 32).  The input gradient
+// of these layers reuses the same kernel with the flipped, transposed weights
+// and the ReLU mask of the layer below.  Layers 1 and 6 (1 <-> 32 channels)
+// and the weight gradients use FP32 FMA with register tiling; all
+// accumulation is FP32, weight-gradient partials are summed in a fixed order.
+#include "surrogate.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace iqm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t u = __float_as_uint(f);
+#else
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+#endif
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u); /* round to nearest even */
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+/* ---- forward ----------------------------------------------------------- */
+
+/* Layer 1: 1 -> 32 channels, FP32 FMA, ReLU.  Image j of the input is read
+ * at x + (j/2)*pstride + (j%2)*istride, times `scale` (so FSIM's Y planes can
+ * be read in place). */
+__global__ __launch_bounds__(kBlock) void sur_conv_first(const float* __restrict__ x, long pstride,
+                                                         long istride, float scale,
+                                                         const float* __restrict__ params,
+                                                         uint16_t* __restrict__ out, int nimg, int h,
+                                                         int w) {
+    __shared__ float wsh[32 * 9 + 32];
+    for (int i = threadIdx.x; i < 32 * 9 + 32; i += blockDim.x) {
+        const float v = params[SurLayout::w1 + i];
+        wsh[i] = i < 32 * 9 ? bf2f(f2bf(v)) : v; /* BF16 weights, FP32 bias */
+    }
+    __syncthreads();
+    const long HW = (long)h * w;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nimg * HW) return;
+    const int j = (int)(p / HW);
+    const long q = p - (long)j * HW;
+    const int y = (int)(q / w), xx = (int)(q - (long)y * w);
+    const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
+        v[t] = (yy >= 0 && yy < h && xc >= 0 && xc < w) ? xi[(long)yy * w + xc] * scale : 0.f;
+    }
+    uint32_t packed[16];
+#pragma unroll
+    for (int co = 0; co < 32; co += 2) {
+        float s0 = wsh[32 * 9 + co], s1 = wsh[32 * 9 + co + 1];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            s0 += wsh[co * 9 + t] * v[t];
+            s1 += wsh[(co + 1) * 9 + t] * v[t];
+        }
+        packed[co / 2] = (uint32_t)f2bf(fmaxf(s0, 0.f)) | ((uint32_t)f2bf(fmaxf(s1, 0.f)) << 16);
+    }
+    uint4* o = (uint4*)(out + p * 32);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        o[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+}
+
+/* Layers 2-5 (and their input gradient): 3x3, 32 -> 32, NHWC BF16, MFMA.
+ * A 16 x 16 pixel tile per workgroup; wavefront v owns tile rows 4v..4v+3.
+ * FWD: out = relu(conv(in, W) + b).  !FWD: out = conv(in, W) * (mask > 0)
+ * (the input gradient through the ReLU of the layer below, with W the
+ * flipped, transposed weights). */
+template <bool FWD>
+__global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restrict__ in,
+                                                       const uint16_t* __restrict__ wgt,
+                                                       const float* __restrict__ bias,
+                                                       const uint16_t* __restrict__ mask,
+                                                       uint16_t* __restrict__ out, int h, int w) {
+    __shared__ __align__(16) uint16_t tin[18 * 18 * 32];
+    __shared__ __align__(16) uint16_t tw[9 * 32 * 32];
+    __shared__ __align__(16) uint16_t tout[16 * 16 * 32];
+    const int img = blockIdx.z, x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+    const long HW = (long)h * w;
+    const uint16_t* src = in + (long)img * HW * 32;
+    /* weights OIHW [co][ci][tap] -> [tap][co][ci] */
+    for (int i = threadIdx.x; i < 9 * 32 * 32; i += blockDim.x) {
+        const int co = i / 288, r = i - co * 288, ci = r / 9, t = r - ci * 9;
+        tw[(t * 32 + co) * 32 + ci] = wgt[i];
+    }
+    /* input tile with a one-pixel zero halo, 16-byte chunks of 8 channels */
+    for (int i = threadIdx.x; i < 18 * 18 * 4; i += blockDim.x) {
+        const int px = i >> 2, part = i & 3;
+        const int ly = px / 18, lx = px - ly * 18;
+        const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = ((const uint4*)(src + ((long)gy * w + gx) * 32))[part];
+        ((uint4*)tin)[i] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    bf16x8 B[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&tw[(t * 32 + nb * 16 + li) * 32 + 8 * g];
+    float b0 = 0.f, b1 = 0.f;
+    if (FWD) {
+        b0 = bias[li];
+        b1 = bias[16 + li];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int r = wave * 4 + rr;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;
+            const bf16x8 A = *(const bf16x8*)&tin[((r + dy) * 18 + li + dx) * 32 + 8 * g];
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][0], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][1], acc1, 0, 0, 0);
+        }
+        /* D layout: lane holds pixel rows 4g..4g+3 of column li */
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int px = 4 * g + e;
+            float v0 = acc0[e], v1 = acc1[e];
+            if (FWD) {
+                v0 = fmaxf(v0 + b0, 0.f);
+                v1 = fmaxf(v1 + b1, 0.f);
+            }
+            tout[(r * 16 + px) * 32 + li] = f2bf(v0);
+            tout[(r * 16 + px) * 32 + 16 + li] = f2bf(v1);
+        }
+    }
+    __syncthreads();
+    uint16_t* dst = out + (long)img * HW * 32;
+    const uint16_t* msk = mask ? mask + (long)img * HW * 32 : nullptr;
+    for (int i = threadIdx.x; i < 16 * 16 * 32; i += blockDim.x) {
+        const int px = i >> 5, c = i & 31;
+        const int ly = px >> 4, lx = px & 15;
+        const int gy = y0 + ly, gx = x0 + lx;
+        if (gy >= h || gx >= w) continue;
+        const long o = ((long)gy * w + gx) * 32 + c;
+        uint16_t v = tout[i];
+        if (!FWD && bf2f(msk[o]) <= 0.f) v = 0;
+        dst[o] = v;
+    }
+}
+
+/* Layer 6: 32 -> 1 channel, FP32 accumulation, no activation. */
+__global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restrict__ a5,
+                                                        const float* __restrict__ params,
+                                                        float* __restrict__ out, int nimg, int h, int w) {
+    __shared__ float wsh[32 * 9 + 1];
+    for (int i = threadIdx.x; i < 32 * 9 + 1; i += blockDim.x) {
+        const float v = params[SurLayout::w6 + i];
+        wsh[i] = i < 32 * 9 ? bf2f(f2bf(v)) : v;
+    }
+    __syncthreads();
+    const long HW = (long)h * w;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nimg * HW) return;
+    const int j = (int)(p / HW);
+    const long q = p - (long)j * HW;
+    const int y = (int)(q / w), x = (int)(q - (long)y * w);
+    float s = wsh[32 * 9];
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+        if (yy < 0 || yy >= h || xx < 0 || xx >= w) continue;
+        const uint4* src = (const uint4*)(a5 + ((long)j * HW + (long)yy * w + xx) * 32);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = src[k];
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int ci = 8 * k + 2 * m;
+                s += wsh[ci * 9 + t] * bf2f((uint16_t)(u[m] & 0xffffu));
+                s += wsh[(ci + 1) * 9 + t] * bf2f((uint16_t)(u[m] >> 16));
+            }
+        }
+    }
+    out[p] = s;
+}
+
+/* ---- backward ---------------------------------------------------------- */
+
+/* dpred = 2 (pred - target) / P and per-block FP64 partial sums of the
+ * squared error. */
+__global__ __launch_bounds__(kBlock) void sur_loss(const float* __restrict__ pred,
+                                                   const float* __restrict__ target,
+                                                   float* __restrict__ dpred, double* __restrict__ part,
+                                                   long P) {
+    __shared__ double sh[kBlock];
+    double acc = 0;
+    const float sc = 2.0f / (float)P;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+        const float d = pred[p] - target[p];
+        dpred[p] = sc * d;
+        acc += (double)d * (double)d;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void sur_loss_final(const double* part, int n, long P, double* loss) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        *loss = s / (double)P;
+    }
+}
+
+/* Input gradient of layer 6 through the ReLU of layer 5:
+ * dz5[p][ci] = (a5[p][ci] > 0) * sum_tap dpred[p - tap] * w6[ci][tap]. */
+__global__ __launch_bounds__(kBlock) void sur_dlast(const float* __restrict__ dpred,
+                                                    const float* __restrict__ params,
+                                                    const uint16_t* __restrict__ a5,
+                                                    uint16_t* __restrict__ dz, int nimg, int h, int w) {
+    __shared__ float wsh[32 * 9];
+    for (int i = threadIdx.x; i < 32 * 9; i += blockDim.x) wsh[i] = bf2f(f2bf(params[SurLayout::w6 + i]));
+    __syncthreads();
+    const long HW = (long)h * w;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nimg * HW) return;
+    const int j = (int)(p / HW);
+    const long q = p - (long)j * HW;
+    const int y = (int)(q / w), x = (int)(q - (long)y * w);
+    float d[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        /* output pixel o = p - offset(t) contributes w[t] to input pixel p */
+        const int yy = y - (t / 3 - 1), xx = x - (t % 3 - 1);
+        d[t] = (yy >= 0 && yy < h && xx >= 0 && xx < w) ? dpred[(long)j * HW + (long)yy * w + xx] : 0.f;
+    }
+    const uint16_t* a = a5 + p * 32;
+    uint16_t* o = dz + p * 32;
+    for (int ci = 0; ci < 32; ++ci) {
+        float s = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) s += d[t] * wsh[ci * 9 + t];
+        o[ci] = bf2f(a[ci]) > 0.f ? f2bf(s) : (uint16_t)0;
+    }
+}
+
+/* Weight gradient of a 32 -> 32 layer: dW[co][ci][tap] = sum_p dz[p][co] *
+ * a[p + offset(tap)][ci] and db[co] = sum_p dz[p][co].  A workgroup of 288
+ * threads walks 16 x 16 tiles (grid-stride); thread (tap, co block of 8, ci
+ * block of 4) keeps 32 FP32 accumulators and writes one partial row. */
+__global__ __launch_bounds__(288) void sur_dw_mid(const uint16_t* __restrict__ dz,
+                                                  const uint16_t* __restrict__ a,
+                                                  float* __restrict__ part, int nimg, int h, int w) {
+    __shared__ __align__(16) uint16_t tdz[16 * 16 * 32];
+    __shared__ __align__(16) uint16_t ta[18 * 18 * 32];
+    const int tid = threadIdx.x;
+    const int t = tid / 32, cb = (tid % 32) / 8, ib = tid % 8; /* tap, co block, ci block */
+    const int dy = t / 3, dx = t % 3;
+    float acc[8][4];
+    for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 4; ++k) acc[i][k] = 0.f;
+    float db = 0.f;
+    const long HW = (long)h * w;
+    const int tx = (w + 15) / 16, ty = (h + 15) / 16;
+    const long ntiles = (long)nimg * tx * ty;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int img = (int)(tile / (tx * ty));
+        const int rem = (int)(tile - (long)img * tx * ty);
+        const int y0 = (rem / tx) * 16, x0 = (rem % tx) * 16;
+        __syncthreads();
+        for (int i = tid; i < 16 * 16 * 4; i += blockDim.x) {
+            const int px = i >> 2, part4 = i & 3;
+            const int gy = y0 + (px >> 4), gx = x0 + (px & 15);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gy < h && gx < w) v = ((const uint4*)(dz + ((long)img * HW + (long)gy * w + gx) * 32))[part4];
+            ((uint4*)tdz)[i] = v;
+        }
+        for (int i = tid; i < 18 * 18 * 4; i += blockDim.x) {
+            const int px = i >> 2, part4 = i & 3;
+            const int ly = px / 18, lx = px - ly * 18;
+            const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = ((const uint4*)(a + ((long)img * HW + (long)gy * w + gx) * 32))[part4];
+            ((uint4*)ta)[i] = v;
+        }
+        __syncthreads();
+        for (int px = 0; px < 256; ++px) {
+            const int ly = px >> 4, lx = px & 15;
+            const uint4 dv = *(const uint4*)&tdz[px * 32 + cb * 8];
+            const uint2 av = *(const uint2*)&ta[((ly + dy) * 18 + lx + dx) * 32 + ib * 4];
+            const uint32_t du[4] = {dv.x, dv.y, dv.z, dv.w};
+            const uint32_t au[2] = {av.x, av.y};
+            float dzf[8], af[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                dzf[2 * k] = __uint_as_float(du[k] << 16);
+                dzf[2 * k + 1] = __uint_as_float(du[k] & 0xffff0000u);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                af[2 * k] = __uint_as_float(au[k] << 16);
+                af[2 * k + 1] = __uint_as_float(au[k] & 0xffff0000u);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[i][k] += dzf[i] * af[k];
+        }
+        if (tid < 32) {
+            for (int px = 0; px < 256; ++px) db += bf2f(tdz[px * 32 + tid]);
+        }
+    }
+    /* partial row: [32][32][9] weights then [32] biases */
+    float* pr = part + (long)blockIdx.x * (32 * 32 * 9 + 32);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pr[((cb * 8 + i) * 32 + ib * 4 + k) * 9 + t] = acc[i][k];
+    if (tid < 32) pr[32 * 32 * 9 + tid] = db;
+}
+
+/* Weight gradients of layer 1 (1 -> 32) and layer 6 (32 -> 1) and their
+ * biases.  Thread (lane q, channel c): c = tid % 32, pixels p = q, q + 8, ...
+ * of the workgroup's grid-stride share; 9 + 9 FP32 accumulators per thread,
+ * folded over the 8 lanes in LDS into one partial row per workgroup. */
+__global__ __launch_bounds__(kBlock) void sur_dw_edge(const float* __restrict__ x, long pstride,
+                                                      long istride, float scale,
+                                                      const uint16_t* __restrict__ dz1,
+                                                      const uint16_t* __restrict__ a5,
+                                                      const float* __restrict__ dpred,
+                                                      float* __restrict__ part1,
+                                                      float* __restrict__ part6, int nimg, int h,
+                                                      int w) {
+    __shared__ float red[8][32][20];
+    const int c = threadIdx.x & 31, q = threadIdx.x >> 5;
+    const long HW = (long)h * w, P = (long)nimg * HW;
+    float g1[9], g6[9], gb1 = 0.f, gb6 = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) g1[t] = g6[t] = 0.f;
+    for (long p = (long)blockIdx.x * 8 + q; p < P; p += (long)gridDim.x * 8) {
+        const int j = (int)(p / HW);
+        const long r = p - (long)j * HW;
+        const int y = (int)(r / w), xx = (int)(r - (long)y * w);
+        const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
+        const float d1 = bf2f(dz1[p * 32 + c]);
+        const float d6 = dpred[p];
+        gb1 += d1;
+        gb6 += d6;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
+            if (yy < 0 || yy >= h || xc < 0 || xc >= w) continue;
+            g1[t] += d1 * (xi[(long)yy * w + xc] * scale);
+            g6[t] += d6 * bf2f(a5[((long)j * HW + (long)yy * w + xc) * 32 + c]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        red[q][c][t] = g1[t];
+        red[q][c][9 + t] = g6[t];
+    }
+    red[q][c][18] = gb1;
+    red[q][c][19] = gb6;
+    __syncthreads();
+    if (q == 0) {
+        float s[20];
+#pragma unroll
+        for (int k = 0; k < 20; ++k) s[k] = 0.f;
+        for (int l = 0; l < 8; ++l)
+#pragma unroll
+            for (int k = 0; k < 20; ++k) s[k] += red[l][c][k];
+        float* p1 = part1 + (long)blockIdx.x * (288 + 32);
+        float* p6 = part6 + (long)blockIdx.x * (288 + 1);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            p1[c * 9 + t] = s[t];
+            p6[c * 9 + t] = s[9 + t];
+        }
+        p1[288 + c] = s[18];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        /* every channel lane saw the same dpred values: take channel 0's sum */
+        float b6 = 0.f;
+        for (int l = 0; l < 8; ++l) b6 += red[l][0][19];
+        part6[(long)blockIdx.x * (288 + 1) + 288] = b6;
+    }
+}
+
+/* grad[dst + k] = sum_b part[b * stride + k], fixed order */
+__global__ __launch_bounds__(kBlock) void sur_reduce(const float* __restrict__ part, int nb,
+                                                     int count, int stride, float* __restrict__ grad,
+                                                     int dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += (double)part[(long)b * stride + k];
+    grad[dst + k] = (float)s;
+}
+
+/* Adam (Kingma & Ba) with FP32 master weights. */
+__global__ __launch_bounds__(kBlock) void sur_adam(float* __restrict__ w, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, int n,
+                                                   float lr, float b1, float b2, float eps, float c1,
+                                                   float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+/* BF16 copies of the mid-layer weights: OIHW and the flipped transpose
+ * Wt[ci][co][2-ky][2-kx] used for the input gradient. */
+__global__ __launch_bounds__(kBlock) void sur_cast(const float* __restrict__ master,
+                                                   uint16_t* __restrict__ wbf, uint16_t* __restrict__ wbt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * 9216) return;
+    const int l = i / 9216, r = i - l * 9216;
+    const int co = r / 288, r2 = r - co * 288, ci = r2 / 9, t = r2 - ci * 9;
+    const uint16_t b = f2bf(master[SurLayout::wl(l + 2) + r]);
+    wbf[i] = b;
+    wbt[l * 9216 + (ci * 32 + co) * 9 + (8 - t)] = b;
+}
+
+/* ---- training data ----------------------------------------------------- */
+
+__device__ __forceinline__ uint64_t sm64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float u01(uint64_t z) { return ((float)(sm64(z) >> 40) + 0.5f) * (1.0f / 16777216.0f); }
+
+/* white N(0,1) noise planes (Box-Muller on counter-based uniforms) */
+__global__ __launch_bounds__(kBlock) void sur_noise(float2* planes, long per, int n, uint64_t seed) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per * n) return;
+    const uint64_t base = seed * 0x632BE59BD9B4E019ull + (uint64_t)i * 2;
+    const float u1 = u01(base), u2 = u01(base + 1);
+    planes[i] = make_float2(sqrtf(-2.f * logf(u1)) * cosf(6.2831853f * u2), 0.f);
+}
+
+/* multiply spectra by 1/|f| on the fftfreq grid (DC := 0): power ~ 1/f^2 */
+__global__ __launch_bounds__(kBlock) void sur_pink(float2* planes, int s, int n) {
+    const long per = (long)s * s;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per * n) return;
+    const long q = i % per;
+    const int y = (int)(q / s), x = (int)(q % s);
+    const float fy = (float)(y < (s + 1) / 2 ? y : y - s) / s, fx = (float)(x < (s + 1) / 2 ? x : x - s) / s;
+    const float f = sqrtf(fx * fx + fy * fy);
+    planes[i] = f > 0.f ? cscale(planes[i], 1.0f / f) : make_float2(0.f, 0.f);
+}
+
+/* per patch: min-max rescale to [0,255]; half of the patches get a seeded
+ * contrast change, additive noise and 16-level quantisation; u8 out */
+__global__ __launch_bounds__(1024) void sur_to_u8(const float2* planes, uint8_t* img, int s, int n,
+                                                  uint64_t seed) {
+    __shared__ float smin[1024], smax[1024];
+    const long per = (long)s * s;
+    const int k = blockIdx.x;
+    const float2* p = planes + (long)k * per;
+    float lo = 3e38f, hi = -3e38f;
+    for (long i = threadIdx.x; i < per; i += blockDim.x) {
+        lo = fminf(lo, p[i].x);
+        hi = fmaxf(hi, p[i].x);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + st]);
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    lo = smin[0];
+    hi = smax[0];
+    const float sc = hi > lo ? 255.f / (hi - lo) : 0.f;
+    const uint64_t ks = seed * 0x9E3779B97F4A7C15ull + (uint64_t)k * 0xD1B54A32D192ED03ull;
+    const bool distort = (sm64(ks) & 1ull) != 0;
+    const float contrast = 0.7f + 0.6f * u01(ks + 11), sigma = 2.f + 18.f * u01(ks + 13);
+    for (long i = threadIdx.x; i < per; i += blockDim.x) {
+        float v = (p[i].x - lo) * sc;
+        if (distort) {
+            const uint64_t z = ks + 0x100000ull + (uint64_t)i * 2;
+            const float nz = sqrtf(-2.f * logf(u01(z))) * cosf(6.2831853f * u01(z + 1));
+            v = 127.5f + contrast * (v - 127.5f) + sigma * nz;
+            v = 17.f * rintf(v / 17.f);
+        }
+        img[(long)k * per + i] = (uint8_t)fminf(fmaxf(rintf(v), 0.f), 255.f);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void sur_gather_y(const float* yiq, long N, int npairs, float* x) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * N * npairs) return;
+    const long j = i / N, q = i - j * N;
+    x[i] = yiq[(j >> 1) * 6 * N + (j & 1) * N + q] * (1.0f / 255.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void sur_gather_pc(const float* pc, long N, int npairs, float* t) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * N * npairs) return;
+    const float v = pc[i];
+    t[i] = isfinite(v) ? v : 0.f; /* flat patches: the reference's 0/0 */
+}
+
+/* Spot check: block (t, im, j) compares the network PC of image im of pair
+ * j * every with the exact PC on one 32 x 32 tile at a seeded position and
+ * writes the tile's max-abs difference (NaN exact values, the reference's
+ * 0/0 on flat regions, are skipped). */
+constexpr int kSpotTiles = 4, kSpotSize = 32;
+
+__global__ __launch_bounds__(kBlock) void sur_spot_kernel(const float* __restrict__ pc_net,
+                                                          const float* __restrict__ pc_exact, int wd,
+                                                          int hd, int every, uint64_t seed,
+                                                          float* __restrict__ err) {
+    __shared__ float red[kBlock / 64];
+    const int t = blockIdx.x, im = blockIdx.y, j = blockIdx.z;
+    const long N = (long)wd * hd;
+    const int tw = min(kSpotSize, wd), th = min(kSpotSize, hd);
+    const uint64_t z = sm64(seed ^ ((uint64_t)j * 8 + (uint64_t)im * 4 + (uint64_t)t) * 0x9E3779B97F4A7C15ull);
+    const int x0 = (int)((z & 0xffffffffull) % (uint64_t)(wd - tw + 1));
+    const int y0 = (int)((z >> 32) % (uint64_t)(hd - th + 1));
+    const float* a = pc_net + ((long)j * every * 2 + im) * N;
+    const float* b = pc_exact + ((long)j * 2 + im) * N;
+    float m = 0.f;
+    for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
+        const long q = (long)(y0 + i / tw) * wd + x0 + i % tw;
+        const float d = fabsf(a[q] - b[q]);
+        if (d == d) m = fmaxf(m, d);
+    }
+    for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.f;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = fmaxf(r, red[w]);
+        err[((long)j * 2 + im) * kSpotTiles + t] = r;
+    }
+}
+
+/* ---- host ---------------------------------------------------------------- */
+
+Surrogate::Surrogate(Ctx& c, uint64_t seed) : c_(c) {
+    const int n = SurLayout::total;
+    IQM_HIP(hipMalloc(&m.master, sizeof(float) * n));
+    IQM_HIP(hipMalloc(&m.grad, sizeof(float) * n));
+    IQM_HIP(hipMalloc(&m.adam_m, sizeof(float) * n));
+    IQM_HIP(hipMalloc(&m.adam_v, sizeof(float) * n));
+    IQM_HIP(hipMalloc(&m.wbf, sizeof(uint16_t) * 4 * 9216));
+    IQM_HIP(hipMalloc(&m.wbf_t, sizeof(uint16_t) * 4 * 9216));
+    IQM_HIP(hipMalloc(&d_loss_, sizeof(double) * 1025));
+    m.seed = seed;
+    /* He-uniform initialisation from a SplitMix64 stream; zero biases */
+    std::vector<float> p(n, 0.f);
+    uint64_t st = seed ? seed : 1;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((double)(z >> 11) + 0.5) / 9007199254740992.0;
+    };
+    auto fill = [&](int off, int count, int fan_in) {
+        const double lim = std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; ++i) p[off + i] = (float)((2.0 * next() - 1.0) * lim);
+    };
+    fill(SurLayout::w1, 32 * 9, 9);
+    for (int l = 2; l <= 5; ++l) fill(SurLayout::wl(l), 9216, 288);
+    fill(SurLayout::w6, 32 * 9, 288);
+    set_params(p.data());
+    IQM_HIP(hipMemsetAsync(m.adam_m, 0, sizeof(float) * n, c_.stream()));
+    IQM_HIP(hipMemsetAsync(m.adam_v, 0, sizeof(float) * n, c_.stream()));
+    IQM_HIP(hipMemsetAsync(m.grad, 0, sizeof(float) * n, c_.stream()));
+    c_.sync();
+}
+
+Surrogate::~Surrogate() {
+    (void)hipStreamSynchronize(c_.stream());
+    for (void* q : {(void*)m.master, (void*)m.grad, (void*)m.adam_m, (void*)m.adam_v, (void*)m.wbf,
+                    (void*)m.wbf_t, (void*)d_loss_, (void*)acts_.x, (void*)acts_.pred,
+                    (void*)acts_.dpred, (void*)acts_.dz[0], (void*)acts_.dz[1], (void*)acts_.partial,
+                    (void*)spot_pc_, (void*)spot_err_})
+        if (q) (void)hipFree(q);
+    for (auto* a : acts_.a)
+        if (a) (void)hipFree(a);
+}
+
+void Surrogate::refresh_bf16() {
+    c_.launch("sur_cast", sur_cast, dim3(ceil_div(4 * 9216, kBlock)), dim3(kBlock), 0,
+              (const float*)m.master, m.wbf, m.wbf_t);
+}
+
+/* Host copies are ordered on the context's (non-blocking) stream. */
+void Surrogate::set_params(const float* host) {
+    IQM_HIP(hipMemcpyAsync(m.master, host, sizeof(float) * SurLayout::total, hipMemcpyHostToDevice, c_.stream()));
+    refresh_bf16();
+    c_.sync();
+}
+void Surrogate::get_params(float* host) const {
+    IQM_HIP(hipMemcpyAsync(host, m.master, sizeof(float) * SurLayout::total, hipMemcpyDeviceToHost, c_.stream()));
+    IQM_HIP(hipStreamSynchronize(c_.stream()));
+}
+void Surrogate::get_grad(float* host) const {
+    IQM_HIP(hipMemcpyAsync(host, m.grad, sizeof(float) * SurLayout::total, hipMemcpyDeviceToHost, c_.stream()));
+    IQM_HIP(hipStreamSynchronize(c_.stream()));
+}
+
+void Surrogate::ensure_acts(int nimg, int h, int w) {
+    if (acts_.nimg >= nimg && acts_.h == h && acts_.w == w) return;
+    c_.sync();
+    for (void* q : {(void*)acts_.x, (void*)acts_.pred, (void*)acts_.dpred, (void*)acts_.dz[0],
+                    (void*)acts_.dz[1], (void*)acts_.partial})
+        if (q) (void)hipFree(q);
+    for (auto*& a : acts_.a)
+        if (a) {
+            (void)hipFree(a);
+            a = nullptr;
+        }
+    const long P = (long)nimg * h * w;
+    acts_.nimg = nimg;
+    acts_.h = h;
+    acts_.w = w;
+    IQM_HIP(hipMalloc(&acts_.x, sizeof(float) * P));
+    IQM_HIP(hipMalloc(&acts_.pred, sizeof(float) * P));
+    IQM_HIP(hipMalloc(&acts_.dpred, sizeof(float) * P));
+    for (auto*& a : acts_.a) IQM_HIP(hipMalloc(&a, sizeof(uint16_t) * 32 * P));
+    IQM_HIP(hipMalloc(&acts_.dz[0], sizeof(uint16_t) * 32 * P));
+    IQM_HIP(hipMalloc(&acts_.dz[1], sizeof(uint16_t) * 32 * P));
+    acts_.npart = 512;
+    IQM_HIP(hipMalloc(&acts_.partial, sizeof(float) * (size_t)acts_.npart * (9216 + 32 + 2 * 320)));
+}
+
+static void launch_forward(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale,
+                           uint16_t* const a[5], float* out, int nimg, int h, int w) {
+    const long P = (long)nimg * h * w;
+    c.launch("sur_conv1", sur_conv_first, dim3(ceil_div(P, kBlock)), dim3(kBlock), 0, x, ps, is, scale,
+             (const float*)m.master, a[0], nimg, h, w);
+    const dim3 g(ceil_div(w, 16), ceil_div(h, 16), nimg);
+    for (int l = 2; l <= 5; ++l)
+        c.launch("sur_conv_mid", sur_conv_mid<true>, g, dim3(kBlock), 0, (const uint16_t*)a[l - 2],
+                 (const uint16_t*)(m.wbf + (l - 2) * 9216), (const float*)(m.master + SurLayout::bl(l)),
+                 (const uint16_t*)nullptr, a[l - 1], h, w);
+    c.launch("sur_conv6", sur_conv_last, dim3(ceil_div(P, kBlock)), dim3(kBlock), 0, (const uint16_t*)a[4],
+             (const float*)m.master, out, nimg, h, w);
+}
+
+void Surrogate::forward(int nimg, int h, int w) {
+    launch_forward(c_, m, acts_.x, 2L * h * w, (long)h * w, 1.0f, acts_.a, acts_.pred, nimg, h, w);
+}
+
+void Surrogate::infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
+                              int nimg, int h, int w) {
+    ensure_acts(nimg, h, w);
+    launch_forward(c_, m, d_x, pstride, istride, scale, acts_.a, d_out, nimg, h, w);
+}
+
+void Surrogate::infer(const float* d_x, float* d_out, int nimg, int h, int w) {
+    ensure_acts(nimg, h, w);
+    launch_forward(c_, m, d_x, 2L * h * w, (long)h * w, 1.0f, acts_.a, d_out, nimg, h, w);
+}
+
+double Surrogate::backward(const float* d_target) {
+    const int nimg = acts_.nimg, h = acts_.h, w = acts_.w;
+    const long P = (long)nimg * h * w;
+    const int lb = 1024;
+    c_.launch("sur_loss", sur_loss, dim3(lb), dim3(kBlock), 0, (const float*)acts_.pred, d_target,
+              acts_.dpred, d_loss_, P);
+    c_.launch("sur_loss_final", sur_loss_final, dim3(1), dim3(64), 0, (const double*)d_loss_, lb, P,
+              d_loss_ + 1024);
+    /* layer 6 -> dz5 */
+    c_.launch("sur_dlast", sur_dlast, dim3(ceil_div(P, kBlock)), dim3(kBlock), 0, (const float*)acts_.dpred,
+              (const float*)m.master, (const uint16_t*)acts_.a[4], acts_.dz[0], nimg, h, w);
+    const dim3 g(ceil_div(w, 16), ceil_div(h, 16), nimg);
+    const int np = acts_.npart;
+    float* part = acts_.partial;
+    int cur = 0;
+    for (int l = 5; l >= 2; --l) {
+        /* dW_l from dz_l and a_{l-1}; then dz_{l-1} = conv(dz_l, Wt_l) * (a_{l-1} > 0) */
+        c_.launch("sur_dw_mid", sur_dw_mid, dim3(np), dim3(288), 0, (const uint16_t*)acts_.dz[cur],
+                  (const uint16_t*)acts_.a[l - 2], part, nimg, h, w);
+        c_.launch("sur_reduce", sur_reduce, dim3(ceil_div(9216 + 32, kBlock)), dim3(kBlock), 0,
+                  (const float*)part, np, 9216 + 32, 9216 + 32, m.grad, SurLayout::wl(l));
+        c_.launch("sur_conv_dx", sur_conv_mid<false>, g, dim3(kBlock), 0, (const uint16_t*)acts_.dz[cur],
+                  (const uint16_t*)(m.wbf_t + (l - 2) * 9216), (const float*)nullptr,
+                  (const uint16_t*)acts_.a[l - 2], acts_.dz[cur ^ 1], h, w);
+        cur ^= 1;
+    }
+    /* layer 1 and layer 6 weights (dz1 = dz[cur]) */
+    float* p1 = part;
+    float* p6 = part + (size_t)np * 320;
+    c_.launch("sur_dw_edge", sur_dw_edge, dim3(np), dim3(kBlock), 0, (const float*)acts_.x,
+              2L * h * w, (long)h * w, 1.0f, (const uint16_t*)acts_.dz[cur], (const uint16_t*)acts_.a[4],
+              (const float*)acts_.dpred, p1, p6, nimg, h, w);
+    c_.launch("sur_reduce", sur_reduce, dim3(ceil_div(320, kBlock)), dim3(kBlock), 0, (const float*)p1, np,
+              320, 320, m.grad, SurLayout::w1);
+    c_.launch("sur_reduce", sur_reduce, dim3(ceil_div(289, kBlock)), dim3(kBlock), 0, (const float*)p6, np,
+              289, 289, m.grad, SurLayout::w6);
+    double loss = 0;
+    IQM_HIP(hipMemcpyAsync(&loss, d_loss_ + 1024, sizeof(double), hipMemcpyDeviceToHost, c_.stream()));
+    IQM_HIP(hipStreamSynchronize(c_.stream()));
+    return loss;
+}
+
+double Surrogate::loss_and_grad(const float* d_x, const float* d_target, int nimg, int h, int w) {
+    ensure_acts(nimg, h, w);
+    IQM_HIP(hipMemcpyAsync(acts_.x, d_x, sizeof(float) * nimg * h * w, hipMemcpyDeviceToDevice, c_.stream()));
+    forward(nimg, h, w);
+    return backward(d_target);
+}
+
+double Surrogate::train_step(const float* d_x, const float* d_target, int nimg, int h, int w, float lr) {
+    const double loss = loss_and_grad(d_x, d_target, nimg, h, w);
+    ++m.step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - (float)std::pow(0.9, (double)m.step), c2 = 1.f - (float)std::pow(0.999, (double)m.step);
+    c_.launch("sur_adam", sur_adam, dim3(ceil_div(SurLayout::total, kBlock)), dim3(kBlock), 0, m.master,
+              (const float*)m.grad, m.adam_m, m.adam_v, SurLayout::total, lr, b1, b2, 1e-8f, c1, c2);
+    refresh_bf16();
+    m.data_cursor += (uint64_t)nimg;
+    return loss;
+}
+
+void Surrogate::make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc) {
+    IQM_CHECK(n > 0 && n % 2 == 0 && size >= 16, IQM_ERR_INVALID, "patch count must be even and size >= 16");
+    const long per = (long)size * size;
+    const int chunk = std::min(n, 256);
+    float2* planes = nullptr;
+    uint8_t* u8 = nullptr;
+    float* pcs = nullptr;
+    IQM_HIP(hipMalloc(&planes, sizeof(float2) * per * chunk));
+    IQM_HIP(hipMalloc(&u8, per * chunk));
+    IQM_HIP(hipMalloc(&pcs, sizeof(float) * 2 * per * (chunk / 2)));
+    struct Free {
+        void* a; void* b; void* c;
+        ~Free() { (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); }
+    } guard{planes, u8, pcs};
+    for (int k0 = 0; k0 < n; k0 += chunk) {
+        const int k = std::min(chunk, n - k0);
+        const uint64_t s = seed + (uint64_t)k0 * 1315423911ull;
+        c_.launch("sur_noise", sur_noise, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0, planes, per, k, s);
+        c_.fft2d_dev(planes, size, size, k, -1, false);
+        c_.launch("sur_pink", sur_pink, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0, planes, size, k);
+        c_.fft2d_dev(planes, size, size, k, +1, true);
+        c_.launch("sur_to_u8", sur_to_u8, dim3(k), dim3(1024), 0, (const float2*)planes, u8, size, k, s);
+        /* exact PC targets: patches 2i and 2i+1 as one FSIM pair (gray) */
+        FsimBatchOut out;
+        out.d_pc = pcs;
+        c_.fsim_batch(u8, u8 + per, size, size, 1, 2 * per, k / 2, out);
+        Ctx::FsimLayout L = c_.fsim_layout(per, k / 2, c_.fsim_ws_base());
+        c_.launch("sur_gather_y", sur_gather_y, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0,
+                  (const float*)L.yiq, per, k / 2, d_x + (long)k0 * per);
+        c_.launch("sur_gather_pc", sur_gather_pc, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0,
+                  (const float*)pcs, per, k / 2, d_pc + (long)k0 * per);
+        c_.sync();
+    }
+}
+
+double* Surrogate::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                              long pair_stride, int n, int spot_every, float tol, float* spot_err,
+                              int* fell_back) {
+    float* pc_net = nullptr;
+    int wd = 0, hd = 0;
+    double* res = c_.fsim_batch_pcfn(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{},
+                                     [&](const float* yiq, long N, int np, float* pc) {
+                                         (void)N;
+                                         const int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+                                         wd = (int)std::round((float)w / (float)F);
+                                         hd = (int)std::round((float)h / (float)F);
+                                         infer_strided(yiq, 6 * N, N, 1.0f / 255.0f, pc, 2 * np, hd, wd);
+                                         pc_net = pc;
+                                     });
+    float err = 0.f;
+    int back = 0;
+    if (spot_every > 0) {
+        const int m = (n + spot_every - 1) / spot_every;
+        const long N = (long)wd * hd;
+        if ((size_t)m * 2 * N > spot_cap_) {
+            c_.sync();
+            if (spot_pc_) (void)hipFree(spot_pc_);
+            if (spot_err_) (void)hipFree(spot_err_);
+            spot_pc_ = nullptr;
+            spot_err_ = nullptr;
+            spot_cap_ = 0;
+            IQM_HIP(hipMalloc(&spot_pc_, sizeof(float) * 2 * N * m));
+            IQM_HIP(hipMalloc(&spot_err_, sizeof(float) * 2 * kSpotTiles * m));
+            spot_cap_ = (size_t)m * 2 * N;
+        }
+        FsimBatchOut ex;
+        ex.d_pc = spot_pc_;
+        c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride * spot_every, m, ex);
+        c_.launch("sur_spot", sur_spot_kernel, dim3(kSpotTiles, 2, m), dim3(kBlock), 0, (const float*)pc_net,
+                  (const float*)spot_pc_, wd, hd, spot_every, (uint64_t)0x5EED5EEDull, spot_err_);
+        std::vector<float> e((size_t)2 * kSpotTiles * m);
+        IQM_HIP(hipMemcpyAsync(e.data(), spot_err_, sizeof(float) * e.size(), hipMemcpyDeviceToHost, c_.stream()));
+        c_.sync();
+        for (float v : e) err = std::max(err, v);
+        if (!(err <= tol)) {
+            back = 1;
+            res = c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{});
+        }
+    }
+    if (spot_err) *spot_err = err;
+    if (fell_back) *fell_back = back;
+    return res;
+}
+
+/* versioned little-endian blob: magic, version, step, seed, cursor, then the
+ * FP32 master weights and both Adam moments */
+std::vector<uint8_t> Surrogate::save() const {
+    const int n = SurLayout::total;
+    std::vector<uint8_t> blob(8 + 4 + 3 * 8 + 3 * 4 * (size_t)n);
+    std::memcpy(blob.data(), "IQMSUR01", 8);
+    const uint32_t ver = 1;
+    std::memcpy(blob.data() + 8, &ver, 4);
+    const uint64_t hdr[3] = {(uint64_t)m.step, m.seed, m.data_cursor};
+    std::memcpy(blob.data() + 12, hdr, sizeof(hdr));
+    uint8_t* p = blob.data() + 12 + sizeof(hdr);
+    for (const float* src : {m.master, m.adam_m, m.adam_v}) {
+        IQM_HIP(hipMemcpyAsync(p, src, sizeof(float) * n, hipMemcpyDeviceToHost, c_.stream()));
+        p += sizeof(float) * n;
+    }
+    IQM_HIP(hipStreamSynchronize(c_.stream()));
+    return blob;
+}
+
+void Surrogate::load(const std::vector<uint8_t>& blob) {
+    const int n = SurLayout::total;
+    IQM_CHECK(blob.size() == 8 + 4 + 3 * 8 + 3 * 4 * (size_t)n && std::memcmp(blob.data(), "IQMSUR01", 8) == 0,
+              IQM_ERR_INVALID, "not a surrogate checkpoint of this version");
+    uint64_t hdr[3];
+    std::memcpy(hdr, blob.data() + 12, sizeof(hdr));
+    m.step = (long)hdr[0];
+    m.seed = hdr[1];
+    m.data_cursor = hdr[2];
+    const uint8_t* p = blob.data() + 12 + sizeof(hdr);
+    for (float* dst : {m.master, m.adam_m, m.adam_v}) {
+        IQM_HIP(hipMemcpyAsync(dst, p, sizeof(float) * n, hipMemcpyHostToDevice, c_.stream()));
+        p += sizeof(float) * n;
+    }
+    refresh_bf16();
+    c_.sync();
+}
+
+}  // namespace iqm

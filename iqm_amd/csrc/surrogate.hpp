@@ -1,0 +1,104 @@
+// Learned phase-congruency surrogate: a 6-layer 3x3 convolutional network,
+// 32 channels, ReLU, from the downscaled luminance (Y / 255) to the PC map.
+// BF16 weights and activations with FP32 accumulation; FP32 master weights
+// and Adam moments for training.  Host side of surrogate.hip.
+#pragma once
+
+#include "ctx.hpp"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace iqm {
+
+constexpr int kSurCh = 32;
+constexpr int kSurLayers = 6;
+
+/* Parameter offsets in the flat FP32 parameter vector (PyTorch OIHW order). */
+struct SurLayout {
+    static constexpr int w1 = 0;                       /* [32][1][3][3]  */
+    static constexpr int b1 = w1 + 32 * 9;             /* [32]           */
+    static constexpr int wmid = b1 + 32;               /* 4 x ([32][32][3][3] + [32]) */
+    static constexpr int mid_stride = 32 * 32 * 9 + 32;
+    static constexpr int w6 = wmid + 4 * mid_stride;   /* [1][32][3][3]  */
+    static constexpr int b6 = w6 + 32 * 9;             /* [1]            */
+    static constexpr int total = b6 + 1;               /* 37,601 */
+    static constexpr int wl(int l) { return wmid + (l - 2) * mid_stride; }           /* l = 2..5 */
+    static constexpr int bl(int l) { return wmid + (l - 2) * mid_stride + 32 * 32 * 9; }
+};
+
+struct SurModel {
+    float* master = nullptr;   /* FP32 parameters (SurLayout)                */
+    float* grad = nullptr;     /* FP32 gradient of the last step             */
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    uint16_t* wbf = nullptr;   /* BF16 copy of the mid-layer weights, OIHW    */
+    uint16_t* wbf_t = nullptr; /* flipped / transposed copy for the input gradient */
+    long step = 0;
+    uint64_t seed = 0;
+    uint64_t data_cursor = 0;  /* patches consumed (for resume)              */
+};
+
+/* Device workspace of one forward/backward over nimg images of h x w. */
+struct SurActs {
+    int nimg = 0, h = 0, w = 0;
+    float* x = nullptr;        /* input Y/255, [img][h][w]                   */
+    uint16_t* a[5] = {};       /* post-ReLU activations of layers 1..5, NHWC bf16 */
+    float* pred = nullptr;     /* output, [img][h][w]                        */
+    float* dpred = nullptr;
+    uint16_t* dz[2] = {};      /* ping-pong gradient buffers, NHWC bf16      */
+    float* partial = nullptr;  /* weight-gradient partials                   */
+    int npart = 0;
+};
+
+class Surrogate {
+   public:
+    Surrogate(Ctx& c, uint64_t seed);
+    ~Surrogate();
+    Surrogate(const Surrogate&) = delete;
+    Surrogate& operator=(const Surrogate&) = delete;
+
+    /* forward of nimg images of h x w (device FP32 Y/255 in, FP32 PC out) */
+    void infer(const float* d_x, float* d_out, int nimg, int h, int w);
+    /* forward reading image j at d_x + (j/2)*pstride + (j%2)*istride, scaled */
+    void infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
+                       int nimg, int h, int w);
+    /* one Adam step on a minibatch; returns the MSE loss (synchronises) */
+    double train_step(const float* d_x, const float* d_target, int nimg, int h, int w, float lr);
+    /* loss and gradient into grad (no update), for gradient checks */
+    double loss_and_grad(const float* d_x, const float* d_target, int nimg, int h, int w);
+    /* generation of seeded training patches and their exact PC targets */
+    void make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc);
+
+    /* FSIM/FSIMc over n device-resident pairs with phase congruency from the
+     * network.  Pairs i with i % spot_every == 0 (spot_every > 0) are also
+     * evaluated exactly; the error is the largest |PC_net - PC_exact| over 4
+     * seeded 32 x 32 tiles of each of their two images.  When it exceeds tol
+     * the whole batch is recomputed exactly.  Returns the device result array
+     * (n x 8 doubles, as Ctx::fsim_batch). */
+    double* fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                       int n, int spot_every, float tol, float* spot_err, int* fell_back);
+
+    void set_params(const float* host);   /* FP32 master, SurLayout order */
+    void get_params(float* host) const;
+    void get_grad(float* host) const;
+    std::vector<uint8_t> save() const;
+    void load(const std::vector<uint8_t>& blob);
+
+    SurModel m;
+
+   private:
+    void refresh_bf16();
+    void ensure_acts(int nimg, int h, int w);
+    void forward(int nimg, int h, int w);
+    double backward(const float* d_target);
+    Ctx& c_;
+    SurActs acts_;
+    double* d_loss_ = nullptr;
+    float* spot_pc_ = nullptr;   /* exact PC maps of the spot-checked pairs */
+    float* spot_err_ = nullptr;  /* per (pair, image, tile) max-abs errors */
+    size_t spot_cap_ = 0;        /* pairs spot_pc_/spot_err_ hold */
+};
+
+}  // namespace iqm
