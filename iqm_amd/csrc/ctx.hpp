@@ -84,6 +84,9 @@ class Ctx {
      * [sum pcm, sum sim, sum simc, fsim, fsimc, -, -, -]. */
     double* fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                        long pair_stride, int n, const FsimBatchOut& out);
+    /* F2 of n pairs into the yiq planes; clears the median histograms */
+    void launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stride, int w, int h, int ch,
+                     int F, int wd, int hd, int n, float* yiq, unsigned* hist);
     /* FSIM with phase congruency supplied by pc_fn instead of the FFT stages:
      * pc_fn(yiq, N, n, pc) fills pc[pair][2][N] from the Y planes (plane 0 and
      * 1 of every 6-plane pair block of yiq).  Results as fsim_batch. */

@@ -184,26 +184,39 @@ const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
 
 constexpr int kHistBins = 1024; /* level-1 radix-select bins: key bits 30..21 */
 
-/* F2: fsim_downsample.glsl:38-54.  A workgroup produces kPrepCols
- * downscaled pixels of one row for both images: the bytes of the F
- * full-resolution rows they cover are staged in LDS as 16-byte aligned
+/* F2: fsim_downsample.glsl:38-54.  A workgroup produces `cols` downscaled
+ * pixels of one row for both images: the bytes of the F full-resolution rows
+ * they cover are staged in LDS for both images at once, as 16-byte aligned
  * chunks (an aligned 16-byte block that holds a byte of the image lies in
- * that byte's page, so these loads never leave mapped memory), then each
- * thread sums its F x F box (zero outside the image, divided by F^2).  yiq
+ * that byte's page, so these loads never leave mapped memory).  Each thread
+ * then sums the R, G and B bytes of its F x F box as integers (exact; zero
+ * outside the image) and applies the YIQ matrix once, divided by F^2 --
+ * the same value as the shader's per-pixel sum in exact arithmetic.  yiq
  * planes per pair: Y_in, Y_ref, I_in, Q_in, I_ref, Q_ref.  Workgroup (0, 0)
  * of each pair also clears the pair's median histograms. */
-constexpr int kPrepCols = 256;
+constexpr int kPrepMaxCols = 256;
+constexpr int kPrepMaxRows = 64;
 
 /* 16-byte chunks staged per full-resolution row for a segment of seg bytes */
 __host__ __device__ __forceinline__ int prep_row_chunks(int seg) { return seg / 16 + 2; }
 
+/* output columns per prep workgroup: the widest power of two whose two
+ * staged images fit kMaxLdsBytes */
+static int prep_cols(int F, int ch) {
+    int c = kPrepMaxCols;
+    while (c > 1 && (size_t)2 * F * prep_row_chunks(c * F * ch) * 16 > (size_t)kMaxLdsBytes) c >>= 1;
+    return c;
+}
+static size_t prep_lds_bytes(int F, int ch, int cols) { return (size_t)2 * F * prep_row_chunks(cols * F * ch) * 16; }
+
 __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
                                                            const uint8_t* __restrict__ ref,
                                                            long pair_stride, int w, int h,
-                                                           int ch, int F, int wd, int hd,
+                                                           int ch, int F, int wd, int hd, int cols,
                                                            float* __restrict__ yiq,
                                                            unsigned* __restrict__ hist) {
     extern __shared__ uint4 stage4[];
+    __shared__ int rowoff[2 * kPrepMaxRows];
     const uint8_t* stage = (const uint8_t*)stage4;
     const int pair = blockIdx.z;
     const int tid = threadIdx.x;
@@ -212,8 +225,8 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
         for (int i = tid; i < 8 * kHistBins; i += blockDim.x) hp[i] = 0u;
     }
     const int y = blockIdx.y;
-    const int xd0 = blockIdx.x * kPrepCols;
-    const int ncols = min(kPrepCols, wd - xd0);
+    const int xd0 = blockIdx.x * cols;
+    const int ncols = min(cols, wd - xd0);
     /* full-resolution window of this block, clipped to the image */
     const int c_lo = xd0 * F - F / 2, c_hi = (xd0 + ncols) * F - F / 2;
     const int r_lo = y * F - F / 2;
@@ -222,45 +235,67 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
     const int seg = max(cc1 - cc0, 0) * ch; /* bytes of each staged row */
     const int nrows = max(rr1 - rr0, 0);
     const int rc = prep_row_chunks(seg);     /* chunks per staged row */
-    const long N = (long)wd * hd;
-    float* base = yiq + (long)pair * 6 * N;
-    const float scaler = (float)F * (float)F;
-    const int x = xd0 + tid;
-    for (int im = 0; im < 2; ++im) {
-        const uint8_t* src = (im ? ref : img) + (long)pair * pair_stride;
-        for (int i = tid; i < nrows * rc; i += blockDim.x) {
-            const int r = i / rc, k = i - r * rc;
-            const uintptr_t a0 = (uintptr_t)(src + ((long)(rr0 + r) * w + cc0) * ch);
-            const uintptr_t first = a0 & ~(uintptr_t)15;
-            const uintptr_t last = (a0 + seg - 1) & ~(uintptr_t)15;
-            const uintptr_t a = first + (uintptr_t)k * 16;
-            if (a <= last) stage4[i] = *(const uint4*)a;
-        }
-        __syncthreads();
-        if (tid < ncols) {
-            float si = 0.f, sq = 0.f, sy = 0.f;
-            const int xs = x * F - F / 2;
-            for (int j = xs; j < xs + F; ++j) {
-                if (j < cc0 || j >= cc1) continue;
-                for (int k = r_lo; k < r_lo + F; ++k) {
-                    if (k < rr0 || k >= rr1) continue;
-                    const int r = k - rr0;
-                    const uintptr_t a0 = (uintptr_t)(src + ((long)k * w + cc0) * ch);
-                    const uint8_t* p = stage + r * rc * 16 + (int)(a0 & 15) + (j - cc0) * ch;
-                    float R, G, B;
-                    if (ch == 1) { R = G = B = (float)p[0]; }
-                    else { R = (float)p[0]; G = (float)p[1]; B = (float)p[2]; }
-                    si += R * 0.596f - G * 0.274f - B * 0.322f;
-                    sq += R * 0.211f - G * 0.523f + B * 0.312f;
-                    sy += R * 0.299f + G * 0.587f + B * 0.114f;
+    const uint8_t* src0 = img + (long)pair * pair_stride;
+    const uint8_t* src1 = ref + (long)pair * pair_stride;
+    const int per_im = nrows * rc;
+    const int nr2 = 2 * nrows; /* staged rows: image 0 rows, then image 1 rows */
+    /* thread = chunk column; 8 rows of loads are issued before their LDS stores */
+    for (int k = tid; k < rc; k += blockDim.x) {
+        for (int r0 = 0; r0 < nr2; r0 += 8) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int rr = r0 + u;
+                v[u] = make_uint4(0u, 0u, 0u, 0u);
+                if (rr < nr2) {
+                    const int im = rr >= nrows, r = rr - im * nrows;
+                    const uintptr_t a0 = (uintptr_t)((im ? src1 : src0) + ((long)(rr0 + r) * w + cc0) * ch);
+                    const uintptr_t first = a0 & ~(uintptr_t)15;
+                    const uintptr_t last = (a0 + seg - 1) & ~(uintptr_t)15;
+                    const uintptr_t a = first + (uintptr_t)k * 16;
+                    if (a <= last) v[u] = *(const uint4*)a;
                 }
             }
-            const long idx = (long)y * wd + x;
-            base[(im ? 1 : 0) * N + idx] = sy / scaler;
-            base[(im ? 4 : 2) * N + idx] = si / scaler;
-            base[(im ? 5 : 3) * N + idx] = sq / scaler;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (r0 + u < nr2) stage4[(r0 + u) * rc + k] = v[u];
         }
-        __syncthreads();
+    }
+    if (tid < 2 * nrows) {
+        const int im = tid >= nrows, r = tid - im * nrows;
+        const uintptr_t a0 = (uintptr_t)((im ? src1 : src0) + ((long)(rr0 + r) * w + cc0) * ch);
+        rowoff[tid] = (im * per_im + r * rc) * 16 + (int)(a0 & 15);
+    }
+    __syncthreads();
+    const long N = (long)wd * hd;
+    float* base = yiq + (long)pair * 6 * N;
+    const float inv = 1.0f / ((float)F * (float)F);
+    for (int t = tid; t < ncols; t += blockDim.x) {
+        const int x = xd0 + t;
+        const int xs = x * F - F / 2;
+        const int j0 = max(xs, cc0), j1 = min(xs + F, cc1);
+        const int nb = (j1 - j0) * ch;
+        const long idx = (long)y * wd + x;
+        for (int im = 0; im < 2; ++im) {
+            int sr = 0, sg = 0, sb = 0;
+            for (int r = 0; r < nrows; ++r) {
+                const uint8_t* p = stage + rowoff[im * nrows + r] + (j0 - cc0) * ch;
+                if (ch == 1) {
+                    for (int b = 0; b < nb; ++b) sr += p[b];
+                } else {
+                    for (int b = 0; b < nb; b += ch) {
+                        sr += p[b];
+                        sg += p[b + 1];
+                        sb += p[b + 2];
+                    }
+                }
+            }
+            if (ch == 1) sg = sb = sr;
+            const float R = (float)sr, G = (float)sg, B = (float)sb;
+            base[(im ? 1 : 0) * N + idx] = (R * 0.299f + G * 0.587f + B * 0.114f) * inv;
+            base[(im ? 4 : 2) * N + idx] = (R * 0.596f - G * 0.274f - B * 0.322f) * inv;
+            base[(im ? 5 : 3) * N + idx] = (R * 0.211f - G * 0.523f + B * 0.312f) * inv;
+        }
     }
 }
 
@@ -779,6 +814,16 @@ Ctx::FsimLayout Ctx::fsim_layout(long N, int n, char* base) const {
     return L;
 }
 
+void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stride, int w, int h, int ch,
+                      int F, int wd, int hd, int n, float* yiq, unsigned* hist) {
+    IQM_CHECK(F <= kPrepMaxRows, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+    const int cols = prep_cols(F, ch);
+    const size_t bytes = prep_lds_bytes(F, ch, cols);
+    IQM_CHECK(bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, cols), hd, n), dim3(kBlock), bytes, d_img, d_ref,
+           pair_stride, w, h, ch, F, wd, hd, cols, yiq, hist);
+}
+
 double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                         long pair_stride, int n, const FsimBatchOut& out) {
     int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
@@ -811,12 +856,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     double* res = (double*)(base + r.res);
 
     /* F2 */
-    {
-        const size_t stage_bytes = (size_t)F * prep_row_chunks(kPrepCols * F * ch) * 16;
-        IQM_CHECK(stage_bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-        launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, kPrepCols), hd, n), dim3(kBlock),
-               stage_bytes, d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, hist);
-    }
+    launch_prep(d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, n, yiq, hist);
     /* row half of F5 (F3 is evaluated in fsim_pcfinal) */
     const bool tl = env_int("IQM_FSIM_GENERIC", tune.two_level ? 0 : 1) == 0;
     const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 160 * 1024));
@@ -893,12 +933,7 @@ double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, 
     float* pc = (float*)(base + o_pc);
     double* partial = (double*)(base + o_part);
     double* res = (double*)(base + o_res);
-    {
-        const size_t stage_bytes = (size_t)F * prep_row_chunks(kPrepCols * F * ch) * 16;
-        IQM_CHECK(stage_bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-        launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, kPrepCols), hd, n), dim3(kBlock), stage_bytes,
-               d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, yiq, (unsigned*)(base + o_hist));
-    }
+    launch_prep(d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, n, yiq, (unsigned*)(base + o_hist));
     pc_fn(yiq, N, n, pc);
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
            dim3(kBlock), 0, (const float*)yiq, (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
