@@ -9,6 +9,14 @@ algorithmic byte model of one exact FSIMc pair,
 against the MI355X HBM peak of 8 TB/s.  The CPU baseline is the FP32 CPU
 restatement of the same algorithm (the oracle library) timed on this host.
 
+Other workloads (`--workload`), each printing one JSON line of the same
+shape:
+  mixed  256 RGB pairs of uneven sizes (long side U[360,2160], aspect U[1,2],
+         seeds 3000+i, generated on the device), SSIM + FSIMc per pair with the
+         SSIM and FSIMc maps written, as one captured batch; against the same
+         pairs as 512 separate per-method calls.  Algorithmic bytes per pair:
+         2*3*W*H + 4*W*H + 32*N*33 + 4*N.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -84,9 +92,103 @@ def cpu_baseline(pairs, budget_s: float) -> dict:
                       f"({med * 1e3:.0f} ms/pair) after 1 warm-up, OpenMP {threads} threads"}
 
 
+def run_mixed(args, ctx) -> dict:
+    """256 uneven pairs, SSIM + FSIMc each, one captured batch vs separate calls."""
+    import iqm_amd
+    from iqm_amd import synth
+
+    sizes = synth.mixed_sizes(args.mixed_pairs, 3000)
+    jobs, alg, npx = [], 0, 0
+    for i, (w, h) in enumerate(sizes):
+        _, wd, hd = iqm_amd.fsim_dims(w, h)
+        di, dr = ctx.alloc(w * h * 3), ctx.alloc(w * h * 3)
+        ctx.synth_pair(w, h, 3000 + i, di, dr)
+        jobs.append({"d_img": di, "d_ref": dr, "w": w, "h": h, "ch": 3, "methods": 3,
+                     "d_ssim_map": ctx.alloc(4 * w * h), "d_simc_map": ctx.alloc(4 * wd * hd)})
+        alg += 2 * 3 * w * h + 4 * w * h + 32 * wd * hd * 33 + 4 * wd * hd
+        npx += w * h
+    ctx.sync()
+    n = len(jobs)
+    timer = HipTimer(ctx.stream)
+
+    def timed(fn, steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        timer.start()
+        for _ in range(steps):
+            out = fn()
+        dev = timer.stop_ms() / steps
+        return out, dev, (time.perf_counter() - t0) * 1e3 / steps
+
+    t0 = time.perf_counter()
+    first = ctx.batch_dev(jobs)  # eager pass (plans, workspaces) + capture
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        ctx.batch_dev(jobs)
+    res, dev_ms, wall_ms = timed(lambda: ctx.batch_dev(jobs), args.steps)
+    assert np.array_equal(res, first)
+    ctx.set_graphs(False)
+    _, eager_dev, eager_wall = timed(lambda: ctx.batch_dev(jobs), max(1, args.steps // 2))
+    ssim, fsim = iqm_amd.SSIM(), iqm_amd.FSIM()
+
+    def separate():
+        out = np.empty((n, 3))
+        for k, j in enumerate(jobs):
+            out[k, 0] = ssim.compute_batch_dev(ctx, j["d_img"], j["d_ref"], j["w"], j["h"], 3, 0, 1,
+                                               j["d_ssim_map"])[0]
+            fs, fc = fsim.compute_batch_dev(ctx, j["d_img"], j["d_ref"], j["w"], j["h"], 3, 0, 1, j["d_simc_map"])
+            out[k, 1], out[k, 2] = fs[0], fc[0]
+        return out
+
+    sep, sep_dev, sep_wall = timed(separate, max(1, args.steps // 2))
+    ctx.set_graphs(True)
+    # single-pair calls take the two-level FFT kernels for 480/270/256-long
+    # lines, the batch the generic ones: equal up to FP32 rounding
+    diff = np.abs(sep - res)
+    two_level = [k for k, j in enumerate(jobs) if set(iqm_amd.fsim_dims(j["w"], j["h"])[1:]) & {480, 270, 256}]
+    differing = [int(k) for k in np.nonzero(diff.max(axis=1) > 0)[0]]
+    print(f"[mixed] max |batch - separate| = {diff.max():.3e}; differing pairs {differing}; "
+          f"pairs with two-level lines {two_level}", file=sys.stderr, flush=True)
+    assert diff.max() <= 1e-5
+    stages = {}
+    if not args.no_stages:
+        ctx.reset_stage_times()
+        ctx.set_stage_timing(True)
+        ctx.batch_dev(jobs)
+        ctx.set_stage_timing(False)
+        for name, (ms, cnt) in ctx.stage_times().items():
+            stages[name] = {"us_per_step": 1e3 * ms, "launches_per_step": cnt}
+    for j in jobs:
+        for k in ("d_img", "d_ref", "d_ssim_map", "d_simc_map"):
+            ctx.free(j[k])
+    achieved = alg / (dev_ms * 1e-3)
+    return {
+        "metric": "mixed-size SSIM+FSIMc pairs/s (one captured batch)",
+        "value": n / (dev_ms * 1e-3), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "mixed_ssim_fsimc_256_uneven_rgb", "pairs_per_step": n,
+                   "megapixels_per_step": npx / 1e6, "inputs": "device-resident u8 RGB, generated on the device"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "alg_bytes_per_step": alg,
+                     "kernel": "whole batch launch sequence"},
+        "wall_ms_per_step": wall_ms,
+        "eager_batch": {"ms_per_step": eager_dev, "wall_ms_per_step": eager_wall, "pairs_per_s": n / (eager_wall * 1e-3)},
+        "separate_calls": {"calls_per_step": 2 * n, "wall_ms_per_step": sep_wall,
+                           "pairs_per_s": n / (sep_wall * 1e-3)},
+        "graph_vs_separate": sep_wall / wall_ms,
+        "max_abs_diff_vs_separate": float(diff.max()),
+        "setup_s": setup_s,
+        "stages_us": stages,
+        "device": ctx.device_name,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed"])
+    ap.add_argument("--mixed-pairs", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=32, help="pairs per step")
@@ -109,6 +211,11 @@ def main() -> int:
     from iqm_amd import synth
 
     ctx = iqm_amd.Context(0)
+    if args.workload == "mixed":
+        out = run_mixed(args, ctx)
+        ctx.close()
+        print(json.dumps(out))
+        return 0
     F, wd, hd = iqm_amd.fsim_dims(W, H)
     B = args.batch
     uniq = [synth.rgb_pair(W, H, seed=2 + i) for i in range(max(1, min(args.unique, B)))]

@@ -125,6 +125,29 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
                        int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                        float* d_simc_maps, double* fsim_out, double* fsimc_out);
 
+/* ---- mixed batches ------------------------------------------------------ */
+/* Pairs of uneven sizes, each scored with SSIM and/or FSIM/FSIMc, issued as
+ * one launch sequence and replayed as one captured hipGraph when the same
+ * batch (same pointers, sizes, methods and SSIM parameters) is submitted
+ * again.  The reference scores one pair per call (src/main.cpp:69-113,
+ * 180-214); this is the batched form of those calls. */
+#define IQM_JOB_SSIM 1
+#define IQM_JOB_FSIM 2
+typedef struct {
+    const uint8_t* d_img;   /* device, packed rows of w * ch bytes */
+    const uint8_t* d_ref;
+    int32_t w, h, ch;       /* ch 1, 3 or 4 */
+    int32_t methods;        /* IQM_JOB_SSIM | IQM_JOB_FSIM */
+    float* d_ssim_map;      /* nullable device, w * h floats */
+    float* d_simc_map;      /* nullable device, wd * hd floats (iqm_fsim_dims) */
+} iqm_pair_job;
+typedef struct {
+    double mssim, fsim, fsimc; /* NaN for a method not asked for */
+} iqm_pair_result;
+/* ssim: nullable (defaults). results: n host structs. */
+int iqm_batch_dev(iqm_ctx* ctx, const iqm_pair_job* jobs, int32_t n, const iqm_ssim_params* ssim,
+                  iqm_pair_result* results);
+
 /* ---- learned phase-congruency surrogate -------------------------------- */
 /* No counterpart in the reference: a 6-layer 3x3 convolutional network
  * (32 channels, ReLU, BF16 weights and activations, FP32 accumulation on the
@@ -171,6 +194,19 @@ int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img,
                                  int32_t spot_every, float spot_tol, double* fsim_out,
                                  double* fsimc_out, float* spot_err, int32_t* fell_back);
 
+/* ---- seeded synthetic workloads on the device -------------------------- */
+/* No counterpart in the reference: the benchmark inputs of iqm_amd/synth.py
+ * generated on the device.  Reference = three channels of colour-correlated
+ * (0.8) 1/f^2 noise, joint min-max to u8; test = the reference blurred
+ * (sigma U[0.5,3]), contrast-scaled about the mean (U[0.7,1.3]), plus
+ * N(0, sigma_n^2) noise (sigma_n U[2,20]), quantised to 16 levels, clipped.
+ * Packed RGB rows.  iqm_synth_pair_dev writes the pair of `seed` (the test
+ * image uses seed + 1000003); iqm_synth_distort_dev distorts a w x h RGB view
+ * with row stride src_row_stride bytes.  Both run on the context's stream. */
+int iqm_synth_pair_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_t* d_img, uint8_t* d_ref);
+int iqm_synth_distort_dev(iqm_ctx* ctx, const uint8_t* d_src, int64_t src_row_stride, int32_t w,
+                          int32_t h, uint64_t seed, uint8_t* d_out);
+
 /* ---- FFT (replaces VkFFT: fsim.cpp:370-437 plan, :527-550 append) ------- */
 
 /* In-place batched 2-D C2C FFT of `batch` interleaved complex float planes
@@ -185,6 +221,10 @@ int iqm_dev_alloc(iqm_ctx* ctx, size_t bytes, void** out);
 int iqm_dev_free(iqm_ctx* ctx, void* p);
 int iqm_memcpy_h2d(iqm_ctx* ctx, void* dst, const void* src, size_t bytes);
 int iqm_memcpy_d2h(iqm_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* 2-D copy of `rows` rows of width_bytes on the context's stream (device to
+ * device, asynchronous: ordered before the context's later work). */
+int iqm_memcpy2d_dev(iqm_ctx* ctx, void* dst, int64_t dst_pitch, const void* src, int64_t src_pitch,
+                     int64_t width_bytes, int64_t rows);
 /* The context's hipStream_t (every kernel of the context runs on it). */
 void* iqm_stream(iqm_ctx* ctx);
 int iqm_sync(iqm_ctx* ctx);

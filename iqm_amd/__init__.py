@@ -37,8 +37,12 @@ EXPORTS = (
     "iqm_sur_param_count", "iqm_sur_create", "iqm_sur_destroy", "iqm_sur_get_params",
     "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches",
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
-    "iqm_fsim_surrogate_batch_dev",
+    "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
+    "iqm_batch_dev",
 )
+
+JOB_SSIM = 1
+JOB_FSIM = 2
 
 
 class IqmError(RuntimeError):
@@ -59,6 +63,16 @@ class SsimParams(ctypes.Structure):
 
 class FsimMaps(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("pc_in", "pc_ref", "pcm", "sim", "simc")]
+
+
+class PairJob(ctypes.Structure):
+    _fields_ = [("d_img", ctypes.c_void_p), ("d_ref", ctypes.c_void_p), ("w", ctypes.c_int32),
+                ("h", ctypes.c_int32), ("ch", ctypes.c_int32), ("methods", ctypes.c_int32),
+                ("d_ssim_map", ctypes.c_void_p), ("d_simc_map", ctypes.c_void_p)]
+
+
+class PairResult(ctypes.Structure):
+    _fields_ = [("mssim", ctypes.c_double), ("fsim", ctypes.c_double), ("fsimc", ctypes.c_double)]
 
 
 class FsimDetail(ctypes.Structure):
@@ -133,6 +147,11 @@ def lib():
     L.iqm_sur_load.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
     L.iqm_fsim_surrogate_batch_dev.argtypes = [vp, vp, vp, vp, i32, i32, i32, i64, i32, i32,
                                                ctypes.c_float, dp, dp, fp, ctypes.POINTER(i32)]
+    L.iqm_batch_dev.argtypes = [vp, ctypes.POINTER(PairJob), i32, ctypes.POINTER(SsimParams),
+                                ctypes.POINTER(PairResult)]
+    L.iqm_synth_pair_dev.argtypes = [vp, i32, i32, u64, vp, vp]
+    L.iqm_synth_distort_dev.argtypes = [vp, vp, i64, i32, i32, u64, vp]
+    L.iqm_memcpy2d_dev.argtypes = [vp, vp, i64, vp, i64, i64, i64]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -212,6 +231,28 @@ class Context:
     def download(self, src: int, out: np.ndarray) -> np.ndarray:
         _check(lib().iqm_memcpy_d2h(self.handle, out.ctypes.data, src, out.nbytes))
         return out
+
+    def synth_pair(self, w: int, h: int, seed: int, d_img: int, d_ref: int) -> None:
+        """Device copy of iqm_amd.synth.rgb_pair(w, h, seed) into packed RGB buffers."""
+        _check(lib().iqm_synth_pair_dev(self.handle, w, h, int(seed), d_img, d_ref))
+
+    def synth_distort(self, d_src: int, row_stride: int, w: int, h: int, seed: int, d_out: int) -> None:
+        _check(lib().iqm_synth_distort_dev(self.handle, d_src, row_stride, w, h, int(seed), d_out))
+
+    def copy2d(self, dst: int, dst_pitch: int, src: int, src_pitch: int, width_bytes: int, rows: int) -> None:
+        _check(lib().iqm_memcpy2d_dev(self.handle, dst, dst_pitch, src, src_pitch, width_bytes, rows))
+
+    def batch_dev(self, jobs, ssim_params: "SsimParams | None" = None) -> np.ndarray:
+        """Scores a mixed batch; jobs are dicts with d_img, d_ref, w, h, ch, methods and
+        optional d_ssim_map / d_simc_map.  Returns an (n, 3) array [mssim, fsim, fsimc]."""
+        n = len(jobs)
+        arr = (PairJob * n)()
+        for i, j in enumerate(jobs):
+            arr[i] = PairJob(j["d_img"], j["d_ref"], j["w"], j["h"], j.get("ch", 3),
+                             j.get("methods", JOB_SSIM | JOB_FSIM), j.get("d_ssim_map"), j.get("d_simc_map"))
+        res = (PairResult * n)()
+        _check(lib().iqm_batch_dev(self.handle, arr, n, ctypes.byref(ssim_params) if ssim_params else None, res))
+        return np.array([[r.mssim, r.fsim, r.fsimc] for r in res], np.float64)
 
     def set_graphs(self, on: bool) -> None:
         _check(lib().iqm_set_graphs(self.handle, 1 if on else 0))

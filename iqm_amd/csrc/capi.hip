@@ -72,6 +72,11 @@ void* Ctx::workspace(int slot, size_t bytes) {
 void Ctx::drop_graphs() {
     for (auto& kv : graphs_cache_) (void)hipGraphExecDestroy(kv.second.exec);
     graphs_cache_.clear();
+    for (auto& kv : mixed_graphs_) {
+        (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.table) (void)hipFree(kv.second.table);
+    }
+    mixed_graphs_.clear();
 }
 
 double* Ctx::fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
@@ -407,6 +412,45 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
     });
 }
 
+int iqm_batch_dev(iqm_ctx* ctx, const iqm_pair_job* jobs, int32_t n, const iqm_ssim_params* ssim,
+                  iqm_pair_result* results) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(jobs && results && n > 0, IQM_ERR_INVALID, "invalid batch arguments");
+        iqm_ssim_params p;
+        iqm_ssim_default_params(&p);
+        if (ssim) p = *ssim;
+        iqm::SsimArgs sa;
+        sa.ks = p.kernel_size;
+        sa.k1 = p.k1;
+        sa.k2 = p.k2;
+        sa.sigma = p.sigma;
+        IQM_CHECK(sa.ks >= 3 && sa.ks <= 31 && (sa.ks % 2) == 1, IQM_ERR_INVALID,
+                  "kernel_size must be odd and in [3, 31]");
+        std::vector<iqm::PairJob> v((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            v[i].img = jobs[i].d_img;
+            v[i].ref = jobs[i].d_ref;
+            v[i].w = jobs[i].w;
+            v[i].h = jobs[i].h;
+            v[i].ch = jobs[i].ch;
+            v[i].methods = jobs[i].methods;
+            v[i].ssim_map = jobs[i].d_ssim_map;
+            v[i].simc_map = jobs[i].d_simc_map;
+        }
+        iqm::Ctx& c = *ctx->c;
+        double* d = c.mixed_batch_graph(v.data(), n, sa);
+        std::vector<double> r((size_t)4 * n);
+        IQM_HIP(hipMemcpyAsync(r.data(), d, sizeof(double) * r.size(), hipMemcpyDeviceToHost, c.stream()));
+        c.sync();
+        for (int i = 0; i < n; ++i) {
+            results[i].mssim = r[(size_t)4 * i];
+            results[i].fsim = r[(size_t)4 * i + 1];
+            results[i].fsimc = r[(size_t)4 * i + 2];
+        }
+    });
+}
+
 struct iqm_sur {
     iqm_ctx* ctx;
     iqm::Surrogate* s;
@@ -611,6 +655,35 @@ int iqm_memcpy_d2h(iqm_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return guarded([&] {
         IQM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c->stream()));
         ctx->c->sync();
+    });
+}
+
+int iqm_memcpy2d_dev(iqm_ctx* ctx, void* dst, int64_t dst_pitch, const void* src, int64_t src_pitch,
+                     int64_t width_bytes, int64_t rows) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(dst && src && width_bytes >= 0 && rows >= 0 && dst_pitch >= width_bytes &&
+                      src_pitch >= width_bytes, IQM_ERR_INVALID, "invalid 2-D copy");
+        IQM_HIP(hipMemcpy2DAsync(dst, (size_t)dst_pitch, src, (size_t)src_pitch, (size_t)width_bytes,
+                                 (size_t)rows, hipMemcpyDeviceToDevice, ctx->c->stream()));
+    });
+}
+
+int iqm_synth_pair_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_t* d_img, uint8_t* d_ref) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_img && d_ref, IQM_ERR_INVALID, "null buffer");
+        ctx->c->synth_reference(w, h, seed, d_ref);
+        ctx->c->synth_distort(d_ref, 3L * w, w, h, seed + 1000003ull, d_img);
+    });
+}
+
+int iqm_synth_distort_dev(iqm_ctx* ctx, const uint8_t* d_src, int64_t src_row_stride, int32_t w,
+                          int32_t h, uint64_t seed, uint8_t* d_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_src && d_out, IQM_ERR_INVALID, "null buffer");
+        ctx->c->synth_distort(d_src, src_row_stride, w, h, seed, d_out);
     });
 }
 

@@ -47,6 +47,63 @@ struct FsimTune {
     double us_per_pair = 0; /* measured by the autotuner (0 = default, not measured) */
 };
 
+/* One pair of a mixed batch (device pointers, packed rows). */
+struct PairJob {
+    const uint8_t* img = nullptr;
+    const uint8_t* ref = nullptr;
+    int w = 0, h = 0, ch = 3;
+    int methods = 3;            /* bit 0 SSIM, bit 1 FSIM */
+    float* ssim_map = nullptr;  /* nullable device, w * h */
+    float* simc_map = nullptr;  /* nullable device, wd * hd */
+};
+
+struct SsimArgs {
+    int ks = 11;
+    float k1 = 0.01f, k2 = 0.03f, sigma = 1.5f;
+};
+
+/* One pair of a mixed batch as the batched kernels read it from the device
+ * job table: sizes, tiles, line plans and per-pair buffers. */
+struct MixJob {
+    const uint8_t* img;
+    const uint8_t* ref;
+    int w, h, ch;
+    /* SSIM */
+    int ssim_gx, ssim_gy;
+    double ssim_count;
+    float* ssim_map;
+    double* ssim_partial;
+    double* ssim_out;
+    /* FSIM */
+    int F, wd, hd, cols, grad_rb, col_bx, nbx, row_rb, pc_gx, pc_gy;
+    long N;
+    float invN;
+    LinePlan Px, Py;
+    const float* bank;
+    FsimNoiseArgs na;
+    float* yiq;
+    float2* A;
+    float2* T;
+    float* an_e;
+    float* keys;
+    unsigned* hist;
+    float* thr;
+    double* stats;
+    double* partial;
+    double* res;
+    float* simc_map;
+    double* out; /* [mssim, fsim, fsimc, -] */
+};
+
+/* Stages of a mixed batch; each is one launch per LDS class (0: the
+ * workgroup fits 64 KiB, 1: it needs more). */
+enum { MIX_SSIM = 0, MIX_SSIMFIN, MIX_PREP, MIX_ROWFFT, MIX_COL, MIX_ROW, MIX_SELECT, MIX_PCF, MIX_FIN,
+       MIX_STAGES };
+struct MixCounts {
+    int blocks[MIX_STAGES][2] = {};
+    size_t lds[MIX_STAGES][2] = {};
+};
+
 struct FsimBatchOut {
     float* d_simc = nullptr;  /* nullable device maps, n * N */
     float* d_pc = nullptr;    /* nullable device, n * 2 * N (pc_in, pc_ref) */
@@ -106,11 +163,35 @@ class Ctx {
     FsimLayout fsim_layout(long N, int n, char* base) const;
     char* fsim_ws_base() { return fsim_ws_; }
 
+    /* Seeded synthetic RGB images on the device (recipe of iqm_amd/synth.py):
+     * the colour-correlated 1/f^2 reference of w x h, and the distortion of a
+     * w x h RGB view (row_stride bytes) into packed rows. */
+    void synth_reference(int w, int h, uint64_t seed, uint8_t* d_out);
+    void synth_distort(const uint8_t* d_src, long row_stride, int w, int h, uint64_t seed, uint8_t* d_out);
+
     /* SSIM over n same-size device-resident pairs; returns n mssim values
      * on the host. */
-    void ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
-                    long pair_stride, int n, int ks, float k1, float k2, float sigma,
-                    float* d_maps, double* mssim_host);
+    double* ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                       long pair_stride, int n, int ks, float k1, float k2, float sigma,
+                       float* d_maps, double* mssim_host);
+
+    /* Pairs of any sizes and methods in one launch sequence on the stream:
+     * returns the device array of n x 4 doubles [mssim, fsim, fsimc, -] (NaN
+     * for a method not asked for).  The _graph variant replays a captured
+     * hipGraph of the whole sequence when called again with the same jobs. */
+    double* mixed_batch(const PairJob* jobs, int n, const SsimArgs& sp, void* table = nullptr,
+                        bool upload = true);
+    double* mixed_batch_graph(const PairJob* jobs, int n, const SsimArgs& sp);
+    /* per-pair setup of a mixed batch: fills the method's fields of mj with
+     * buffers carved from base + off (base may be null to size the
+     * workspace), adds its workgroups to cnt (blk: per-stage counts of this
+     * pair, cls: its LDS class per stage) */
+    void mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, MixCounts& cnt, int* blk, int* cls);
+    void mix_ssim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, MixCounts& cnt, int* blk, int* cls,
+                       const SsimArgs& sp);
+    /* launches of the batched stages over the device job table */
+    void mix_fsim_launch(const MixJob* d_jobs, const int* d_starts, int n, const MixCounts& cnt);
+    void mix_ssim_launch(const MixJob* d_jobs, const int* d_starts, int n, const MixCounts& cnt, const SsimArgs& sp);
 
     template <class Kernel, class... Args>
     void launch(const char* tag, Kernel k, dim3 grid, dim3 block, size_t shmem, Args... args) {
@@ -180,8 +261,10 @@ class Ctx {
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
         double* res = nullptr;
+        void* table = nullptr; /* device job table a mixed-batch graph reads */
     };
     std::map<GraphKey, GraphEntry> graphs_cache_;
+    std::map<std::string, GraphEntry> mixed_graphs_;
     void drop_graphs();
     bool graphs_ = true;
     bool capturing_ = false;
@@ -194,7 +277,8 @@ class Ctx {
 };
 
 /* workspace slots */
-enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR };
+enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR, WS_GEN, WS_BATCH,
+       WS_MIX, WS_MIXTAB };
 
 size_t line_lds_bytes(const LinePlan& P, int lines, int bufs);
 int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap);

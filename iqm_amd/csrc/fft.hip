@@ -11,27 +11,50 @@ namespace iqm {
 
 namespace {
 
-/* iterative radix-2 FFT in double (plan-time only), sign -1 */
-void host_fft_pow2(std::vector<std::complex<double>>& a) {
+/* recursive mixed-radix DFT in double (plan-time only), sign -1: splits by
+ * the smallest prime factor p, direct O(n^2) sums for a prime length */
+void host_dft(std::vector<std::complex<double>>& a) {
     const size_t n = a.size();
-    for (size_t i = 1, j = 0; i < n; ++i) {
-        size_t bit = n >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
-        if (i < j) std::swap(a[i], a[j]);
-    }
+    if (n <= 1) return;
     const double two_pi = 6.283185307179586476925286766559;
-    for (size_t len = 2; len <= n; len <<= 1) {
-        for (size_t i = 0; i < n; i += len) {
-            for (size_t k = 0; k < len / 2; ++k) {
-                const double ang = -two_pi * (double)k / (double)len;
-                const std::complex<double> w(std::cos(ang), std::sin(ang));
-                const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
-                a[i + k] = u + v;
-                a[i + k + len / 2] = u - v;
+    size_t p = 2;
+    while (p * p <= n && n % p) ++p;
+    if (n % p) p = n;
+    if (p == n) {
+        std::vector<std::complex<double>> o(n);
+        for (size_t k = 0; k < n; ++k) {
+            std::complex<double> acc = 0.0;
+            for (size_t j = 0; j < n; ++j) {
+                const double ang = -two_pi * (double)((j * k) % n) / (double)n;
+                acc += a[j] * std::complex<double>(std::cos(ang), std::sin(ang));
             }
+            o[k] = acc;
         }
+        a.swap(o);
+        return;
     }
+    const size_t q = n / p;
+    std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(q));
+    for (size_t r = 0; r < p; ++r)
+        for (size_t k = 0; k < q; ++k) sub[r][k] = a[k * p + r];
+    for (auto& v : sub) host_dft(v);
+    for (size_t k = 0; k < q; ++k)
+        for (size_t s = 0; s < p; ++s) {
+            const size_t kk = k + q * s;
+            std::complex<double> acc = 0.0;
+            for (size_t r = 0; r < p; ++r) {
+                const double ang = -two_pi * (double)((r * kk) % n) / (double)n;
+                acc += sub[r][k] * std::complex<double>(std::cos(ang), std::sin(ang));
+            }
+            a[kk] = acc;
+        }
+}
+
+/* smallest length >= n that the direct Stockham path plans */
+int next_plannable(int n) {
+    std::vector<int> rad;
+    for (int m = n;; ++m)
+        if (plan_radices(m, rad)) return m;
 }
 
 float2* upload_table(const std::vector<float2>& v) {
@@ -71,8 +94,8 @@ const LinePlanHost& Ctx::line_plan(int n) {
         P.lds_len = n;
         lp.owned = {tw};
     } else {
-        int m = 1;
-        while (m < 2 * n - 1) m <<= 1;
+        /* any plannable m >= 2n - 1 holds the linear convolution */
+        const int m = next_plannable(2 * n - 1);
         std::vector<int> mr;
         IQM_CHECK(plan_radices(m, mr), IQM_ERR_UNSUPPORTED, "Bluestein length not plannable");
         P.blue = 1;
@@ -98,8 +121,8 @@ const LinePlanHost& Ctx::line_plan(int n) {
                 gi[m - d] = gi[d];
             }
         }
-        host_fft_pow2(gf);
-        host_fft_pow2(gi);
+        host_dft(gf);
+        host_dft(gi);
         std::vector<float2> gff(m), gif(m);
         for (int j = 0; j < m; ++j) {
             gff[j] = make_float2((float)gf[j].real(), (float)gf[j].imag());
@@ -190,8 +213,11 @@ __global__ __launch_bounds__(kBlock) void fft_cols_kernel(float2* data, int w, i
 void Ctx::fft2d_dev(float2* d, int w, int h, int planes, int dir, bool normalize) {
     const LinePlan& px = line_plan(w).plan;
     const LinePlan& py = line_plan(h).plan;
-    const size_t budget = 65536;
-    const int lr = lines_that_fit(px, 2, budget, 8), lc = lines_that_fit(py, 2, budget, 8);
+    auto fit = [&](const LinePlan& P) {
+        const int l = lines_that_fit(P, 2, std::min<size_t>(lds_per_block_, 65536), 8);
+        return l > 0 ? l : lines_that_fit(P, 2, lds_per_block_, 8);
+    };
+    const int lr = fit(px), lc = fit(py);
     IQM_CHECK(lr > 0 && lc > 0, IQM_ERR_UNSUPPORTED, "FFT line too long for LDS");
     const long ps = (long)w * h;
     const float s = normalize ? (float)(1.0 / ((double)w * (double)h)) : 1.0f;

@@ -209,23 +209,20 @@ static int prep_cols(int F, int ch) {
 }
 static size_t prep_lds_bytes(int F, int ch, int cols) { return (size_t)2 * F * prep_row_chunks(cols * F * ch) * 16; }
 
-__global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
-                                                           const uint8_t* __restrict__ ref,
-                                                           long pair_stride, int w, int h,
-                                                           int ch, int F, int wd, int hd, int cols,
-                                                           float* __restrict__ yiq,
-                                                           unsigned* __restrict__ hist) {
+__device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
+                                               long pair_stride, int w, int h, int ch, int F, int wd, int hd,
+                                               int cols, float* __restrict__ yiq, unsigned* __restrict__ hist,
+                                               int bx, int by, int pair) {
     extern __shared__ uint4 stage4[];
     __shared__ int rowoff[2 * kPrepMaxRows];
     const uint8_t* stage = (const uint8_t*)stage4;
-    const int pair = blockIdx.z;
     const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (bx == 0 && by == 0) {
         unsigned* hp = hist + (long)pair * 8 * kHistBins;
         for (int i = tid; i < 8 * kHistBins; i += blockDim.x) hp[i] = 0u;
     }
-    const int y = blockIdx.y;
-    const int xd0 = blockIdx.x * cols;
+    const int y = by;
+    const int xd0 = bx * cols;
     const int ncols = min(cols, wd - xd0);
     /* full-resolution window of this block, clipped to the image */
     const int c_lo = xd0 * F - F / 2, c_hi = (xd0 + ncols) * F - F / 2;
@@ -299,20 +296,28 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
     }
 }
 
+__global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
+                                                           const uint8_t* __restrict__ ref,
+                                                           long pair_stride, int w, int h,
+                                                           int ch, int F, int wd, int hd, int cols,
+                                                           float* __restrict__ yiq,
+                                                           unsigned* __restrict__ hist) {
+    fsim_prep_body(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, yiq, hist, blockIdx.x, blockIdx.y,
+                   blockIdx.z);
+}
+
 /* Row half of F5 on the packed rows Y_in + i*Y_ref of `rb` rows; the two
  * real-input row spectra are split with X(k) = (Z(k) + conj Z(-k)) / 2,
  * Y(k) = (Z(k) - conj Z(-k)) / (2i). */
-__global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __restrict__ yiq,
-                                                             float2* __restrict__ A, int wd, int hd,
-                                                             int rb, LinePlan P) {
+__device__ __forceinline__ void fsim_rowfft_body(const float* __restrict__ yiq, float2* __restrict__ A, int wd,
+                                                 int hd, int rb, const LinePlan& P, int bx, int pair) {
     extern __shared__ float2 lds[];
     const int ld = lds_pitch(P.lds_len);
     float2* b0 = lds;
     float2* b1 = lds + rb * ld;
     const float2* tw = stage_twiddles(P, lds + 2 * rb * ld);
-    const int pair = blockIdx.y;
     const long N = (long)wd * hd;
-    const int y0 = blockIdx.x * rb;
+    const int y0 = bx * rb;
     const int nr = min(rb, hd - y0);
     const float* Yin = yiq + (long)pair * 6 * N;
     const float* Yref = Yin + N;
@@ -335,6 +340,12 @@ __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __rest
     }
 }
 
+__global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __restrict__ yiq,
+                                                             float2* __restrict__ A, int wd, int hd,
+                                                             int rb, LinePlan P) {
+    fsim_rowfft_body(yiq, A, wd, hd, rb, P, blockIdx.x, blockIdx.y);
+}
+
 /* Column half of F5, then F9 and the column half of F11.  A workgroup owns
  * `bx` adjacent columns of one image of one pair: the forward column FFT
  * gives the spectrum; each of the 16 filter products H_z * F is inverse
@@ -342,14 +353,12 @@ __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __rest
  * The flat grid is laid out (pair, img, padded column block) with the
  * column-block count padded to a multiple of 8, so column block cb always
  * runs on XCD cb mod 8 and each XCD's L2 holds only its slice of the bank. */
-__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* __restrict__ A,
-                                                              float2* __restrict__ Tout,
-                                                              const float* __restrict__ bank,
-                                                              int wd, int hd, int bx, int nbx_pad,
-                                                              LinePlan P) {
+__device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, float2* __restrict__ Tout,
+                                                  const float* __restrict__ bank, int wd, int hd, int bx,
+                                                  int nbx_pad, const LinePlan& P, int blk) {
     extern __shared__ float2 lds[];
-    const int cb = blockIdx.x % nbx_pad;
-    const int rest = blockIdx.x / nbx_pad;
+    const int cb = blk % nbx_pad;
+    const int rest = blk / nbx_pad;
     const int img = rest & 1, pair = rest >> 1;
     const int x0 = cb * bx;
     if (x0 >= wd) return;
@@ -385,16 +394,22 @@ __global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* _
     }
 }
 
+__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* __restrict__ A,
+                                                              float2* __restrict__ Tout,
+                                                              const float* __restrict__ bank,
+                                                              int wd, int hd, int bx, int nbx_pad,
+                                                              LinePlan P) {
+    fsim_colpass_body(A, Tout, bank, wd, hd, bx, nbx_pad, P, blockIdx.x);
+}
+
 /* Row half of F11 (normalised by 1/N) fused with F12
  * (fsim_sum_filter_responses.glsl:33-83) and the F13 packing
  * (fsim_pack_for_median.glsl:33-44) for `rb` rows of one image and
  * orientation; T planes img*16 + 4*o + s. */
-__global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* __restrict__ Tin,
-                                                              float* __restrict__ an_e,
-                                                              float* __restrict__ keys,
-                                                              unsigned* __restrict__ hist,
-                                                              int wd, int hd, int rb, LinePlan P,
-                                                              float invN) {
+__device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin, float* __restrict__ an_e,
+                                                  float* __restrict__ keys, unsigned* __restrict__ hist, int wd,
+                                                  int hd, int rb, const LinePlan& P, float invN, int bx, int io,
+                                                  int pair) {
     extern __shared__ float2 lds[];
     const int ld = lds_pitch(P.lds_len);
     const int nl = 4 * rb;
@@ -402,10 +417,8 @@ __global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* _
     float2* b1 = lds + nl * ld;
     const int ntw = lds_twiddles(P);
     const float2* tw = stage_twiddles(P, lds + 2 * nl * ld);
-    const int io = blockIdx.y;
-    const int pair = blockIdx.z;
     const long N = (long)wd * hd;
-    const int y0 = blockIdx.x * rb;
+    const int y0 = bx * rb;
     const int nr = min(rb, hd - y0);
     for (int idx = threadIdx.x; idx < nl * wd; idx += blockDim.x) {
         const int l = idx / wd, x = idx - l * wd;
@@ -457,6 +470,15 @@ __global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* _
     }
 }
 
+__global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* __restrict__ Tin,
+                                                              float* __restrict__ an_e,
+                                                              float* __restrict__ keys,
+                                                              unsigned* __restrict__ hist,
+                                                              int wd, int hd, int rb, LinePlan P,
+                                                              float invN) {
+    fsim_rowpass_body(Tin, an_e, keys, hist, wd, hd, rb, P, invN, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 /* Finds the bucket holding the element of rank k (0-based, ascending) in
  * counts[0..nbins); results in *bucket / *rest (shared).  All threads of
  * the block must call it. */
@@ -502,26 +524,20 @@ __device__ void rank_bucket(const unsigned* counts, int nbins, unsigned k, int* 
     __syncthreads();
 }
 
-struct NoiseArgs {
-    double em_n[4];
-    double est[8];
-};
+using NoiseArgs = FsimNoiseArgs;
 
 /* F13 + F15 thresholds: exact element N/2 of the ascending keys
  * (fsim_noise_power.cpp:198-199) by a 3-level radix select on the float bit
  * patterns (keys are >= 0, so the bits order like the values and bit 31 is
  * clear): level 1 uses the histogram built by fsim_rowpass (bits 30..21),
  * level 2 bits 20..10, level 3 bits 9..0. */
-__global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restrict__ keys,
-                                                           const unsigned* __restrict__ hist,
-                                                           float* __restrict__ thr,
-                                                           double* __restrict__ stats, long N,
-                                                           NoiseArgs na) {
+__device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys, const unsigned* __restrict__ hist,
+                                                 float* __restrict__ thr, double* __restrict__ stats, long N,
+                                                 const NoiseArgs& na, int io, int pair) {
     __shared__ unsigned h[2048];
     __shared__ unsigned wsum[16];
     __shared__ int bucket;
     __shared__ unsigned rest;
-    const int io = blockIdx.x, pair = blockIdx.y;
     const long slot = (long)pair * 8 + io;
     const unsigned* kb = (const unsigned*)(keys + slot * N);
     const unsigned* hg = hist + slot * kHistBins;
@@ -599,6 +615,14 @@ __global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restri
     }
 }
 
+__global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restrict__ keys,
+                                                           const unsigned* __restrict__ hist,
+                                                           float* __restrict__ thr,
+                                                           double* __restrict__ stats, long N,
+                                                           NoiseArgs na) {
+    fsim_select_body(keys, hist, thr, stats, N, na, blockIdx.x, blockIdx.y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
@@ -610,16 +634,17 @@ __device__ __forceinline__ double wave_sum(double v) {
  * staged in LDS with a one-pixel halo for the Scharr stencil. */
 constexpr int kPcTx = 64, kPcTy = 4;
 
-__global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
-    const float* __restrict__ yiq, float* __restrict__ grad, const float* __restrict__ an_e,
-    const float* __restrict__ thr, double* __restrict__ partial, int wd, int hd, float* simc_map,
-    float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src) {
+__device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq, float* __restrict__ grad,
+                                                  const float* __restrict__ an_e, const float* __restrict__ thr,
+                                                  double* __restrict__ partial, int wd, int hd, float* simc_map,
+                                                  float* pc_map, float* pcm_map, float* sim_map,
+                                                  const float* __restrict__ pc_src, int bx, int by, int gx,
+                                                  int gy, int pair) {
     constexpr int HX = kPcTx + 2, HY = kPcTy + 2;
     __shared__ float ys[2][HY][HX];
     __shared__ double red[3][kBlock / 64];
     const long N = (long)wd * hd;
-    const int pair = blockIdx.z;
-    const int x0 = blockIdx.x * kPcTx, y0 = blockIdx.y * kPcTy;
+    const int x0 = bx * kPcTx, y0 = by * kPcTy;
     const float* yq = yiq + (long)pair * 6 * N;
     for (int i = threadIdx.x; i < 2 * HY * HX; i += blockDim.x) {
         const int im = i / (HY * HX), rem = i - im * HY * HX;
@@ -705,8 +730,8 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
         double s3[3] = {0, 0, 0};
         for (int w = 0; w < (int)(blockDim.x / 64); ++w)
             for (int j = 0; j < 3; ++j) s3[j] += red[j][w];
-        const int nblk = gridDim.x * gridDim.y;
-        double* p = partial + ((long)pair * nblk + blockIdx.y * gridDim.x + blockIdx.x) * 4;
+        const int nblk = gx * gy;
+        double* p = partial + ((long)pair * nblk + by * gx + bx) * 4;
         p[0] = s3[0];
         p[1] = s3[1];
         p[2] = s3[2];
@@ -714,12 +739,19 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
     }
 }
 
+__global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
+    const float* __restrict__ yiq, float* __restrict__ grad, const float* __restrict__ an_e,
+    const float* __restrict__ thr, double* __restrict__ partial, int wd, int hd, float* simc_map,
+    float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src) {
+    fsim_pcfinal_body(yiq, grad, an_e, thr, partial, wd, hd, simc_map, pc_map, pcm_map, sim_map, pc_src,
+                      blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, blockIdx.z);
+}
+
 /* F17 (fsim_final_multiply.cpp:294-300): fixed-order sum of the block
  * partials; fsim = sum sim / sum pcm, fsimc = sum simc / sum pcm. */
-__global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __restrict__ partial,
-                                                               int nblk, double* __restrict__ res) {
+__device__ __forceinline__ void fsim_finalize_body(const double* __restrict__ partial, int nblk,
+                                                   double* __restrict__ res, int pair) {
     __shared__ double sh[3][kBlock];
-    const int pair = blockIdx.x;
     double s[3] = {0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double* p = partial + ((long)pair * nblk + b) * 4;
@@ -741,6 +773,11 @@ __global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __r
         r[4] = sh[2][0] / sh[0][0];
         r[5] = r[6] = r[7] = 0.0;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __restrict__ partial,
+                                                               int nblk, double* __restrict__ res) {
+    fsim_finalize_body(partial, nblk, res, blockIdx.x);
 }
 
 /* ---- host orchestration ------------------------------------------------- */
@@ -786,10 +823,17 @@ struct FsimTiles {
 
 FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long N, const FsimTune& tune) {
     FsimTiles t;
-    const size_t budget = std::min<size_t>(c.lds_per_block(), 65536);
-    t.grad_rb = lines_that_fit(Px, 2, budget, env_int("IQM_FSIM_GRB", tune.fft_rb));
-    t.col_bx = lines_that_fit(Py, 3, budget, env_int("IQM_FSIM_BX", tune.col_bx));
-    t.row_rb = lines_that_fit(Px, 2, budget, 4 * env_int("IQM_FSIM_RB", tune.row_rb)) / 4;
+    /* 64 KiB per workgroup keeps two workgroups per CU; when that holds
+     * fewer than 4 lines (or the cap), long (Bluestein) lines take the
+     * device's whole LDS for more work per barrier */
+    auto fit = [&](const LinePlan& P, int bufs, int cap, int group) {
+        const int l = lines_that_fit(P, bufs, std::min<size_t>(c.lds_per_block(), 65536), group * cap) / group;
+        if (l >= std::min(cap, 4)) return l;
+        return std::max(l, lines_that_fit(P, bufs, c.lds_per_block(), group * cap) / group);
+    };
+    t.grad_rb = fit(Px, 2, env_int("IQM_FSIM_GRB", tune.fft_rb), 1);
+    t.col_bx = fit(Py, 3, env_int("IQM_FSIM_BX", tune.col_bx), 1);
+    t.row_rb = fit(Px, 2, env_int("IQM_FSIM_RB", tune.row_rb), 4); /* 4 scales per row */
     /* pairs per column/row-pass launch: one launch over the whole batch
      * measured fastest on MI355X (larger grids beat cache residency of the
      * response planes); the cap bounds the response-plane workspace */
@@ -940,6 +984,164 @@ double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, 
            partial, wd, hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)pc);
     launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial, nblk, res);
     return res;
+}
+
+/* ---- mixed batches: one launch per stage over pairs of any sizes ---------- */
+
+/* pair j owns workgroups [st[j], st[j+1]) of a stage; st[n] is the total */
+__device__ __forceinline__ int mix_find(const int* __restrict__ st, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (st[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void mix_prep_kernel(const MixJob* __restrict__ jobs, const int* __restrict__ st,
+                                                          int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    const int local = blockIdx.x - st[j];
+    const int gx = (J.wd + J.cols - 1) / J.cols;
+    fsim_prep_body(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, J.yiq, J.hist, local % gx, local / gx,
+                   0);
+}
+
+__global__ __launch_bounds__(kBlock) void mix_rowfft_kernel(const MixJob* __restrict__ jobs,
+                                                            const int* __restrict__ st, int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    fsim_rowfft_body(J.yiq, J.A, J.wd, J.hd, J.grad_rb, J.Px, blockIdx.x - st[j], 0);
+}
+
+__global__ __launch_bounds__(kBlockFft) void mix_colpass_kernel(const MixJob* __restrict__ jobs,
+                                                                const int* __restrict__ st, int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    fsim_colpass_body(J.A, J.T, J.bank, J.wd, J.hd, J.col_bx, J.nbx, J.Py, blockIdx.x - st[j]);
+}
+
+__global__ __launch_bounds__(kBlockFft) void mix_rowpass_kernel(const MixJob* __restrict__ jobs,
+                                                                const int* __restrict__ st, int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    const int local = blockIdx.x - st[j];
+    const int gx = (J.hd + J.row_rb - 1) / J.row_rb;
+    fsim_rowpass_body(J.T, J.an_e, J.keys, J.hist, J.wd, J.hd, J.row_rb, J.Px, J.invN, local % gx, local / gx, 0);
+}
+
+__global__ __launch_bounds__(1024) void mix_select_kernel(const MixJob* __restrict__ jobs, const int* __restrict__ st,
+                                                          int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    fsim_select_body(J.keys, J.hist, J.thr, J.stats, J.N, J.na, blockIdx.x - st[j], 0);
+}
+
+__global__ __launch_bounds__(kBlock) void mix_pcfinal_kernel(const MixJob* __restrict__ jobs,
+                                                             const int* __restrict__ st, int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    const int local = blockIdx.x - st[j];
+    fsim_pcfinal_body(J.yiq, nullptr, J.an_e, J.thr, J.partial, J.wd, J.hd, J.simc_map, nullptr, nullptr, nullptr,
+                      nullptr, local % J.pc_gx, local / J.pc_gx, J.pc_gx, J.pc_gy, 0);
+}
+
+/* finalize, then the pair's scalars into its row of the batch result */
+__global__ __launch_bounds__(kBlock) void mix_finalize_kernel(const MixJob* __restrict__ jobs,
+                                                              const int* __restrict__ st, int n) {
+    const int j = mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    fsim_finalize_body(J.partial, J.pc_gx * J.pc_gy, J.res, 0);
+    if (threadIdx.x == 0) {
+        J.out[1] = J.res[3];
+        J.out[2] = J.res[4];
+    }
+}
+
+void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, MixCounts& cnt, int* blk,
+                        int* cls) {
+    const int F = std::max(1, (int)std::round(std::min(pj.w, pj.h) / 256.0));
+    const int wd = (int)std::round((float)pj.w / (float)F);
+    const int hd = (int)std::round((float)pj.h / (float)F);
+    IQM_CHECK(F <= kPrepMaxRows, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+    const FsimSizePlan& plan = fsim_plan(wd, hd);
+    const LinePlan& Px = line_plan(wd).plan;
+    const LinePlan& Py = line_plan(hd).plan;
+    const long N = (long)wd * hd;
+    const FsimTiles t = pick_tiles(*this, Px, Py, N, fsim_tune(wd, hd));
+    mj.F = F;
+    mj.wd = wd;
+    mj.hd = hd;
+    mj.N = N;
+    mj.invN = (float)(1.0 / (double)N);
+    mj.cols = prep_cols(F, pj.ch);
+    mj.grad_rb = t.grad_rb;
+    mj.col_bx = t.col_bx;
+    mj.nbx = ceil_div(wd, t.col_bx);
+    mj.row_rb = t.row_rb;
+    mj.pc_gx = ceil_div(wd, kPcTx);
+    mj.pc_gy = ceil_div(hd, kPcTy);
+    mj.Px = Px;
+    mj.Py = Py;
+    mj.bank = plan.bank;
+    for (int i = 0; i < 4; ++i) mj.na.em_n[i] = plan.em_n[i];
+    for (int i = 0; i < 8; ++i) mj.na.est[i] = plan.est[i];
+    const FsimRegions r = fsim_regions(N, 1, 1, mj.pc_gx * mj.pc_gy);
+    char* b = base ? base + off : nullptr;
+    mj.yiq = (float*)(b + r.yiq);
+    mj.A = (float2*)(b + r.A);
+    mj.T = (float2*)(b + r.T);
+    mj.an_e = (float*)(b + r.an_e);
+    mj.keys = (float*)(b + r.keys);
+    mj.hist = (unsigned*)(b + r.hist);
+    mj.thr = (float*)(b + r.thr);
+    mj.stats = (double*)(b + r.stats);
+    mj.partial = (double*)(b + r.partial);
+    mj.res = (double*)(b + r.res);
+    mj.simc_map = pj.simc_map;
+    off += r.total;
+    /* workgroups and LDS per stage */
+    const size_t kSmall = 65536;
+    auto add = [&](int stage, int nb, size_t lds) {
+        const int c = lds > kSmall ? 1 : 0;
+        blk[stage] = nb;
+        cls[stage] = c;
+        cnt.blocks[stage][c] += nb;
+        cnt.lds[stage][c] = std::max(cnt.lds[stage][c], lds);
+    };
+    add(MIX_PREP, ceil_div(wd, mj.cols) * hd, prep_lds_bytes(F, pj.ch, mj.cols));
+    add(MIX_ROWFFT, ceil_div(hd, t.grad_rb), line_lds_bytes(Px, t.grad_rb, 2));
+    add(MIX_COL, mj.nbx * 2, line_lds_bytes(Py, t.col_bx, 3));
+    size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);
+    if ((size_t)4 * t.row_rb * lds_pitch(Px.lds_len) * sizeof(float2) < kHistBins * sizeof(unsigned))
+        row_sh += kHistBins * sizeof(unsigned);
+    add(MIX_ROW, ceil_div(hd, t.row_rb) * 8, row_sh);
+    add(MIX_SELECT, 8, 0);
+    add(MIX_PCF, mj.pc_gx * mj.pc_gy, 0);
+    add(MIX_FIN, 1, 0);
+}
+
+void Ctx::mix_fsim_launch(const MixJob* d_jobs, const int* d_starts, int n, const MixCounts& cnt) {
+    struct K {
+        int stage;
+        const char* tag;
+        void (*fn)(const MixJob*, const int*, int);
+        int threads;
+    };
+    const K ks[] = {{MIX_PREP, "fsim_prep", mix_prep_kernel, kBlock},
+                    {MIX_ROWFFT, "fsim_rowfft", mix_rowfft_kernel, kBlock},
+                    {MIX_COL, "fsim_colpass", mix_colpass_kernel, kBlockFft},
+                    {MIX_ROW, "fsim_rowpass", mix_rowpass_kernel, kBlockFft},
+                    {MIX_SELECT, "fsim_select", mix_select_kernel, 1024},
+                    {MIX_PCF, "fsim_pcfinal", mix_pcfinal_kernel, kBlock},
+                    {MIX_FIN, "fsim_finalize", mix_finalize_kernel, kBlock}};
+    for (const K& k : ks)
+        for (int c = 0; c < 2; ++c)
+            if (cnt.blocks[k.stage][c] > 0)
+                launch(k.tag, k.fn, dim3(cnt.blocks[k.stage][c]), dim3(k.threads), cnt.lds[k.stage][c], d_jobs,
+                       d_starts + ((size_t)k.stage * 2 + c) * (n + 1), n);
 }
 
 }  // namespace iqm

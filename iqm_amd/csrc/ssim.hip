@@ -43,22 +43,20 @@ __device__ __forceinline__ float luma_at(const uint8_t* p, int ch) {
 }
 
 template <int R, int TX, int TY>
-__global__ __launch_bounds__(kBlock) void ssim_kernel(const uint8_t* __restrict__ img,
-                                                      const uint8_t* __restrict__ ref,
-                                                      long pair_stride, int w, int h, int ch,
-                                                      SsimWeights wt, float c1, float c2,
-                                                      float* __restrict__ map,
-                                                      double* __restrict__ partial) {
+__device__ __forceinline__ void ssim_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
+                                          long pair_stride, int w, int h, int ch, const SsimWeights& wt,
+                                          float c1, float c2, float* __restrict__ map,
+                                          double* __restrict__ partial, int bx, int by, int gx, int gy,
+                                          int pair) {
     constexpr int LW = TX + 2 * R, LH = TY + 2 * R;
     __shared__ float lx[LH][LW];
     __shared__ float ly[LH][LW];
     __shared__ float hp[5][LH][TX];
     __shared__ double red[kBlock / 64];
 
-    const int pair = blockIdx.z;
     const uint8_t* A = img + (long)pair * pair_stride;
     const uint8_t* B = ref + (long)pair * pair_stride;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int tid = threadIdx.x;
 
     const int cx = min(x0 + TX / 2, w - 1), cy = min(y0 + TY / 2, h - 1);
@@ -141,15 +139,24 @@ __global__ __launch_bounds__(kBlock) void ssim_kernel(const uint8_t* __restrict_
     if (tid == 0) {
         double s = 0.0;
         for (int i = 0; i < kBlock / 64; ++i) s += red[i];
-        partial[((long)pair * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+        partial[((long)pair * gy + by) * gx + bx] = s;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __restrict__ partial,
-                                                               int nblk, double count,
-                                                               double* __restrict__ out) {
+template <int R, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void ssim_kernel(const uint8_t* __restrict__ img,
+                                                      const uint8_t* __restrict__ ref,
+                                                      long pair_stride, int w, int h, int ch,
+                                                      SsimWeights wt, float c1, float c2,
+                                                      float* __restrict__ map,
+                                                      double* __restrict__ partial) {
+    ssim_body<R, TX, TY>(img, ref, pair_stride, w, h, ch, wt, c1, c2, map, partial, blockIdx.x, blockIdx.y,
+                         gridDim.x, gridDim.y, blockIdx.z);
+}
+
+__device__ __forceinline__ void ssim_finalize_body(const double* __restrict__ partial, int nblk, double count,
+                                                   double* __restrict__ out, int pair) {
     __shared__ double sh[kBlock];
-    const int pair = blockIdx.x;
     double s = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(long)pair * nblk + b];
     sh[threadIdx.x] = s;
@@ -159,6 +166,12 @@ __global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __r
         __syncthreads();
     }
     if (threadIdx.x == 0) out[pair] = sh[0] / count;
+}
+
+__global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __restrict__ partial,
+                                                               int nblk, double count,
+                                                               double* __restrict__ out) {
+    ssim_finalize_body(partial, nblk, count, out, blockIdx.x);
 }
 
 template <int R>
@@ -173,9 +186,9 @@ static void launch_ssim(Ctx& c, dim3 grid, const uint8_t* a, const uint8_t* b, l
                  c2, map, partial);
 }
 
-void Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
-                     long pair_stride, int n, int ks, float k1, float k2, float sigma,
-                     float* d_maps, double* mssim_host) {
+double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                        long pair_stride, int n, int ks, float k1, float k2, float sigma,
+                        float* d_maps, double* mssim_host) {
     IQM_CHECK(ks >= 3 && ks <= 2 * kMaxRadius + 1 && (ks % 2) == 1, IQM_ERR_INVALID,
               "kernel_size must be odd and in [3, 31]");
     const int R = (ks - 1) / 2;
@@ -206,6 +219,97 @@ void Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, i
         IQM_HIP(hipMemcpyAsync(mssim_host, out, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
         IQM_HIP(hipStreamSynchronize(stream_));
     }
+    return out;
+}
+
+/* ---- mixed batches ------------------------------------------------------ */
+
+__device__ __forceinline__ int ssim_mix_find(const int* __restrict__ st, int n, int b) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (st[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int R, int TX, int TY>
+__global__ __launch_bounds__(kBlock) void mix_ssim_kernel(const MixJob* __restrict__ jobs, const int* __restrict__ st,
+                                                          int n, SsimWeights wt, float c1, float c2) {
+    const int j = ssim_mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    const int local = blockIdx.x - st[j];
+    ssim_body<R, TX, TY>(J.img, J.ref, 0, J.w, J.h, J.ch, wt, c1, c2, J.ssim_map, J.ssim_partial,
+                         local % J.ssim_gx, local / J.ssim_gx, J.ssim_gx, J.ssim_gy, 0);
+}
+
+/* finalize, then the pair's MSSIM into its row of the batch result */
+__global__ __launch_bounds__(kBlock) void mix_ssim_finalize_kernel(const MixJob* __restrict__ jobs,
+                                                                   const int* __restrict__ st, int n) {
+    const int j = ssim_mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    ssim_finalize_body(J.ssim_partial, J.ssim_gx * J.ssim_gy, J.ssim_count, J.ssim_out, 0);
+    if (threadIdx.x == 0) J.out[0] = J.ssim_out[0];
+}
+
+static void ssim_tile(int R, int& TX, int& TY) {
+    TX = R <= 7 ? 64 : 32;
+    TY = R <= 7 ? 16 : 8;
+}
+
+void Ctx::mix_ssim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, MixCounts& cnt, int* blk,
+                        int* cls, const SsimArgs& sp) {
+    const int R = (sp.ks - 1) / 2;
+    int TX, TY;
+    ssim_tile(R, TX, TY);
+    mj.ssim_gx = ceil_div(pj.w, TX);
+    mj.ssim_gy = ceil_div(pj.h, TY);
+    mj.ssim_count = (double)((long)(pj.w - 2 * R + 1) * (long)(pj.h - 2 * R + 1));
+    mj.ssim_map = pj.ssim_map;
+    const int nb = mj.ssim_gx * mj.ssim_gy;
+    char* b = base ? base + off : nullptr;
+    mj.ssim_partial = (double*)b;
+    mj.ssim_out = (double*)(b + sizeof(double) * nb);
+    off += (sizeof(double) * (nb + 1) + 255) & ~(size_t)255;
+    blk[MIX_SSIM] = nb;
+    blk[MIX_SSIMFIN] = 1;
+    cls[MIX_SSIM] = cls[MIX_SSIMFIN] = 0;
+    cnt.blocks[MIX_SSIM][0] += nb;
+    cnt.blocks[MIX_SSIMFIN][0] += 1;
+}
+
+template <int R>
+static void launch_ssim_mix(Ctx& c, int nblocks, const MixJob* jobs, const int* st, int n, const SsimWeights& wt,
+                            float c1, float c2) {
+    if constexpr (R <= 7)
+        c.launch("ssim", mix_ssim_kernel<R, 64, 16>, dim3(nblocks), dim3(kBlock), 0, jobs, st, n, wt, c1, c2);
+    else
+        c.launch("ssim", mix_ssim_kernel<R, 32, 8>, dim3(nblocks), dim3(kBlock), 0, jobs, st, n, wt, c1, c2);
+}
+
+void Ctx::mix_ssim_launch(const MixJob* d_jobs, const int* d_starts, int n, const MixCounts& cnt,
+                          const SsimArgs& sp) {
+    if (cnt.blocks[MIX_SSIM][0] == 0) return;
+    const int R = (sp.ks - 1) / 2;
+    SsimWeights wt{};
+    const double sg = (double)sp.sigma;
+    for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
+    const float c1 = sp.k1 * sp.k1, c2 = sp.k2 * sp.k2;
+    const int* st = d_starts + (size_t)MIX_SSIM * 2 * (n + 1);
+    const int nb = cnt.blocks[MIX_SSIM][0];
+    switch (R) {
+#define IQM_SSIM_MIX_CASE(r) \
+    case r: launch_ssim_mix<r>(*this, nb, d_jobs, st, n, wt, c1, c2); break;
+        IQM_SSIM_MIX_CASE(1) IQM_SSIM_MIX_CASE(2) IQM_SSIM_MIX_CASE(3) IQM_SSIM_MIX_CASE(4) IQM_SSIM_MIX_CASE(5)
+        IQM_SSIM_MIX_CASE(6) IQM_SSIM_MIX_CASE(7) IQM_SSIM_MIX_CASE(8) IQM_SSIM_MIX_CASE(9) IQM_SSIM_MIX_CASE(10)
+        IQM_SSIM_MIX_CASE(11) IQM_SSIM_MIX_CASE(12) IQM_SSIM_MIX_CASE(13) IQM_SSIM_MIX_CASE(14) IQM_SSIM_MIX_CASE(15)
+#undef IQM_SSIM_MIX_CASE
+        default: IQM_CHECK(false, IQM_ERR_INVALID, "kernel_size must be odd and in [3, 31]");
+    }
+    launch("ssim_finalize", mix_ssim_finalize_kernel, dim3(cnt.blocks[MIX_SSIMFIN][0]), dim3(kBlock), 0, d_jobs,
+           d_starts + (size_t)MIX_SSIMFIN * 2 * (n + 1), n);
 }
 
 }  // namespace iqm
+

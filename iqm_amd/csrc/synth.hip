@@ -1,0 +1,247 @@
+// Seeded synthetic workloads generated on the device, with the recipe of the
+// host generator iqm_amd/synth.py (used by the benchmarks, where host
+// generation of hundreds of large pairs would take minutes):
+//   reference: three channels sqrt(0.8) s + sqrt(0.2) n_c of 1/f^2 fields
+//              (white N(0,1) -> DFT -> / |f| on the fftfreq grid, DC := 0 ->
+//              inverse DFT, real part), joint min-max rescale to u8;
+//   distort:   separable Gaussian blur (radius ceil(3 sigma), replicate
+//              border), contrast about the mean, additive N(0, sigma_n^2),
+//              16-level quantisation (17 * round(v / 17)), clip.
+// The random numbers are the host's: SplitMix64 outputs, Box-Muller in FP64.
+// The 1/f^2 fields are synthesised on the smallest 2/3/5/7-smooth grid that
+// holds the image and cropped, so for sizes that are already smooth the
+// fields use the host's sample order; the FFTs and the blur run in FP32
+// (the host uses FP64), so a rescaled or quantised value can differ by one
+// step where it lies on a rounding boundary.
+#include "ctx.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace iqm {
+
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+__host__ __device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+/* output idx (0-based) of the stream started at seed, as a double in (0, 1] */
+__host__ __device__ __forceinline__ double sm_uniform(uint64_t seed, uint64_t idx) {
+    const uint64_t z = sm_mix(seed + (idx + 1) * kGamma);
+    return ((double)(z >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+/* element i of the host's normal(seed, n): Box-Muller on uniforms 2(i/2), 2(i/2)+1 */
+__device__ __forceinline__ double sm_normal(uint64_t seed, uint64_t i) {
+    const uint64_t m = i >> 1;
+    const double u1 = sm_uniform(seed, 2 * m), u2 = sm_uniform(seed, 2 * m + 1);
+    const double r = sqrt(-2.0 * log(u1));
+    const double a = 6.283185307179586 * u2;
+    return (i & 1) ? r * sin(a) : r * cos(a);
+}
+
+struct GenSeeds {
+    uint64_t s[4];
+};
+
+__global__ __launch_bounds__(kBlock) void gen_white_kernel(float2* __restrict__ planes, long per, GenSeeds seeds) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per) return;
+    planes[(long)blockIdx.y * per + i] = make_float2((float)sm_normal(seeds.s[blockIdx.y], (uint64_t)i), 0.f);
+}
+
+/* spectrum / |f| on the fftfreq grid of W x H, DC := 0 */
+__global__ __launch_bounds__(kBlock) void gen_invf_kernel(float2* __restrict__ planes, int W, int H) {
+    const long per = (long)W * H;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per) return;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    const double fy = (double)(y < (H + 1) / 2 ? y : y - H) / H;
+    const double fx = (double)(x < (W + 1) / 2 ? x : x - W) / W;
+    const double f = sqrt(fx * fx + fy * fy);
+    float2* p = planes + (long)blockIdx.y * per + i;
+    *p = f > 0.0 ? make_float2((float)(p->x / f), (float)(p->y / f)) : make_float2(0.f, 0.f);
+}
+
+/* channels of the w x h crop and per-block min / max */
+__global__ __launch_bounds__(kBlock) void gen_combine_kernel(const float2* __restrict__ planes, int W, int H,
+                                                             int w, int h, float* __restrict__ rgb,
+                                                             float2* __restrict__ mm) {
+    __shared__ float lo_s[kBlock], hi_s[kBlock];
+    const long per = (long)W * H;
+    const long n = (long)w * h;
+    float lo = 3e38f, hi = -3e38f;
+    const float a = (float)sqrt(0.8), b = (float)sqrt(0.2);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(i / w), x = (int)(i - (long)y * w);
+        const long q = (long)y * W + x;
+        const float s = planes[q].x;
+        for (int c = 0; c < 3; ++c) {
+            const float v = a * s + b * planes[(long)(c + 1) * per + q].x;
+            rgb[i * 3 + c] = v;
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+    }
+    lo_s[threadIdx.x] = lo;
+    hi_s[threadIdx.x] = hi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            lo_s[threadIdx.x] = fminf(lo_s[threadIdx.x], lo_s[threadIdx.x + st]);
+            hi_s[threadIdx.x] = fmaxf(hi_s[threadIdx.x], hi_s[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mm[blockIdx.x] = make_float2(lo_s[0], hi_s[0]);
+}
+
+__global__ __launch_bounds__(kBlock) void gen_rescale_kernel(const float* __restrict__ rgb, long n3,
+                                                             const float2* __restrict__ mm, int nb,
+                                                             uint8_t* __restrict__ out) {
+    __shared__ float lo_s, hi_s;
+    if (threadIdx.x == 0) {
+        float lo = 3e38f, hi = -3e38f;
+        for (int i = 0; i < nb; ++i) {
+            lo = fminf(lo, mm[i].x);
+            hi = fmaxf(hi, mm[i].y);
+        }
+        lo_s = lo;
+        hi_s = hi;
+    }
+    __syncthreads();
+    const double lo = lo_s, hi = hi_s;
+    const double sc = 255.0 / (hi > lo ? hi - lo : 1.0);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x)
+        out[i] = (uint8_t)fmin(fmax(rint(((double)rgb[i] - lo) * sc), 0.0), 255.0);
+}
+
+struct BlurTaps {
+    float k[32];
+    int r;
+};
+
+/* vertical pass (the host blurs axis 0 first): replicate border */
+__global__ __launch_bounds__(kBlock) void gen_blur_v_kernel(const uint8_t* __restrict__ src, long row_stride,
+                                                            int w, int h, BlurTaps t, float* __restrict__ acc) {
+    const long n3 = (long)w * h * 3;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const long px = i / 3;
+    const int c = (int)(i - px * 3);
+    const int y = (int)(px / w), x = (int)(px - (long)y * w);
+    float s = 0.f;
+    for (int j = -t.r; j <= t.r; ++j) {
+        const int yy = min(max(y + j, 0), h - 1);
+        s += t.k[j + t.r] * (float)src[(long)yy * row_stride + (long)x * 3 + c];
+    }
+    acc[i] = s;
+}
+
+/* horizontal pass and per-block FP64 sums for the mean */
+__global__ __launch_bounds__(kBlock) void gen_blur_h_kernel(const float* __restrict__ acc, int w, int h, BlurTaps t,
+                                                            float* __restrict__ out, double* __restrict__ part) {
+    __shared__ double sh[kBlock];
+    const long n3 = (long)w * h * 3;
+    double sum = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        const long px = i / 3;
+        const int c = (int)(i - px * 3);
+        const int y = (int)(px / w), x = (int)(px - (long)y * w);
+        float s = 0.f;
+        for (int j = -t.r; j <= t.r; ++j) {
+            const int xx = min(max(x + j, 0), w - 1);
+            s += t.k[j + t.r] * acc[((long)y * w + xx) * 3 + c];
+        }
+        out[i] = s;
+        sum += s;
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void gen_finish_kernel(const float* __restrict__ v, long n3,
+                                                            const double* __restrict__ part, int nb,
+                                                            double contrast, double sigma_n, uint64_t nseed,
+                                                            uint8_t* __restrict__ out) {
+    __shared__ double mean_s;
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < nb; ++i) s += part[i];
+        mean_s = s / (double)n3;
+    }
+    __syncthreads();
+    const double m = mean_s;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        double x = m + contrast * ((double)v[i] - m);
+        x += sigma_n * sm_normal(nseed, (uint64_t)i);
+        x = 17.0 * rint(x / 17.0);
+        out[i] = (uint8_t)fmin(fmax(rint(x), 0.0), 255.0);
+    }
+}
+
+static int smooth_at_least(int n) {
+    for (int m = n;; ++m) {
+        int r = m;
+        for (int p : {2, 3, 5, 7})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+constexpr int kGenBlocks = 1024;
+
+void Ctx::synth_reference(int w, int h, uint64_t seed, uint8_t* d_out) {
+    IQM_CHECK(w >= 2 && h >= 2, IQM_ERR_INVALID, "synthetic image must be at least 2 x 2");
+    const int W = smooth_at_least(w), H = smooth_at_least(h);
+    IQM_CHECK(W <= 4096 && H <= 4096, IQM_ERR_UNSUPPORTED, "synthetic image side above 4096");
+    const long per = (long)W * H, n = (long)w * h;
+    const size_t b_planes = sizeof(float2) * 4 * per, b_rgb = sizeof(float) * 3 * n;
+    char* base = (char*)workspace(WS_GEN, b_planes + b_rgb + sizeof(float2) * kGenBlocks);
+    float2* planes = (float2*)base;
+    float* rgb = (float*)(base + b_planes);
+    float2* mm = (float2*)(base + b_planes + b_rgb);
+    GenSeeds s{{seed * 4 + 0, seed * 4 + 1, seed * 4 + 2, seed * 4 + 3}};
+    launch("gen_white", gen_white_kernel, dim3(ceil_div(per, kBlock), 4), dim3(kBlock), 0, planes, per, s);
+    fft2d_dev(planes, W, H, 4, -1, false);
+    launch("gen_invf", gen_invf_kernel, dim3(ceil_div(per, kBlock), 4), dim3(kBlock), 0, planes, W, H);
+    fft2d_dev(planes, W, H, 4, +1, true);
+    launch("gen_combine", gen_combine_kernel, dim3(kGenBlocks), dim3(kBlock), 0, (const float2*)planes, W, H, w, h,
+           rgb, mm);
+    launch("gen_rescale", gen_rescale_kernel, dim3(kGenBlocks), dim3(kBlock), 0, (const float*)rgb, 3 * n,
+           (const float2*)mm, kGenBlocks, d_out);
+}
+
+void Ctx::synth_distort(const uint8_t* d_src, long row_stride, int w, int h, uint64_t seed, uint8_t* d_out) {
+    IQM_CHECK(w >= 1 && h >= 1 && row_stride >= 3L * w, IQM_ERR_INVALID, "invalid distortion arguments");
+    /* the host's uniform(seed ^ 0x5DEECE66D, 4) */
+    const uint64_t ps = seed ^ 0x5DEECE66Dull;
+    const double sigma_b = 0.5 + 2.5 * sm_uniform(ps, 0);
+    const double contrast = 0.7 + 0.6 * sm_uniform(ps, 1);
+    const double sigma_n = 2.0 + 18.0 * sm_uniform(ps, 2);
+    BlurTaps t{};
+    t.r = (int)std::ceil(3.0 * sigma_b);
+    IQM_CHECK(2 * t.r + 1 <= 32, IQM_ERR_INVALID, "blur radius too large");
+    double ks = 0;
+    std::vector<double> k(2 * t.r + 1);
+    for (int j = -t.r; j <= t.r; ++j) ks += (k[j + t.r] = std::exp(-(double)(j * j) / (2.0 * sigma_b * sigma_b)));
+    for (int j = 0; j < 2 * t.r + 1; ++j) t.k[j] = (float)(k[j] / ks);
+    const long n3 = 3L * w * h;
+    char* base = (char*)workspace(WS_GEN, 2 * sizeof(float) * n3 + sizeof(double) * kGenBlocks);
+    float* acc = (float*)base;
+    float* v = acc + n3;
+    double* part = (double*)(base + 2 * sizeof(float) * n3);
+    launch("gen_blur_v", gen_blur_v_kernel, dim3(ceil_div(n3, kBlock)), dim3(kBlock), 0, d_src, row_stride, w, h, t,
+           acc);
+    launch("gen_blur_h", gen_blur_h_kernel, dim3(kGenBlocks), dim3(kBlock), 0, (const float*)acc, w, h, t, v, part);
+    launch("gen_finish", gen_finish_kernel, dim3(kGenBlocks), dim3(kBlock), 0, (const float*)v, n3,
+           (const double*)part, kGenBlocks, contrast, sigma_n, seed ^ 0x2545F4914F6CDD1Dull, d_out);
+}
+
+}  // namespace iqm

@@ -27,7 +27,7 @@ def _report(tag, **errs):
     print(f"[{tag}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
 
 
-@pytest.mark.parametrize("w,h", [(480, 270), (270, 480), (64, 48), (45, 36), (1, 8), (7, 1)])
+@pytest.mark.parametrize("w,h", [(480, 270), (270, 480), (64, 48), (45, 36), (1, 8), (7, 1), (97, 61), (1009, 3), (659, 17)])
 def test_fft2d_matches_numpy(ctx, w, h):
     rng = np.random.default_rng(w * 1000 + h)
     x = (rng.standard_normal((3, h, w)) + 1j * rng.standard_normal((3, h, w))).astype(np.complex64)
