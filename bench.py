@@ -16,6 +16,14 @@ shape:
          SSIM and FSIMc maps written, as one captured batch; against the same
          pairs as 512 separate per-method calls.  Algorithmic bytes per pair:
          2*3*W*H + 4*W*H + 32*N*33 + 4*N.
+  stream 600 consecutive 3840x2160 RGB frame pairs: crops of a seeded
+         6240x4560 1/f^2 canvas (seed 5) shifted by (dx, dy) in {1..4}^2 per
+         frame (seeds 5000+t), test frame = distortion of the crop (seed
+         6000+t), all made on the device; FSIMc in batches of 30 frames with
+         the network's PC and an exact spot check on every 30th frame
+         (fallback to exact above 0.02), against the exact path on the same
+         frames; plus a host-fed run (pinned host frames, double-buffered
+         H2D on a second stream overlapped with the exact FSIMc).
 
 Prints one JSON line.
 """
@@ -184,10 +192,143 @@ def run_mixed(args, ctx) -> dict:
     }
 
 
+def run_stream(args, ctx) -> dict:
+    import iqm_amd
+    from iqm_amd import synth
+
+    FW, FH, CW, CHH, B = 3840, 2160, 6240, 4560, 30
+    T = args.stream_frames
+    fb = FW * FH * 3
+    _, wd, hd = iqm_amd.fsim_dims(FW, FH)
+    timer = HipTimer(ctx.stream)
+    d_canvas = ctx.alloc(CW * CHH * 3)
+    timer.start()
+    ctx.synth_reference(CW, CHH, 5, d_canvas)
+    canvas_ms = timer.stop_ms()
+    offs, ox, oy = [], 0, 0
+    for t in range(T):
+        if t:
+            u = synth.uniform(5000 + t, 2)
+            ox += 1 + min(3, int(4 * u[0]))
+            oy += 1 + min(3, int(4 * u[1]))
+        offs.append((ox, oy))
+    assert ox <= CW - FW and oy <= CHH - FH
+    d_img, d_ref = ctx.alloc(B * fb), ctx.alloc(B * fb)
+    sur = iqm_amd.Surrogate(ctx, seed=1)
+    with open(os.path.join(ROOT, "iqm_amd", "weights", "pc_surrogate.iqmsur"), "rb") as f:
+        sur.load(f.read())
+    fsim = iqm_amd.FSIM()
+    if not args.no_tune:
+        ctx.autotune_fsim(FW, FH, 3, 8)
+    gen_ms = ex_ms = sp_ms = so_ms = 0.0
+    rows = []
+    for b0 in range(0, T, B):
+        nb = min(B, T - b0)
+        timer.start()
+        for i in range(nb):
+            t = b0 + i
+            off = (offs[t][1] * CW + offs[t][0]) * 3
+            ctx.copy2d(d_ref + i * fb, FW * 3, d_canvas + off, CW * 3, FW * 3, FH)
+            ctx.synth_distort(d_canvas + off, CW * 3, FW, FH, 6000 + t, d_img + i * fb)
+        gen_ms += timer.stop_ms()
+        timer.start()
+        fe, fce = fsim.compute_batch_dev(ctx, d_img, d_ref, FW, FH, 3, fb, nb)
+        ex_ms += timer.stop_ms()
+        timer.start()
+        fs, fcs, err, back = sur.fsim_batch_dev(d_img, d_ref, FW, FH, 3, fb, nb, spot_every=30, spot_tol=0.02)
+        sp_ms += timer.stop_ms()
+        timer.start()
+        fo, fco, _, _ = sur.fsim_batch_dev(d_img, d_ref, FW, FH, 3, fb, nb, spot_every=0)
+        so_ms += timer.stop_ms()
+        for i in range(nb):
+            rows.append((fe[i], fce[i], fs[i], fcs[i], fo[i], fco[i], err, back))
+        print(f"[stream] frames {b0}..{b0 + nb - 1}: exact fsimc {fce[0]:.5f} net {fco[0]:.5f} "
+              f"spot {err:.3f}{' fallback' if back else ''}", file=sys.stderr, flush=True)
+    a = np.array([r[:6] for r in rows])
+    # host-fed: pinned host frames, H2D of batch k+1 on a copy stream while batch k is scored
+    hip = _hip()
+    vp = ctypes.c_void_p
+    hip.hipHostMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint]
+    hip.hipHostFree.argtypes = [vp]
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipStreamDestroy.argtypes = [vp]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+    hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    hip.hipStreamWaitEvent.argtypes = [vp, vp, ctypes.c_uint]
+    hip.hipStreamSynchronize.argtypes = [vp]
+    host = vp()
+    assert hip.hipHostMalloc(ctypes.byref(host), 2 * B * fb, 0) == 0
+    assert hip.hipMemcpy(vp(host.value), vp(d_img), B * fb, 2) == 0
+    assert hip.hipMemcpy(vp(host.value + B * fb), vp(d_ref), B * fb, 2) == 0
+    bufs = [(d_img, d_ref), (ctx.alloc(B * fb), ctx.alloc(B * fb))]
+    cs = vp()
+    assert hip.hipStreamCreate(ctypes.byref(cs)) == 0
+    evs = [vp(), vp()]
+    for e in evs:
+        assert hip.hipEventCreate(ctypes.byref(e)) == 0
+
+    def h2d(k):
+        di, dr = bufs[k % 2]
+        assert hip.hipMemcpyAsync(vp(di), vp(host.value), B * fb, 1, cs) == 0
+        assert hip.hipMemcpyAsync(vp(dr), vp(host.value + B * fb), B * fb, 1, cs) == 0
+        assert hip.hipEventRecord(evs[k % 2], cs) == 0
+
+    K = max(2, T // B)
+    ctx.sync()
+    for k in range(2):  # warm: each buffer's first call builds its graph
+        h2d(k)
+        assert hip.hipStreamSynchronize(cs) == 0
+        fsim.compute_batch_dev(ctx, bufs[k][0], bufs[k][1], FW, FH, 3, fb, B)
+    t0 = time.perf_counter()
+    h2d(0)
+    for k in range(K):
+        if k + 1 < K:
+            h2d(k + 1)  # the buffer it fills was last read by batch k-1, finished below
+        assert hip.hipStreamWaitEvent(vp(ctx.stream), evs[k % 2], 0) == 0
+        fsim.compute_batch_dev(ctx, bufs[k % 2][0], bufs[k % 2][1], FW, FH, 3, fb, B)
+    fed_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(4):
+        h2d(k)
+    assert hip.hipStreamSynchronize(cs) == 0
+    h2d_gbs = 4 * 2 * B * fb / (time.perf_counter() - t0) / 1e9
+    hip.hipStreamDestroy(cs)
+    hip.hipHostFree(host)
+    for p in (bufs[1][0], bufs[1][1], d_img, d_ref, d_canvas):
+        ctx.free(p)
+    sur.close()
+    n = len(rows)
+    exact_bytes = 2 * 3 * FW * FH + 32 * wd * hd * 33 + 4 * wd * hd
+    return {
+        "metric": "4K stream FSIMc frames/s (network PC, exact spot check every 30th frame)",
+        "value": n / (sp_ms * 1e-3), "unit": "frames/s", "n_gpus": 1, "steps": n // B, "warmup": 0,
+        "ms_per_step": sp_ms / (n / B), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network, f32 FSIM", "data": "synthetic",
+        "config": {"workload": "stream_4k_600_frames", "frames": n, "frames_per_batch": B,
+                   "downscaled": [wd, hd], "spot_every": 30, "spot_tol": 0.02,
+                   "inputs": "device-resident, made on the device per batch"},
+        "exact_frames_per_s": n / (ex_ms * 1e-3),
+        "surrogate_only_frames_per_s": n / (so_ms * 1e-3),
+        "frame_synthesis_frames_per_s": n / (gen_ms * 1e-3),
+        "fallback_batches": int(sum(r[7] for r in rows[::B])),
+        "spot_err_median": float(np.median([r[6] for r in rows[::B]])),
+        "fsim_err_vs_exact": {"surrogate_only_mean_abs_dfsim": float(np.mean(np.abs(a[:, 4] - a[:, 0]))),
+                              "surrogate_only_max_abs_dfsim": float(np.max(np.abs(a[:, 4] - a[:, 0]))),
+                              "surrogate_only_mean_abs_dfsimc": float(np.mean(np.abs(a[:, 5] - a[:, 1]))),
+                              "reported_max_abs_dfsimc": float(np.max(np.abs(a[:, 3] - a[:, 1])))},
+        "exact_roofline_frac": exact_bytes * n / (ex_ms * 1e-3) / HBM_PEAK,
+        "host_fed": {"frames_per_s": K * B / fed_s, "h2d_GB_per_s": h2d_gbs,
+                     "bytes_per_frame_pair": 2 * fb, "path": "exact FSIMc, pinned host frames, 2 device buffers"},
+        "canvas_ms": canvas_ms,
+        "device": ctx.device_name,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed"])
+    ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed", "stream"])
+    ap.add_argument("--stream-frames", type=int, default=600)
     ap.add_argument("--mixed-pairs", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -211,8 +352,8 @@ def main() -> int:
     from iqm_amd import synth
 
     ctx = iqm_amd.Context(0)
-    if args.workload == "mixed":
-        out = run_mixed(args, ctx)
+    if args.workload in ("mixed", "stream"):
+        out = run_mixed(args, ctx) if args.workload == "mixed" else run_stream(args, ctx)
         ctx.close()
         print(json.dumps(out))
         return 0

@@ -204,13 +204,15 @@ int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img,
  * image uses seed + 1000003); iqm_synth_distort_dev distorts a w x h RGB view
  * with row stride src_row_stride bytes.  Both run on the context's stream. */
 int iqm_synth_pair_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_t* d_img, uint8_t* d_ref);
+/* The reference image of `seed` alone (sides up to 8192). */
+int iqm_synth_reference_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_t* d_out);
 int iqm_synth_distort_dev(iqm_ctx* ctx, const uint8_t* d_src, int64_t src_row_stride, int32_t w,
                           int32_t h, uint64_t seed, uint8_t* d_out);
 
 /* ---- FFT (replaces VkFFT: fsim.cpp:370-437 plan, :527-550 append) ------- */
 
 /* In-place batched 2-D C2C FFT of `batch` interleaved complex float planes
- * of w x h (w contiguous), host memory.  dir = -1 forward (unnormalised),
+ * of w x h (w contiguous), host memory (line lengths up to 8192 that fit the LDS).  dir = -1 forward (unnormalised),
  * +1 inverse; normalize != 0 divides by w*h. */
 int iqm_fft2d(iqm_ctx* ctx, float* data, int32_t w, int32_t h, int32_t batch, int32_t dir,
               int32_t normalize);

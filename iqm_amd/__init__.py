@@ -38,7 +38,7 @@ EXPORTS = (
     "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches",
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
-    "iqm_batch_dev",
+    "iqm_batch_dev", "iqm_synth_reference_dev",
 )
 
 JOB_SSIM = 1
@@ -151,6 +151,7 @@ def lib():
                                 ctypes.POINTER(PairResult)]
     L.iqm_synth_pair_dev.argtypes = [vp, i32, i32, u64, vp, vp]
     L.iqm_synth_distort_dev.argtypes = [vp, vp, i64, i32, i32, u64, vp]
+    L.iqm_synth_reference_dev.argtypes = [vp, i32, i32, u64, vp]
     L.iqm_memcpy2d_dev.argtypes = [vp, vp, i64, vp, i64, i64, i64]
     for name in EXPORTS:
         getattr(L, name)
@@ -235,6 +236,10 @@ class Context:
     def synth_pair(self, w: int, h: int, seed: int, d_img: int, d_ref: int) -> None:
         """Device copy of iqm_amd.synth.rgb_pair(w, h, seed) into packed RGB buffers."""
         _check(lib().iqm_synth_pair_dev(self.handle, w, h, int(seed), d_img, d_ref))
+
+    def synth_reference(self, w: int, h: int, seed: int, d_out: int) -> None:
+        """Device copy of iqm_amd.synth.rgb_reference(w, h, seed)."""
+        _check(lib().iqm_synth_reference_dev(self.handle, w, h, int(seed), d_out))
 
     def synth_distort(self, d_src: int, row_stride: int, w: int, h: int, seed: int, d_out: int) -> None:
         _check(lib().iqm_synth_distort_dev(self.handle, d_src, row_stride, w, h, int(seed), d_out))

@@ -678,6 +678,14 @@ int iqm_synth_pair_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_
     });
 }
 
+int iqm_synth_reference_dev(iqm_ctx* ctx, int32_t w, int32_t h, uint64_t seed, uint8_t* d_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_out, IQM_ERR_INVALID, "null buffer");
+        ctx->c->synth_reference(w, h, seed, d_out);
+    });
+}
+
 int iqm_synth_distort_dev(iqm_ctx* ctx, const uint8_t* d_src, int64_t src_row_stride, int32_t w,
                           int32_t h, uint64_t seed, uint8_t* d_out) {
     if (!ctx) return fail(IQM_ERR_INVALID, "null context");

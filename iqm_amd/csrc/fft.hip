@@ -79,8 +79,8 @@ std::vector<float2> twiddle_table(int n) {
 const LinePlanHost& Ctx::line_plan(int n) {
     auto it = line_plans_.find(n);
     if (it != line_plans_.end()) return it->second;
-    IQM_CHECK(n >= 1 && n <= 4096, IQM_ERR_UNSUPPORTED,
-              "FFT length " + std::to_string(n) + " outside [1, 4096]");
+    IQM_CHECK(n >= 1 && n <= 8192, IQM_ERR_UNSUPPORTED,
+              "FFT length " + std::to_string(n) + " outside [1, 8192]");
     LinePlanHost lp;
     LinePlan& P = lp.plan;
     P = LinePlan{};

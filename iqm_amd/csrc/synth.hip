@@ -200,7 +200,7 @@ constexpr int kGenBlocks = 1024;
 void Ctx::synth_reference(int w, int h, uint64_t seed, uint8_t* d_out) {
     IQM_CHECK(w >= 2 && h >= 2, IQM_ERR_INVALID, "synthetic image must be at least 2 x 2");
     const int W = smooth_at_least(w), H = smooth_at_least(h);
-    IQM_CHECK(W <= 4096 && H <= 4096, IQM_ERR_UNSUPPORTED, "synthetic image side above 4096");
+    IQM_CHECK(W <= 8192 && H <= 8192, IQM_ERR_UNSUPPORTED, "synthetic image side above 8192");
     const long per = (long)W * H, n = (long)w * h;
     const size_t b_planes = sizeof(float2) * 4 * per, b_rgb = sizeof(float) * 3 * n;
     char* base = (char*)workspace(WS_GEN, b_planes + b_rgb + sizeof(float2) * kGenBlocks);
