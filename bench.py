@@ -100,6 +100,39 @@ def cpu_baseline(pairs, budget_s: float) -> dict:
                       f"({med * 1e3:.0f} ms/pair) after 1 warm-up, OpenMP {threads} threads"}
 
 
+def surrogate_leg(args, ctx, fsim, d_img, d_ref, B, pair_bytes, fs_exact, fc_exact, n_unique) -> dict:
+    """The same batch with phase congruency from the trained network: pairs/s
+    with no spot check, pairs/s with the exact spot check on pair 0 of every
+    batch (fallback above 0.02), and |dFSIM| against the exact scores."""
+    import iqm_amd
+
+    sur = iqm_amd.Surrogate(ctx, seed=1)
+    with open(os.path.join(ROOT, "iqm_amd", "weights", "pc_surrogate.iqmsur"), "rb") as f:
+        sur.load(f.read())
+    timer = HipTimer(ctx.stream)
+    out = {}
+    for name, every in (("no_spot_check", 0), ("spot_check_every_batch", B)):
+        for _ in range(max(1, args.warmup)):
+            sur.fsim_batch_dev(d_img, d_ref, W, H, CH, pair_bytes, B, spot_every=every)
+        ctx.sync()
+        timer.start()
+        for _ in range(args.steps):
+            fs, fc, err, back = sur.fsim_batch_dev(d_img, d_ref, W, H, CH, pair_bytes, B, spot_every=every)
+        ms = timer.stop_ms() / args.steps
+        out[name] = {"pairs_per_s": B / (ms * 1e-3), "ms_per_step": ms}
+        if every:
+            out[name].update({"spot_err": err, "fell_back": back})
+        else:
+            d = np.abs(fs[:n_unique] - fs_exact[:n_unique])
+            dc = np.abs(fc[:n_unique] - fc_exact[:n_unique])
+            out["abs_dfsim"] = {"mean": float(d.mean()), "max": float(d.max()), "pairs": n_unique}
+            out["abs_dfsimc"] = {"mean": float(dc.mean()), "max": float(dc.max()), "pairs": n_unique}
+    sur.close()
+    out["speedup_vs_exact"] = None
+    out["network"] = "6 x conv3x3, 32 channels, BF16 MFMA, weights iqm_amd/weights/pc_surrogate.iqmsur"
+    return out
+
+
 def run_mixed(args, ctx) -> dict:
     """256 uneven pairs, SSIM + FSIMc each, one captured batch vs separate calls."""
     import iqm_amd
@@ -338,6 +371,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-stages", action="store_true", help="skip the per-kernel timing pass")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the network-PC leg")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
@@ -412,6 +446,10 @@ def main() -> int:
             stages[name] = {"avg_us": 1e3 * ms / cnt, "launches_per_step": cnt / nrep,
                             "us_per_step": 1e3 * ms / nrep}
 
+    surrogate = None
+    if not args.no_surrogate:
+        surrogate = surrogate_leg(args, ctx, fsim, d_img, d_ref, B, pair_bytes, fs, fc, len(uniq))
+
     bytes_pair = algorithmic_bytes_fsimc(W, H, CH, wd, hd)
     achieved = bytes_pair * B / (ms_per_step * 1e-3)
     dominant = max(stages.items(), key=lambda kv: kv[1]["us_per_step"])[0] if stages else None
@@ -439,8 +477,11 @@ def main() -> int:
         "stages_us": stages,
         "tiles": json.loads(ctx.export_tunes()).get("fsim", []),
         "dominant_kernel": dominant,
+        "surrogate": surrogate,
         "device": ctx.device_name,
     }
+    if surrogate:
+        surrogate["speedup_vs_exact"] = surrogate["no_spot_check"]["pairs_per_s"] / pairs_per_s
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(uniq, args.cpu_seconds)
     ctx.free(d_img)

@@ -81,9 +81,11 @@ __global__ __launch_bounds__(kBlock) void sur_conv_first(const float* __restrict
 
 /* Layers 2-5 (and their input gradient): 3x3, 32 -> 32, NHWC BF16, MFMA.
  * A 16 x 16 pixel tile per workgroup; wavefront v owns tile rows 4v..4v+3.
- * FWD: out = relu(conv(in, W) + b).  !FWD: out = conv(in, W) * (mask > 0)
- * (the input gradient through the ReLU of the layer below, with W the
- * flipped, transposed weights). */
+ * Weights arrive as [tap][co][ci] BF16 (9 x 32 x 32), so each lane loads its
+ * B fragments (k = ci, n = co) straight from global memory, where the 18 KiB
+ * of a layer stay cached.  FWD: out = relu(conv(in, W) + b).  !FWD: out =
+ * conv(in, W) * (mask > 0) (the input gradient through the ReLU of the layer
+ * below, with W the flipped, transposed weights). */
 template <bool FWD>
 __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restrict__ in,
                                                        const uint16_t* __restrict__ wgt,
@@ -91,16 +93,17 @@ __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restric
                                                        const uint16_t* __restrict__ mask,
                                                        uint16_t* __restrict__ out, int h, int w) {
     __shared__ __align__(16) uint16_t tin[18 * 18 * 32];
-    __shared__ __align__(16) uint16_t tw[9 * 32 * 32];
     __shared__ __align__(16) uint16_t tout[16 * 16 * 32];
     const int img = blockIdx.z, x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
     const long HW = (long)h * w;
     const uint16_t* src = in + (long)img * HW * 32;
-    /* weights OIHW [co][ci][tap] -> [tap][co][ci] */
-    for (int i = threadIdx.x; i < 9 * 32 * 32; i += blockDim.x) {
-        const int co = i / 288, r = i - co * 288, ci = r / 9, t = r - ci * 9;
-        tw[(t * 32 + co) * 32 + ci] = wgt[i];
-    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    bf16x8 B[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&wgt[(t * 32 + nb * 16 + li) * 32 + 8 * g];
     /* input tile with a one-pixel zero halo, 16-byte chunks of 8 channels */
     for (int i = threadIdx.x; i < 18 * 18 * 4; i += blockDim.x) {
         const int px = i >> 2, part = i & 3;
@@ -110,19 +113,12 @@ __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restric
         if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = ((const uint4*)(src + ((long)gy * w + gx) * 32))[part];
         ((uint4*)tin)[i] = v;
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, g = lane >> 4;
-    bf16x8 B[9][2];
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&tw[(t * 32 + nb * 16 + li) * 32 + 8 * g];
     float b0 = 0.f, b1 = 0.f;
     if (FWD) {
         b0 = bias[li];
         b1 = bias[16 + li];
     }
+    __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
         const int r = wave * 4 + rr;
@@ -150,15 +146,27 @@ __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restric
     __syncthreads();
     uint16_t* dst = out + (long)img * HW * 32;
     const uint16_t* msk = mask ? mask + (long)img * HW * 32 : nullptr;
-    for (int i = threadIdx.x; i < 16 * 16 * 32; i += blockDim.x) {
-        const int px = i >> 5, c = i & 31;
-        const int ly = px >> 4, lx = px & 15;
-        const int gy = y0 + ly, gx = x0 + lx;
+    for (int i = threadIdx.x; i < 16 * 16 * 4; i += blockDim.x) {
+        const int px = i >> 2, part = i & 3;
+        const int gy = y0 + (px >> 4), gx = x0 + (px & 15);
         if (gy >= h || gx >= w) continue;
-        const long o = ((long)gy * w + gx) * 32 + c;
-        uint16_t v = tout[i];
-        if (!FWD && bf2f(msk[o]) <= 0.f) v = 0;
-        dst[o] = v;
+        const long o = ((long)gy * w + gx) * 32;
+        uint4 v = ((const uint4*)tout)[i];
+        if (!FWD) {
+            /* keep a gradient where the activation below is > 0 (BF16 bits
+             * 0x0001..0x7fff) */
+            const uint4 m = ((const uint4*)(msk + o))[part];
+            uint32_t* vv = (uint32_t*)&v;
+            const uint32_t* mm = (const uint32_t*)&m;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t lo = mm[k] & 0xffffu, hi = mm[k] >> 16;
+                const uint32_t keep = ((lo != 0u && lo < 0x8000u) ? 0x0000ffffu : 0u) |
+                                      ((hi != 0u && hi < 0x8000u) ? 0xffff0000u : 0u);
+                vv[k] &= keep;
+            }
+        }
+        ((uint4*)(dst + o))[part] = v;
     }
 }
 
@@ -432,8 +440,9 @@ __global__ __launch_bounds__(kBlock) void sur_adam(float* __restrict__ w, const 
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-/* BF16 copies of the mid-layer weights: OIHW and the flipped transpose
- * Wt[ci][co][2-ky][2-kx] used for the input gradient. */
+/* BF16 copies of the mid-layer weights in the matrix-core B layout
+ * [tap][n][k]: forward W[tap][co][ci], and for the input gradient the flipped
+ * transpose Wt[tap'][ci][co] = W[co][ci][8 - tap']. */
 __global__ __launch_bounds__(kBlock) void sur_cast(const float* __restrict__ master,
                                                    uint16_t* __restrict__ wbf, uint16_t* __restrict__ wbt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,8 +450,8 @@ __global__ __launch_bounds__(kBlock) void sur_cast(const float* __restrict__ mas
     const int l = i / 9216, r = i - l * 9216;
     const int co = r / 288, r2 = r - co * 288, ci = r2 / 9, t = r2 - ci * 9;
     const uint16_t b = f2bf(master[SurLayout::wl(l + 2) + r]);
-    wbf[i] = b;
-    wbt[l * 9216 + (ci * 32 + co) * 9 + (8 - t)] = b;
+    wbf[l * 9216 + (t * 32 + co) * 32 + ci] = b;
+    wbt[l * 9216 + ((8 - t) * 32 + ci) * 32 + co] = b;
 }
 
 /* ---- training data ----------------------------------------------------- */

@@ -33,8 +33,8 @@ struct SurModel {
     float* grad = nullptr;     /* FP32 gradient of the last step             */
     float* adam_m = nullptr;
     float* adam_v = nullptr;
-    uint16_t* wbf = nullptr;   /* BF16 copy of the mid-layer weights, OIHW    */
-    uint16_t* wbf_t = nullptr; /* flipped / transposed copy for the input gradient */
+    uint16_t* wbf = nullptr;   /* BF16 mid-layer weights, [layer][tap][co][ci] */
+    uint16_t* wbf_t = nullptr; /* flipped transpose for the input gradient, [layer][tap][ci][co] */
     long step = 0;
     uint64_t seed = 0;
     uint64_t data_cursor = 0;  /* patches consumed (for resume)              */
