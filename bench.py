@@ -3,11 +3,17 @@
 A step is one batched FSIM/FSIMc evaluation of `--batch` device-resident
 1920x1080 RGB pairs (inputs already in HBM), including the copy of the
 per-pair scalars back to the host.  Timing uses HIP events recorded on the
-library's own stream.  The roofline object prices the step with the fixed
-algorithmic byte model of one exact FSIMc pair,
+library's own stream.  `roofline` is the dominant kernel's: its algorithmic
+bytes per launch (per-pair stage bytes of `kernel_bytes_per_pair` x the
+batch) over its average launch time from a per-launch HIP-event pass, with
+`traffic` = HBM bytes per launch from the committed rocprofv3 PMC summary.
+`pipeline_roofline` prices the whole step with the fixed model of one exact
+FSIMc pair,
     2*3*W*H + 32*N*33 + 4*N   bytes   (N = 480*270 after the 4x downscale),
-against the MI355X HBM peak of 8 TB/s.  The CPU baseline is the FP32 CPU
-restatement of the same algorithm (the oracle library) timed on this host.
+against the MI355X HBM peak of 8 TB/s.  The `surrogate` object times the
+same batch with phase congruency from the trained network.  The CPU
+baseline is the FP32 CPU restatement of the same algorithm (the oracle
+library) timed on this host.
 
 Other workloads (`--workload`), each printing one JSON line of the same
 shape:
@@ -78,6 +84,37 @@ class HipTimer:
 def algorithmic_bytes_fsimc(w: int, h: int, c_in: int, wd: int, hd: int) -> int:
     N = wd * hd
     return 2 * c_in * w * h + 32 * N * 33 + 4 * N
+
+
+def kernel_bytes_per_pair(w: int, h: int, c_in: int, wd: int, hd: int) -> dict:
+    """Algorithmic HBM bytes of each FSIM stage for one pair: what the stage
+    must read and write once (the per-size filter bank, L2-resident across
+    pairs, is not counted)."""
+    N = wd * hd
+    return {
+        "fsim_prep": 2 * c_in * w * h + 6 * N * 4,          # u8 pair in, 6 YIQ planes out
+        "fsim_rowfft": 2 * N * 4 + 2 * N * 8,               # Y_in, Y_ref in; 2 row spectra out
+        "fsim_colpass": 2 * N * 8 + 32 * N * 8,             # spectra in; 32 response planes out
+        "fsim_rowpass": 32 * N * 8 + 24 * N * 4,            # 32 planes in; An, E (16) + keys (8) out
+        "fsim_select": 2 * 8 * N * 4,                       # keys, two radix-select passes
+        "fsim_pcfinal": 6 * N * 4 + 16 * N * 4,             # YIQ + An/E in
+        "fsim_finalize": 0,
+    }
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (two --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated against a
+    device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "fsim_rowpass_1080p_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch[{kernel}] ="):
+                    return int(line.split("=")[1].split()[0]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def cpu_baseline(pairs, budget_s: float) -> dict:
@@ -453,6 +490,21 @@ def main() -> int:
     bytes_pair = algorithmic_bytes_fsimc(W, H, CH, wd, hd)
     achieved = bytes_pair * B / (ms_per_step * 1e-3)
     dominant = max(stages.items(), key=lambda kv: kv[1]["us_per_step"])[0] if stages else None
+    kbytes = kernel_bytes_per_pair(W, H, CH, wd, hd)
+    if dominant:
+        dom_bytes = kbytes[dominant] * B  # one launch covers the whole batch
+        dom_us = stages[dominant]["avg_us"]
+        traffic, traffic_src = read_traffic(dominant)
+        roofline = {"bound": "hbm", "achieved": dom_bytes / (dom_us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": dom_bytes / (dom_us * 1e-6) / HBM_PEAK, "traffic": traffic,
+                    "kernel": dominant, "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_us,
+                    "traffic_source": traffic_src}
+    else:
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "whole launch sequence"}
+    for name, st in stages.items():
+        if name in kbytes and kbytes[name]:
+            st["GB_per_s"] = kbytes[name] * B / (st["avg_us"] * 1e-6) / 1e9
     out = {
         "metric": "1080p FSIMc pairs/s (exact phase congruency)",
         "value": pairs_per_s,
@@ -469,10 +521,10 @@ def main() -> int:
         "config": {"workload": "fsimc_exact_1920x1080_rgb", "pairs_per_step": B,
                    "unique_pairs": len(uniq), "downscaled": [wd, hd], "inputs": "device-resident u8 RGB",
                    "fsim_pair0": float(fs[0]), "fsimc_pair0": float(fc[0])},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "one FSIMc batch launch sequence (all kernels of the step)",
-                     "alg_bytes_per_pair": bytes_pair},
+        "roofline": roofline,
+        "pipeline_roofline": {"achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK, "alg_bytes_per_pair": bytes_pair,
+                              "model": "2*C*W*H + 32*N*33 + 4*N per pair over the whole step"},
         "wall_pairs_per_s": B * args.steps / wall_s,
         "stages_us": stages,
         "tiles": json.loads(ctx.export_tunes()).get("fsim", []),

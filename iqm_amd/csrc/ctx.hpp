@@ -43,7 +43,7 @@ struct FsimTune {
     int two_level = 1; /* use the register two-level passes when the size has a split */
     int col_bx = 16;   /* columns per column-pass workgroup */
     int row_rb = 2;    /* rows per row-pass workgroup */
-    int fft_rb = 4;    /* rows per row-FFT workgroup */
+    int fft_rb = 8;    /* rows per row-FFT workgroup */
     double us_per_pair = 0; /* measured by the autotuner (0 = default, not measured) */
 };
 
