@@ -22,6 +22,9 @@ shape:
          SSIM and FSIMc maps written, as one captured batch; against the same
          pairs as 512 separate per-method calls.  Algorithmic bytes per pair:
          2*3*W*H + 4*W*H + 32*N*33 + 4*N.
+  ssim   the 512x512 gray pair of the plain-SSIM workload (seed 1) as a
+         batch of 256 device-resident pairs with the map written; algorithmic
+         bytes per pair 2*1*W*H + 4*W*H + 8; the FP32 CPU restatement beside it.
   stream 600 consecutive 3840x2160 RGB frame pairs: crops of a seeded
          6240x4560 1/f^2 canvas (seed 5) shifted by (dx, dy) in {1..4}^2 per
          frame (seeds 5000+t), test frame = distortion of the crop (seed
@@ -262,6 +265,61 @@ def run_mixed(args, ctx) -> dict:
     }
 
 
+def run_ssim(args, ctx) -> dict:
+    import iqm_amd
+    from iqm_amd import synth
+
+    w = h = 512
+    B = 256
+    img, ref = synth.ssim_gray_pair(w, h, seed=1)
+    d_img, d_ref = ctx.alloc(B * w * h), ctx.alloc(B * w * h)
+    ctx.upload(d_img, np.ascontiguousarray(np.broadcast_to(img, (B, h, w))))
+    ctx.upload(d_ref, np.ascontiguousarray(np.broadcast_to(ref, (B, h, w))))
+    d_map = ctx.alloc(4 * B * w * h)
+    ssim = iqm_amd.SSIM()
+    for _ in range(args.warmup):
+        ssim.compute_batch_dev(ctx, d_img, d_ref, w, h, 1, w * h, B, d_map)
+    timer = HipTimer(ctx.stream)
+    ctx.sync()
+    timer.start()
+    for _ in range(args.steps):
+        m = ssim.compute_batch_dev(ctx, d_img, d_ref, w, h, 1, w * h, B, d_map)
+    ms = timer.stop_ms() / args.steps
+    single = ssim.compute_metric(ctx, img, ref, want_map=False).mssim
+    assert np.all(m == single)
+    stages = {}
+    ctx.reset_stage_times()
+    ctx.set_stage_timing(True)
+    ssim.compute_batch_dev(ctx, d_img, d_ref, w, h, 1, w * h, B, d_map)
+    ctx.set_stage_timing(False)
+    for name, (t, cnt) in ctx.stage_times().items():
+        stages[name] = {"avg_us": 1e3 * t / cnt}
+    for p in (d_img, d_ref, d_map):
+        ctx.free(p)
+    per_pair = 2 * 1 * w * h + 4 * w * h + 8
+    k_us = stages["ssim"]["avg_us"]
+    out = {"metric": "512x512 gray SSIM pairs/s (map + MSSIM)", "value": B / (ms * 1e-3), "unit": "pairs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "ssim_512x512_gray_batch256", "pairs_per_step": B, "mssim": float(m[0])},
+           "roofline": {"bound": "hbm", "achieved": per_pair * B / (k_us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": per_pair * B / (k_us * 1e-6) / HBM_PEAK, "traffic": None,
+                        "kernel": "ssim", "alg_bytes_per_launch": per_pair * B, "avg_launch_us": k_us},
+           "stages_us": stages, "device": ctx.device_name}
+    if not args.no_cpu:
+        from oracle import oracle
+        oracle.ssim(img, ref, prec=32)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.ssim(img, ref, prec=32)
+            times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "pairs/s", "cores": oracle.get_threads(), "kind": "port",
+                               "sample": f"FP32 CPU restatement, 1 pair 512x512 gray, median of 5 ({med*1e3:.1f} ms)"}
+    return out
+
+
 def run_stream(args, ctx) -> dict:
     import iqm_amd
     from iqm_amd import synth
@@ -397,7 +455,7 @@ def run_stream(args, ctx) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed", "stream"])
+    ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed", "stream", "ssim"])
     ap.add_argument("--stream-frames", type=int, default=600)
     ap.add_argument("--mixed-pairs", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20)
@@ -423,8 +481,8 @@ def main() -> int:
     from iqm_amd import synth
 
     ctx = iqm_amd.Context(0)
-    if args.workload in ("mixed", "stream"):
-        out = run_mixed(args, ctx) if args.workload == "mixed" else run_stream(args, ctx)
+    if args.workload in ("mixed", "stream", "ssim"):
+        out = {"mixed": run_mixed, "stream": run_stream, "ssim": run_ssim}[args.workload](args, ctx)
         ctx.close()
         print(json.dumps(out))
         return 0

@@ -59,23 +59,62 @@ __device__ __forceinline__ void ssim_body(const uint8_t* __restrict__ img, const
     const int x0 = bx * TX, y0 = by * TY;
     const int tid = threadIdx.x;
 
+    /* the bytes of the tile's rows (clipped to the image) of both images are
+     * staged as 16-byte aligned chunks in the space of hp, with every load of
+     * a thread issued before its LDS stores; an aligned 16-byte block holding
+     * a byte of the image lies in that byte's page */
+    const int c0 = max(x0 - R, 0), cend = min(x0 + TX + R, w);
+    const int r0 = max(y0 - R, 0), rend = min(y0 + TY + R, h);
+    const int seg = (cend - c0) * ch, nrows = rend - r0;
+    const int rc = seg / 16 + 2;
+    static_assert(2 * LH * ((LW * 4) / 16 + 2) * 16 <= (int)sizeof(hp), "SSIM staging does not fit");
+    uint4* stage = (uint4*)&hp[0][0][0];
+    const uint8_t* sb = (const uint8_t*)stage;
+    {
+        const int total = 2 * nrows * rc;
+        for (int base = 0; base < total; base += 8 * (int)blockDim.x) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = base + u * blockDim.x + tid;
+                v[u] = make_uint4(0u, 0u, 0u, 0u);
+                if (i < total) {
+                    const int rr = i / rc, k = i - rr * rc;
+                    const int im = rr >= nrows, r = rr - im * nrows;
+                    const uintptr_t a0 = (uintptr_t)((im ? B : A) + ((long)(r0 + r) * w + c0) * ch);
+                    const uintptr_t a = (a0 & ~(uintptr_t)15) + (uintptr_t)k * 16;
+                    if (a <= ((a0 + seg - 1) & ~(uintptr_t)15)) v[u] = *(const uint4*)a;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = base + u * blockDim.x + tid;
+                if (i < total) stage[i] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    /* byte address in the staging of pixel (gx, gy) of image im */
+    auto staged = [&](int im, int gx, int gy) {
+        const uintptr_t a0 = (uintptr_t)((im ? B : A) + ((long)gy * w + c0) * ch);
+        return sb + ((im * nrows + (gy - r0)) * rc) * 16 + (int)(a0 & 15) + (gx - c0) * ch;
+    };
     const int cx = min(x0 + TX / 2, w - 1), cy = min(y0 + TY / 2, h - 1);
-    const float sx = luma_at(A + ((long)cy * w + cx) * ch, ch);
-    const float sy = luma_at(B + ((long)cy * w + cx) * ch, ch);
+    const float sx = luma_at(staged(0, cx, cy), ch);
+    const float sy = luma_at(staged(1, cx, cy), ch);
 
     for (int i = tid; i < LH * LW; i += blockDim.x) {
         const int ry = i / LW, rx = i - ry * LW;
         const int gx = x0 - R + rx, gy = y0 - R + ry;
         float a = 0.f, b = 0.f;
         if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-            const long o = ((long)gy * w + gx) * ch;
-            a = luma_at(A + o, ch) - sx;
-            b = luma_at(B + o, ch) - sy;
+            a = luma_at(staged(0, gx, gy), ch) - sx;
+            b = luma_at(staged(1, gx, gy), ch) - sy;
         }
         lx[ry][rx] = a;
         ly[ry][rx] = b;
     }
-    __syncthreads();
+    __syncthreads(); /* hp is written next */
 
     for (int i = tid; i < LH * TX; i += blockDim.x) {
         const int ry = i / TX, c = i - ry * TX;
@@ -100,33 +139,53 @@ __device__ __forceinline__ void ssim_body(const uint8_t* __restrict__ img, const
     __syncthreads();
 
     double crop = 0.0;
+    /* each thread finishes RPT adjacent rows of one column, reading every
+     * input row of hp once (the sums keep the tap order d = 0..2R) */
     constexpr int ROWS_PER_PASS = kBlock / TX;
+    constexpr int RPT = TY / ROWS_PER_PASS;
+    static_assert(TY % ROWS_PER_PASS == 0, "rows per thread");
     const int c = tid % TX;
+    const int rb = (tid / TX) * RPT;
     const int x = x0 + c;
     float gxs = 0.f;
 #pragma unroll
     for (int d = -R; d <= R; ++d)
         if (x + d >= 0 && x + d < w) gxs += wt.g[d + R];
-    for (int r = tid / TX; r < TY; r += ROWS_PER_PASS) {
-        const int y = y0 + r;
+    float m[RPT][5];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) m[q][k] = 0.f;
+#pragma unroll
+    for (int j = 0; j < RPT + 2 * R; ++j) {
+        float hv[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) hv[k] = hp[k][rb + j][c];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int d = j - q;
+            if (d >= 0 && d <= 2 * R) {
+                const float g = wt.g[d];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) m[q][k] += g * hv[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int y = y0 + rb + q;
         if (x >= w || y >= h) continue;
-        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, gys = 0.f;
+        float gys = 0.f;
 #pragma unroll
         for (int d = 0; d <= 2 * R; ++d) {
-            const float g = wt.g[d];
-            m0 += g * hp[0][r + d][c];
-            m1 += g * hp[1][r + d][c];
-            m2 += g * hp[2][r + d][c];
-            m3 += g * hp[3][r + d][c];
-            m4 += g * hp[4][r + d][c];
             const int yy = y + d - R;
-            if (yy >= 0 && yy < h) gys += g;
+            if (yy >= 0 && yy < h) gys += wt.g[d];
         }
         const float inv = 1.0f / (gxs * gys);
-        const float ma = m0 * inv, mb = m1 * inv;
-        const float va = m2 * inv - ma * ma;
-        const float vb = m3 * inv - mb * mb;
-        const float cov = m4 * inv - ma * mb;
+        const float ma = m[q][0] * inv, mb = m[q][1] * inv;
+        const float va = m[q][2] * inv - ma * ma;
+        const float vb = m[q][3] * inv - mb * mb;
+        const float cov = m[q][4] * inv - ma * mb;
         const float mi = ma + sx, mr = mb + sy;
         const float v = ((2.0f * mi * mr + c1) * (2.0f * cov + c2)) /
                         ((mi * mi + mr * mr + c1) * (va + vb + c2));
