@@ -185,34 +185,37 @@ const FsimSizePlan& Ctx::fsim_plan(int wd, int hd) {
 constexpr int kHistBins = 1024; /* level-1 radix-select bins: key bits 30..21 */
 
 /* F2: fsim_downsample.glsl:38-54.  A workgroup produces `cols` downscaled
- * pixels of one row for both images: the bytes of the F full-resolution rows
- * they cover are staged in LDS for both images at once, as 16-byte aligned
- * chunks (an aligned 16-byte block that holds a byte of the image lies in
- * that byte's page, so these loads never leave mapped memory).  Each thread
- * then sums the R, G and B bytes of its F x F box as integers (exact; zero
- * outside the image) and applies the YIQ matrix once, divided by F^2 --
- * the same value as the shader's per-pixel sum in exact arithmetic.  yiq
- * planes per pair: Y_in, Y_ref, I_in, Q_in, I_ref, Q_ref.  Workgroup (0, 0)
- * of each pair also clears the pair's median histograms. */
+ * pixels of `prb` consecutive rows for both images: the bytes of the
+ * full-resolution rows they cover are staged in LDS for both images at once,
+ * as 16-byte aligned chunks (an aligned 16-byte block that holds a byte of the
+ * image lies in that byte's page, so these loads never leave mapped memory),
+ * with each thread's loads issued before its LDS stores.  Each thread then
+ * sums the R, G and B bytes of its F x F box as integers (exact; zero outside
+ * the image) and applies the YIQ matrix once, divided by F^2 -- the same
+ * value as the shader's per-pixel sum in exact arithmetic.  yiq planes per
+ * pair: Y_in, Y_ref, I_in, Q_in, I_ref, Q_ref.  Workgroup (0, 0) of each pair
+ * also clears the pair's median histograms. */
 constexpr int kPrepMaxCols = 256;
 constexpr int kPrepMaxRows = 64;
 
 /* 16-byte chunks staged per full-resolution row for a segment of seg bytes */
 __host__ __device__ __forceinline__ int prep_row_chunks(int seg) { return seg / 16 + 2; }
 
+static size_t prep_lds_bytes(int F, int ch, int cols, int prb) {
+    return (size_t)2 * prb * F * prep_row_chunks(cols * F * ch) * 16;
+}
 /* output columns per prep workgroup: the widest power of two whose two
  * staged images fit kMaxLdsBytes */
-static int prep_cols(int F, int ch) {
+static int prep_cols(int F, int ch, int prb) {
     int c = kPrepMaxCols;
-    while (c > 1 && (size_t)2 * F * prep_row_chunks(c * F * ch) * 16 > (size_t)kMaxLdsBytes) c >>= 1;
+    while (c > 1 && prep_lds_bytes(F, ch, c, prb) > (size_t)kMaxLdsBytes) c >>= 1;
     return c;
 }
-static size_t prep_lds_bytes(int F, int ch, int cols) { return (size_t)2 * F * prep_row_chunks(cols * F * ch) * 16; }
 
 __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
                                                long pair_stride, int w, int h, int ch, int F, int wd, int hd,
-                                               int cols, float* __restrict__ yiq, unsigned* __restrict__ hist,
-                                               int bx, int by, int pair) {
+                                               int cols, int prb, float* __restrict__ yiq,
+                                               unsigned* __restrict__ hist, int bx, int by, int pair) {
     extern __shared__ uint4 stage4[];
     __shared__ int rowoff[2 * kPrepMaxRows];
     const uint8_t* stage = (const uint8_t*)stage4;
@@ -221,41 +224,42 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
         unsigned* hp = hist + (long)pair * 8 * kHistBins;
         for (int i = tid; i < 8 * kHistBins; i += blockDim.x) hp[i] = 0u;
     }
-    const int y = by;
+    const int y0 = by * prb;
+    const int nout = min(prb, hd - y0);
     const int xd0 = bx * cols;
     const int ncols = min(cols, wd - xd0);
     /* full-resolution window of this block, clipped to the image */
     const int c_lo = xd0 * F - F / 2, c_hi = (xd0 + ncols) * F - F / 2;
-    const int r_lo = y * F - F / 2;
+    const int r_lo = y0 * F - F / 2, r_hi = (y0 + nout) * F - F / 2;
     const int cc0 = max(c_lo, 0), cc1 = min(c_hi, w);
-    const int rr0 = max(r_lo, 0), rr1 = min(r_lo + F, h);
+    const int rr0 = max(r_lo, 0), rr1 = min(r_hi, h);
     const int seg = max(cc1 - cc0, 0) * ch; /* bytes of each staged row */
     const int nrows = max(rr1 - rr0, 0);
     const int rc = prep_row_chunks(seg);     /* chunks per staged row */
     const uint8_t* src0 = img + (long)pair * pair_stride;
     const uint8_t* src1 = ref + (long)pair * pair_stride;
     const int per_im = nrows * rc;
-    const int nr2 = 2 * nrows; /* staged rows: image 0 rows, then image 1 rows */
-    /* thread = chunk column; 8 rows of loads are issued before their LDS stores */
-    for (int k = tid; k < rc; k += blockDim.x) {
-        for (int r0 = 0; r0 < nr2; r0 += 8) {
-            uint4 v[8];
+    /* staged rows: image 0 rows, then image 1 rows; (row, chunk) flattened
+     * over the workgroup, 8 loads per thread in flight */
+    const int total = 2 * per_im;
+    for (int base = 0; base < total; base += 8 * (int)blockDim.x) {
+        uint4 v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int rr = r0 + u;
-                v[u] = make_uint4(0u, 0u, 0u, 0u);
-                if (rr < nr2) {
-                    const int im = rr >= nrows, r = rr - im * nrows;
-                    const uintptr_t a0 = (uintptr_t)((im ? src1 : src0) + ((long)(rr0 + r) * w + cc0) * ch);
-                    const uintptr_t first = a0 & ~(uintptr_t)15;
-                    const uintptr_t last = (a0 + seg - 1) & ~(uintptr_t)15;
-                    const uintptr_t a = first + (uintptr_t)k * 16;
-                    if (a <= last) v[u] = *(const uint4*)a;
-                }
+        for (int u = 0; u < 8; ++u) {
+            const int i = base + u * blockDim.x + tid;
+            v[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < total) {
+                const int rr = i / rc, k = i - rr * rc;
+                const int im = rr >= nrows, r = rr - im * nrows;
+                const uintptr_t a0 = (uintptr_t)((im ? src1 : src0) + ((long)(rr0 + r) * w + cc0) * ch);
+                const uintptr_t a = (a0 & ~(uintptr_t)15) + (uintptr_t)k * 16;
+                if (a <= ((a0 + seg - 1) & ~(uintptr_t)15)) v[u] = *(const uint4*)a;
             }
+        }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (r0 + u < nr2) stage4[(r0 + u) * rc + k] = v[u];
+        for (int u = 0; u < 8; ++u) {
+            const int i = base + u * blockDim.x + tid;
+            if (i < total) stage4[i] = v[u];
         }
     }
     if (tid < 2 * nrows) {
@@ -267,16 +271,18 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
     const long N = (long)wd * hd;
     float* base = yiq + (long)pair * 6 * N;
     const float inv = 1.0f / ((float)F * (float)F);
-    for (int t = tid; t < ncols; t += blockDim.x) {
-        const int x = xd0 + t;
-        const int xs = x * F - F / 2;
+    for (int t = tid; t < ncols * nout; t += blockDim.x) {
+        const int oy = t / ncols, xc = t - oy * ncols;
+        const int x = xd0 + xc, y = y0 + oy;
+        const int xs = x * F - F / 2, ys = y * F - F / 2;
         const int j0 = max(xs, cc0), j1 = min(xs + F, cc1);
+        const int k0 = max(ys, rr0), k1 = min(ys + F, rr1);
         const int nb = (j1 - j0) * ch;
         const long idx = (long)y * wd + x;
         for (int im = 0; im < 2; ++im) {
             int sr = 0, sg = 0, sb = 0;
-            for (int r = 0; r < nrows; ++r) {
-                const uint8_t* p = stage + rowoff[im * nrows + r] + (j0 - cc0) * ch;
+            for (int k = k0; k < k1; ++k) {
+                const uint8_t* p = stage + rowoff[im * nrows + (k - rr0)] + (j0 - cc0) * ch;
                 if (ch == 1) {
                     for (int b = 0; b < nb; ++b) sr += p[b];
                 } else {
@@ -299,10 +305,10 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
 __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
                                                            const uint8_t* __restrict__ ref,
                                                            long pair_stride, int w, int h,
-                                                           int ch, int F, int wd, int hd, int cols,
+                                                           int ch, int F, int wd, int hd, int cols, int prb,
                                                            float* __restrict__ yiq,
                                                            unsigned* __restrict__ hist) {
-    fsim_prep_body(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, yiq, hist, blockIdx.x, blockIdx.y,
+    fsim_prep_body(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist, blockIdx.x, blockIdx.y,
                    blockIdx.z);
 }
 
@@ -861,11 +867,13 @@ Ctx::FsimLayout Ctx::fsim_layout(long N, int n, char* base) const {
 void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stride, int w, int h, int ch,
                       int F, int wd, int hd, int n, float* yiq, unsigned* hist) {
     IQM_CHECK(F <= kPrepMaxRows, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-    const int cols = prep_cols(F, ch);
-    const size_t bytes = prep_lds_bytes(F, ch, cols);
+    int prb = std::max(1, env_int("IQM_FSIM_PREP_RB", fsim_tune(wd, hd).prep_rb));
+    while (prb > 1 && prb * F > kPrepMaxRows) --prb;
+    const int cols = prep_cols(F, ch, prb);
+    const size_t bytes = prep_lds_bytes(F, ch, cols, prb);
     IQM_CHECK(bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, cols), hd, n), dim3(kBlock), bytes, d_img, d_ref,
-           pair_stride, w, h, ch, F, wd, hd, cols, yiq, hist);
+    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, cols), ceil_div(hd, prb), n), dim3(kBlock), bytes,
+           d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist);
 }
 
 double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
@@ -1005,8 +1013,8 @@ __global__ __launch_bounds__(kBlock) void mix_prep_kernel(const MixJob* __restri
     const MixJob& J = jobs[j];
     const int local = blockIdx.x - st[j];
     const int gx = (J.wd + J.cols - 1) / J.cols;
-    fsim_prep_body(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, J.yiq, J.hist, local % gx, local / gx,
-                   0);
+    fsim_prep_body(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx,
+                   local / gx, 0);
 }
 
 __global__ __launch_bounds__(kBlock) void mix_rowfft_kernel(const MixJob* __restrict__ jobs,
@@ -1076,7 +1084,7 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     mj.hd = hd;
     mj.N = N;
     mj.invN = (float)(1.0 / (double)N);
-    mj.cols = prep_cols(F, pj.ch);
+    mj.cols = prep_cols(F, pj.ch, 1);
     mj.grad_rb = t.grad_rb;
     mj.col_bx = t.col_bx;
     mj.nbx = ceil_div(wd, t.col_bx);
@@ -1111,7 +1119,7 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
         cnt.blocks[stage][c] += nb;
         cnt.lds[stage][c] = std::max(cnt.lds[stage][c], lds);
     };
-    add(MIX_PREP, ceil_div(wd, mj.cols) * hd, prep_lds_bytes(F, pj.ch, mj.cols));
+    add(MIX_PREP, ceil_div(wd, mj.cols) * hd, prep_lds_bytes(F, pj.ch, mj.cols, 1));
     add(MIX_ROWFFT, ceil_div(hd, t.grad_rb), line_lds_bytes(Px, t.grad_rb, 2));
     add(MIX_COL, mj.nbx * 2, line_lds_bytes(Py, t.col_bx, 3));
     size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);

@@ -49,14 +49,16 @@ FsimTune Ctx::autotune_fsim(int w, int h, int ch, int n) {
     for (int tl = split ? 1 : 0; tl >= 0; --tl)
         for (int bx : {8, 16})
             for (int rb : {1, 2, 4})
-                for (int fr : {2, 4, 8}) {
-                    FsimTune t;
-                    t.two_level = tl;
-                    t.col_bx = bx;
-                    t.row_rb = rb;
-                    t.fft_rb = fr;
-                    cand.push_back(t);
-                }
+                for (int fr : {2, 4, 8})
+                    for (int pr : {1, 2}) {
+                        FsimTune t;
+                        t.two_level = tl;
+                        t.col_bx = bx;
+                        t.row_rb = rb;
+                        t.fft_rb = fr;
+                        t.prep_rb = pr;
+                        cand.push_back(t);
+                    }
     const bool was_timing = timing_;
     timing_ = false;
     hipEvent_t e0 = take_event(), e1 = take_event();
@@ -95,7 +97,7 @@ std::string Ctx::export_tunes() const {
         const FsimTune& t = kv.second;
         o << (first ? "" : ", ") << "{\"wd\": " << kv.first.first << ", \"hd\": " << kv.first.second
           << ", \"two_level\": " << t.two_level << ", \"col_bx\": " << t.col_bx
-          << ", \"row_rb\": " << t.row_rb << ", \"fft_rb\": " << t.fft_rb
+          << ", \"row_rb\": " << t.row_rb << ", \"fft_rb\": " << t.fft_rb << ", \"prep_rb\": " << t.prep_rb
           << ", \"us_per_pair\": " << t.us_per_pair << "}";
         first = false;
     }
@@ -120,19 +122,21 @@ void Ctx::import_tunes(const std::string& json) {
         const size_t end = json.find('}', pos);
         IQM_CHECK(end != std::string::npos, IQM_ERR_INVALID, "tune plan: unterminated entry");
         const std::string obj = json.substr(pos, end - pos + 1);
-        double wd, hd, tl, bx, rb, fr, us = 0;
+        double wd, hd, tl, bx, rb, fr, us = 0, pr = 1;
         if (!field(obj, "wd", wd)) continue; /* the outer object */
         IQM_CHECK(field(obj, "hd", hd) && field(obj, "two_level", tl) && field(obj, "col_bx", bx) &&
                       field(obj, "row_rb", rb) && field(obj, "fft_rb", fr),
                   IQM_ERR_INVALID, "tune plan: entry misses a field");
         field(obj, "us_per_pair", us);
-        IQM_CHECK(bx >= 1 && bx <= 64 && rb >= 1 && rb <= 16 && fr >= 1 && fr <= 64, IQM_ERR_INVALID,
-                  "tune plan: tile out of range");
+        field(obj, "prep_rb", pr); /* absent in older plan files */
+        IQM_CHECK(bx >= 1 && bx <= 64 && rb >= 1 && rb <= 16 && fr >= 1 && fr <= 64 && pr >= 1 && pr <= 16,
+                  IQM_ERR_INVALID, "tune plan: tile out of range");
         FsimTune t;
         t.two_level = (int)tl;
         t.col_bx = (int)bx;
         t.row_rb = (int)rb;
         t.fft_rb = (int)fr;
+        t.prep_rb = (int)pr;
         t.us_per_pair = us;
         set_fsim_tune((int)wd, (int)hd, t);
         ++count;
