@@ -178,8 +178,11 @@ int iqm_sur_loss_grad(iqm_sur* s, const float* d_x, const float* d_target, int32
 /* One Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr. */
 int iqm_sur_train_step(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
                        int32_t w, float lr, double* loss);
-/* Network output for n images of h x w (device in, device out). */
-int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w);
+/* Network output for n images of h x w (device in, device out).  fused != 0:
+ * one kernel for all six layers (activations stay in LDS; what the FSIM path
+ * uses); 0: the layer-by-layer kernels of training. */
+int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w,
+                  int32_t fused);
 /* Checkpoint (weights, Adam moments, step, seed, cursor) as a byte blob;
  * returns its length and writes it when cap allows.  -1 on error. */
 int64_t iqm_sur_save(iqm_sur* s, uint8_t* buf, int64_t cap);

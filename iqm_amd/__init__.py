@@ -141,7 +141,7 @@ def lib():
     L.iqm_sur_make_patches.argtypes = [vp, i32, i32, u64, vp, vp]
     L.iqm_sur_loss_grad.argtypes = [vp, vp, vp, i32, i32, i32, dp]
     L.iqm_sur_train_step.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, dp]
-    L.iqm_sur_infer.argtypes = [vp, vp, vp, i32, i32, i32]
+    L.iqm_sur_infer.argtypes = [vp, vp, vp, i32, i32, i32, i32]
     L.iqm_sur_save.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
     L.iqm_sur_save.restype = i64
     L.iqm_sur_load.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
@@ -473,8 +473,10 @@ class Surrogate:
         _check(lib().iqm_sur_train_step(self.handle, d_x, d_t, n, h, w, float(lr), _dptr(loss)))
         return float(loss[0])
 
-    def infer(self, d_x: int, d_out: int, n: int, h: int, w: int) -> None:
-        _check(lib().iqm_sur_infer(self.handle, d_x, d_out, n, h, w))
+    def infer(self, d_x: int, d_out: int, n: int, h: int, w: int, fused: bool = False) -> None:
+        """Layer-by-layer forward (the training kernels), or with fused=True the
+        single all-layer kernel the FSIM path uses."""
+        _check(lib().iqm_sur_infer(self.handle, d_x, d_out, n, h, w, 1 if fused else 0))
 
     def save(self) -> bytes:
         n = int(lib().iqm_sur_save(self.handle, None, 0))

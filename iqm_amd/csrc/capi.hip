@@ -551,11 +551,11 @@ int iqm_sur_train_step(iqm_sur* s, const float* d_x, const float* d_target, int3
     });
 }
 
-int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w) {
+int iqm_sur_infer(iqm_sur* s, const float* d_x, float* d_out, int32_t n, int32_t h, int32_t w, int32_t fused) {
     SUR_GUARD(s);
     return guarded([&] {
         sur_check_shape(d_x, d_out, n, h, w);
-        s->s->infer(d_x, d_out, n, h, w);
+        s->s->infer(d_x, d_out, n, h, w, fused != 0);
         s->ctx->c->sync();
     });
 }

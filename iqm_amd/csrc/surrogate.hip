@@ -20,17 +20,8 @@ namespace iqm {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
-#ifdef __HIP_DEVICE_COMPILE__
-    uint32_t u = __float_as_uint(f);
-#else
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-#endif
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u); /* round to nearest even */
-    return (uint16_t)(u >> 16);
-}
+/* FP32 -> BF16 bits, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950) */
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 /* ---- forward ----------------------------------------------------------- */
@@ -204,6 +195,213 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
         }
     }
     out[p] = s;
+}
+
+/* All six layers of the forward pass in one kernel, for inference.  A
+ * workgroup streams a strip of kFuSW output columns of one image down its
+ * rows; every layer's activations stay in LDS in a ring of 4 rows.  Layer l
+ * (1..6) works on row s - 2(l-1) at step s, so all layers advance together
+ * with one barrier per step: layer l reads rows r-1..r+1 of layer l-1, which
+ * were written in earlier steps, while layer l-1 writes row r+2 into the
+ * fourth slot.  The halo shrinks by one column per layer (layer l covers
+ * kFuSW + 2(6-l) columns), and positions outside the image are written as
+ * zeros, which is the zero padding of the layer-by-layer network, so layers
+ * 1-5 produce the same bits as sur_conv_first / sur_conv_mid.  Waves 2k and
+ * 2k+1 own layer k+2 (B fragments of both 16-channel halves in registers for
+ * the whole kernel, pixel tiles split between the two waves); the FP32 layers
+ * 1 and 6 are shared by all threads. */
+constexpr int kFuSW = 64;                          /* output columns per strip */
+constexpr int kFuPitch = 84;                       /* ring row pitch (pixels) */
+constexpr int kFuXPitch = 80;                      /* input ring row pitch (floats) */
+constexpr int kFuThreads = 512;
+constexpr size_t kFuRingBytes = (size_t)5 * 4 * kFuPitch * 32 * 2;
+constexpr size_t kFuLds = (size_t)4 * kFuXPitch * 4 + kFuRingBytes;
+
+__global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
+                                                               long istride, float scale,
+                                                               const float* __restrict__ params,
+                                                               const uint16_t* __restrict__ wbf,
+                                                               float* __restrict__ out, int h, int w) {
+    extern __shared__ __align__(16) uint8_t fu_smem[];
+    __shared__ float b6s;
+    float* xr = (float*)fu_smem;                                  /* [4][kFuXPitch] */
+    uint16_t* ring = (uint16_t*)(fu_smem + 4 * kFuXPitch * 4);    /* [layer 1..5][4][kFuPitch][32] */
+    auto R = [&](int layer, int row, int p) { return ring + (((layer - 1) * 4 + (row & 3)) * kFuPitch + p) * 32; };
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kFuSW;
+    const int j = blockIdx.y;
+    const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
+    float* oj = out + (long)j * h * w;
+
+    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
+    if (tid == 0) b6s = params[SurLayout::b6];
+    /* this wave's layer and its B fragments (k = ci, n = co) and biases */
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int ml = 2 + (wave >> 1), half = wave & 1;
+    const uint16_t* wl = wbf + (ml - 2) * 9216;
+    bf16x8 B[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
+    const float bias0 = params[SurLayout::bl(ml) + li], bias1 = params[SurLayout::bl(ml) + 16 + li];
+    /* layer 1: thread = (channel l1co, pixels 5 l1pg ..); layer 6: thread =
+     * (pixel tid / 8, channels 4 (tid % 8) ..); their BF16-rounded weights */
+    /* which of this lane's 12 mid-layer outputs lie inside the layer's columns
+     * and the image (bit e: tile 2h, 4 + e: tile 2h+1, 8 + e: tile 4) */
+    unsigned inmask = 0;
+    {
+        const int wl_cols = kFuSW + 2 * (6 - ml);
+        for (int e = 0; e < 4; ++e) {
+            const int ps[3] = {2 * half * 16 + 4 * g + e, (2 * half + 1) * 16 + 4 * g + e, 64 + 4 * g + e};
+            for (int k = 0; k < 3; ++k) {
+                const int gx = x0 - (6 - ml) + ps[k];
+                if (ps[k] < wl_cols && gx >= 0 && gx < w) inmask |= 1u << (4 * k + e);
+            }
+        }
+    }
+    const int l1co = tid & 31, l1pg = tid >> 5;
+    float w1r[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
+    const float b1r = params[SurLayout::b1 + l1co];
+    float w6r[4][9];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) w6r[k][t] = bf2f(f2bf(params[SurLayout::w6 + (4 * (tid & 7) + k) * 9 + t]));
+    __syncthreads();
+    /* input rows 0 and 1 (row -1 stays zero) */
+    for (int i = tid; i < 2 * (kFuSW + 12); i += blockDim.x) {
+        const int r = i / (kFuSW + 12), q = i - r * (kFuSW + 12);
+        const int gx = x0 - 6 + q;
+        xr[r * kFuXPitch + q] = (r < h && gx >= 0 && gx < w) ? xi[(long)r * w + gx] * scale : 0.f;
+    }
+    /* input rows are loaded one step before they are stored to LDS, so the
+     * load's latency overlaps a step of work */
+    const int xq = tid, xgx = x0 - 6 + tid;
+    const bool xload = tid < kFuSW + 12 && xgx >= 0 && xgx < w;
+    float xnext = (xload && 2 < h) ? xi[2L * w + xgx] : 0.f;
+    __syncthreads();
+
+    for (int s = 0; s < h + 10; ++s) {
+        /* input row s + 2 (loaded last step); issue row s + 3 */
+        if (tid < kFuSW + 12) {
+            xr[((s + 2) & 3) * kFuXPitch + xq] = xnext * scale;
+            xnext = (xload && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
+        }
+        auto layer1 = [&]() {
+        /* layer 1, row s: kFuSW + 10 columns x 32 channels; thread = (channel,
+         * 5 adjacent pixels) with the channel's weights in registers */
+        {
+            const int r = s;
+            const int p0 = l1pg * 5;
+            if (p0 < kFuSW + 10) {
+                float xw[3][7];
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) xw[dy][q] = xr[((r + dy - 1) & 3) * kFuXPitch + p0 + q];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const int p = p0 + k;
+                    const int gx = x0 - 5 + p;
+                    float v = 0.f;
+                    if (r < h && gx >= 0 && gx < w) {
+                        v = b1r;
+#pragma unroll
+                        for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
+                        v = fmaxf(v, 0.f);
+                    }
+                    if (p < kFuSW + 10) R(1, r, p)[l1co] = f2bf(v);
+                }
+            }
+        }
+        };
+        auto layers_mid = [&]() {
+        /* layers 2-5 on the matrix cores: a layer row is 5 pixel tiles x 2
+         * channel halves; wave `half` of the layer takes tiles 2h and 2h+1
+         * (both halves, one A load per two instructions) and half h of tile
+         * 4, as five independent accumulator chains */
+        {
+            static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
+            const int r = s - 2 * (ml - 1);
+            const int wl_cols = kFuSW + 2 * (6 - ml);
+            if (r >= 0) {
+                const bool live = r < h;
+                const int ta = 2 * half, tb = 2 * half + 1;
+                f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, b0 = a0, b1 = a0, c0 = a0;
+                if (live) {
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) {
+                        const uint16_t* row = R(ml - 1, r + t / 3 - 1, li + t % 3) + 8 * g;
+                        const bf16x8 A = *(const bf16x8*)(row + ta * 16 * 32);
+                        const bf16x8 Bv = *(const bf16x8*)(row + tb * 16 * 32);
+                        const bf16x8 C = *(const bf16x8*)(row + 4 * 16 * 32);
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][0], a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bv, B[t][0], b0, 0, 0, 0);
+                        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(C, half ? B[t][1] : B[t][0], c0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][1], a1, 0, 0, 0);
+                        b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bv, B[t][1], b1, 0, 0, 0);
+                    }
+                }
+                /* D layout: lane holds pixels 4g..4g+3 of a tile, channel li */
+                const float bc = half ? bias1 : bias0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int pa = ta * 16 + 4 * g + e, pb = tb * 16 + 4 * g + e, pc = 64 + 4 * g + e;
+                    const bool ia = live && ((inmask >> e) & 1u), ib = live && ((inmask >> (4 + e)) & 1u),
+                               ic = live && ((inmask >> (8 + e)) & 1u);
+                    uint16_t* da = R(ml, r, pa);
+                    uint16_t* db = R(ml, r, pb);
+                    uint16_t* dc = R(ml, r, pc);
+                    da[li] = ia ? f2bf(fmaxf(a0[e] + bias0, 0.f)) : (uint16_t)0;
+                    da[16 + li] = ia ? f2bf(fmaxf(a1[e] + bias1, 0.f)) : (uint16_t)0;
+                    db[li] = ib ? f2bf(fmaxf(b0[e] + bias0, 0.f)) : (uint16_t)0;
+                    db[16 + li] = ib ? f2bf(fmaxf(b1[e] + bias1, 0.f)) : (uint16_t)0;
+                    dc[16 * half + li] = ic ? f2bf(fmaxf(c0[e] + bc, 0.f)) : (uint16_t)0;
+                }
+            }
+        }
+        };
+        auto layer6 = [&]() {
+        /* layer 6, row s - 10: thread = (pixel, group of 4 channels) */
+        {
+            const int r = s - 10;
+            const int p = tid >> 3, cg = tid & 7;
+            float v = 0.f;
+            if (r >= 0 && r < h) {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    const uint2 a = *(const uint2*)(R(5, r + t / 3 - 1, p + t % 3) + 4 * cg);
+                    v += w6r[0][t] * __uint_as_float(a.x << 16);
+                    v += w6r[1][t] * __uint_as_float(a.x & 0xffff0000u);
+                    v += w6r[2][t] * __uint_as_float(a.y << 16);
+                    v += w6r[3][t] * __uint_as_float(a.y & 0xffff0000u);
+                }
+            }
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            const int gx = x0 + p;
+            if (cg == 0 && r >= 0 && r < h && gx < w) oj[(long)r * w + gx] = v + b6s;
+        }
+        };
+        /* the layers of a step touch disjoint ring rows, so their order is
+         * free: waves 0-3 run the FP32 layers first, waves 4-7 the matrix-core
+         * layer first, and each SIMD (waves k and k+4) overlaps the two */
+        if (wave < 4) {
+            layer1();
+            layer6();
+            layers_mid();
+        } else {
+            layers_mid();
+            layer1();
+            layer6();
+        }
+        lds_barrier();
+    }
 }
 
 /* ---- backward ---------------------------------------------------------- */
@@ -686,13 +884,23 @@ void Surrogate::forward(int nimg, int h, int w) {
     launch_forward(c_, m, acts_.x, 2L * h * w, (long)h * w, 1.0f, acts_.a, acts_.pred, nimg, h, w);
 }
 
-void Surrogate::infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
-                              int nimg, int h, int w) {
-    ensure_acts(nimg, h, w);
-    launch_forward(c_, m, d_x, pstride, istride, scale, acts_.a, d_out, nimg, h, w);
+static void launch_fused(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale, float* out,
+                         int nimg, int h, int w) {
+    IQM_CHECK(nimg <= 65535, IQM_ERR_INVALID, "too many images for one fused launch");
+    c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps, is,
+             scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
 }
 
-void Surrogate::infer(const float* d_x, float* d_out, int nimg, int h, int w) {
+void Surrogate::infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
+                              int nimg, int h, int w) {
+    launch_fused(c_, m, d_x, pstride, istride, scale, d_out, nimg, h, w);
+}
+
+void Surrogate::infer(const float* d_x, float* d_out, int nimg, int h, int w, bool fused) {
+    if (fused) {
+        launch_fused(c_, m, d_x, 2L * h * w, (long)h * w, 1.0f, d_out, nimg, h, w);
+        return;
+    }
     ensure_acts(nimg, h, w);
     launch_forward(c_, m, d_x, 2L * h * w, (long)h * w, 1.0f, acts_.a, d_out, nimg, h, w);
 }

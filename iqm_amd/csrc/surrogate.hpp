@@ -59,9 +59,11 @@ class Surrogate {
     Surrogate(const Surrogate&) = delete;
     Surrogate& operator=(const Surrogate&) = delete;
 
-    /* forward of nimg images of h x w (device FP32 Y/255 in, FP32 PC out) */
-    void infer(const float* d_x, float* d_out, int nimg, int h, int w);
-    /* forward reading image j at d_x + (j/2)*pstride + (j%2)*istride, scaled */
+    /* forward of nimg images of h x w (device FP32 Y/255 in, FP32 PC out):
+     * fused = one kernel for all layers (inference), else the layer-by-layer
+     * kernels that training uses */
+    void infer(const float* d_x, float* d_out, int nimg, int h, int w, bool fused = true);
+    /* fused forward reading image j at d_x + (j/2)*pstride + (j%2)*istride, scaled */
     void infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
                        int nimg, int h, int w);
     /* one Adam step on a minibatch; returns the MSE loss (synchronises) */

@@ -70,6 +70,30 @@ def test_forward_matches_torch(ctx, sur, n, h, w):
     assert err.max() <= 2e-2 * scale and err.mean() <= 2e-3 * scale
 
 
+@pytest.mark.parametrize("n,h,w", [(3, 37, 29), (1, 5, 70), (2, 270, 480), (1, 1, 7), (1, 131, 200)])
+def test_fused_forward_matches_layers(ctx, sur, n, h, w):
+    """The single all-layer kernel equals the layer-by-layer kernels: layers
+    1-5 bit for bit, layer 6 up to FP32 summation order."""
+    print(f"[sur fused] n={n} {w}x{h}", flush=True)
+    p = sur_ref.he_uniform(300 + w)
+    sur.set_params(p)
+    x = np.random.default_rng(h + w).uniform(0, 1, (n, h, w)).astype(np.float32)
+    dx, d1, d2 = _dev(ctx, x), ctx.alloc(x.nbytes), ctx.alloc(x.nbytes)
+    sur.infer(dx, d1, n, h, w, fused=False)
+    sur.infer(dx, d2, n, h, w, fused=True)
+    a = ctx.download(d1, np.empty_like(x))
+    b = ctx.download(d2, np.empty_like(x))
+    for d_ in (dx, d1, d2):
+        ctx.free(d_)
+    scale = float(np.max(np.abs(a)))
+    err = float(np.max(np.abs(a - b)))
+    print(f"[sur fused] max |fused - layers| = {err:.3e} (scale {scale:.3e})", flush=True)
+    assert scale > 1e-3 and err <= 1e-5 * scale
+    if n * h * w <= 20000:
+        ref = sur_ref.predict(p, x)
+        assert np.max(np.abs(b - ref)) <= 2e-2 * float(np.max(np.abs(ref)))
+
+
 def test_gradient_matches_torch(ctx, sur):
     n, h, w = 4, 40, 36
     print(f"[sur gradient] n={n} {w}x{h}", flush=True)
