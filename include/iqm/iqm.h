@@ -73,7 +73,7 @@ typedef struct {
     float* yiq_ref;
     float* grad_in;     /* Scharr magnitude (F3)                               */
     float* grad_ref;
-    float* an_e;        /* 8 x 2 planes (An, E) for [img*4 + o] (F12)          */
+    float* an_e;        /* 2 x 5 planes per image: sum_o An, E_0..E_3 (F12)    */
     float* keys;        /* 8 planes |EO_{img, s=0, o}|^2 (F13 median input)    */
     double em_n[4];     /* F10 */
     double est[8];      /* F14 */

@@ -384,7 +384,7 @@ class FSIM:
         if detail:
             det = FsimDetail()
             for name, planes in (("yiq_in", 3), ("yiq_ref", 3), ("grad_in", 1), ("grad_ref", 1),
-                                 ("an_e", 16), ("keys", 8)):
+                                 ("an_e", 10), ("keys", 8)):
                 arr = np.empty((planes, hd, wd) if planes > 1 else (hd, wd), np.float32)
                 dk[name] = arr
                 setattr(det, name, arr.ctypes.data)

@@ -367,7 +367,7 @@ int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* f
             }
             d2h(detail->grad_in, L.grad, N);
             d2h(detail->grad_ref, L.grad + N, N);
-            d2h(detail->an_e, L.an_e, 16 * N);
+            d2h(detail->an_e, L.an_e, 2 * iqm::kAnePlanes * N);
             d2h(detail->keys, L.keys, 8 * N);
             IQM_HIP(hipMemcpyAsync(stats, L.stats, sizeof(stats), hipMemcpyDeviceToHost, c.stream()));
         }

@@ -32,6 +32,10 @@ struct FsimSizePlan {
     double est[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+/* Planes per image of the row-pass output read by the PC kernel:
+ * sum over orientations of An, then E_o for o = 0..3. */
+constexpr int kAnePlanes = 5;
+
 /* Noise constants handed to the median/threshold kernel by value. */
 struct FsimNoiseArgs {
     double em_n[4];
@@ -156,7 +160,7 @@ class Ctx {
         long N;
         float* yiq;    /* [pair][6][N]: Y_in, Y_ref, I_in, Q_in, I_ref, Q_ref */
         float* grad;   /* [pair][2][N] */
-        float* an_e;   /* [pair][8][2][N] */
+        float* an_e;   /* [pair][2][kAnePlanes][N]: sum_o An, E_0..E_3 */
         float* keys;   /* [pair][8][N] */
         double* stats; /* [pair][8][4]: median, noise power, T, - */
         double* res;   /* [pair][8] */

@@ -198,8 +198,9 @@ constexpr int kHistBins = 1024; /* level-1 radix-select bins: key bits 30..21 */
 constexpr int kPrepMaxCols = 256;
 constexpr int kPrepMaxRows = 64;
 
-/* 16-byte chunks staged per full-resolution row for a segment of seg bytes */
-__host__ __device__ __forceinline__ int prep_row_chunks(int seg) { return seg / 16 + 2; }
+/* 16-byte chunks staged per full-resolution row for a segment of seg bytes
+ * (one spare chunk keeps the box sums' whole-dword reads inside the row) */
+__host__ __device__ __forceinline__ int prep_row_chunks(int seg) { return seg / 16 + 3; }
 
 static size_t prep_lds_bytes(int F, int ch, int cols, int prb) {
     return (size_t)2 * prb * F * prep_row_chunks(cols * F * ch) * 16;
@@ -212,6 +213,59 @@ static int prep_cols(int F, int ch, int prb) {
     return c;
 }
 
+/* Box sums of the F x F box (F = FT known at compile time, window not
+ * clipped horizontally) over the nk staged rows starting at row index k0r of
+ * image block rbase, from byte xoff of each row.  Each row's F*CH bytes are
+ * read as whole dwords and realigned with v_alignbyte; bytes are added in
+ * 16-bit lanes (even bytes, odd bytes), which hold up to 257 rows of 255.
+ * CH = 3: byte p of the window is channel p % 3; CH = 4: p % 4 (alpha
+ * skipped); CH = 1: all bytes are gray. */
+template <int FT, int CH>
+__device__ __forceinline__ void prep_box_swar(const uint8_t* __restrict__ stage, const int* rowoff, int rbase,
+                                              int k0r, int nk, int xoff, int& sr, int& sg, int& sb) {
+    constexpr int L = FT * CH;
+    constexpr int DW = (L + 3) / 4;
+    unsigned e[DW], o[DW];
+#pragma unroll
+    for (int j = 0; j < DW; ++j) e[j] = o[j] = 0u;
+#pragma unroll
+    for (int kk = 0; kk < FT; ++kk) {
+        if (kk < nk) {
+            const int sb0 = rowoff[rbase + k0r + kk] + xoff;
+            const unsigned* b = (const unsigned*)(stage + (sb0 & ~3));
+            const unsigned sh = (unsigned)(sb0 & 3);
+            unsigned d[DW + 1];
+#pragma unroll
+            for (int j = 0; j < DW; ++j) d[j] = b[j];
+            d[DW] = ((int)sh + L > 4 * DW) ? b[DW] : 0u;
+#pragma unroll
+            for (int j = 0; j < DW; ++j) {
+                unsigned w = __builtin_amdgcn_alignbyte(d[j + 1], d[j], sh);
+                if (j == DW - 1 && (L % 4) != 0) w &= (1u << (8 * (L % 4))) - 1u;
+                e[j] += w & 0x00FF00FFu;
+                o[j] += (w >> 8) & 0x00FF00FFu;
+            }
+        }
+    }
+    int acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < DW; ++j) {
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int p = 4 * j + bb;
+            if (p < L) {
+                const unsigned v = (bb == 0) ? (e[j] & 0xFFFFu) : (bb == 1) ? (o[j] & 0xFFFFu)
+                                 : (bb == 2) ? (e[j] >> 16) : (o[j] >> 16);
+                acc[CH == 1 ? 0 : p % CH] += (int)v;
+            }
+        }
+    }
+    sr = acc[0];
+    sg = CH == 1 ? acc[0] : acc[1];
+    sb = CH == 1 ? acc[0] : acc[2];
+}
+
+template <int FT>
 __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
                                                long pair_stride, int w, int h, int ch, int F, int wd, int hd,
                                                int cols, int prb, float* __restrict__ yiq,
@@ -279,21 +333,32 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
         const int k0 = max(ys, rr0), k1 = min(ys + F, rr1);
         const int nb = (j1 - j0) * ch;
         const long idx = (long)y * wd + x;
+        const bool full = FT > 0 && j0 == xs && j1 == xs + F;
         for (int im = 0; im < 2; ++im) {
             int sr = 0, sg = 0, sb = 0;
-            for (int k = k0; k < k1; ++k) {
-                const uint8_t* p = stage + rowoff[im * nrows + (k - rr0)] + (j0 - cc0) * ch;
-                if (ch == 1) {
-                    for (int b = 0; b < nb; ++b) sr += p[b];
-                } else {
-                    for (int b = 0; b < nb; b += ch) {
-                        sr += p[b];
-                        sg += p[b + 1];
-                        sb += p[b + 2];
-                    }
+            if constexpr (FT > 0) {
+                if (full) {
+                    const int xoff = (j0 - cc0) * ch;
+                    if (ch == 3) prep_box_swar<FT, 3>(stage, rowoff, im * nrows, k0 - rr0, k1 - k0, xoff, sr, sg, sb);
+                    else if (ch == 4) prep_box_swar<FT, 4>(stage, rowoff, im * nrows, k0 - rr0, k1 - k0, xoff, sr, sg, sb);
+                    else prep_box_swar<FT, 1>(stage, rowoff, im * nrows, k0 - rr0, k1 - k0, xoff, sr, sg, sb);
                 }
             }
-            if (ch == 1) sg = sb = sr;
+            if (!full) {
+                for (int k = k0; k < k1; ++k) {
+                    const uint8_t* p = stage + rowoff[im * nrows + (k - rr0)] + (j0 - cc0) * ch;
+                    if (ch == 1) {
+                        for (int b = 0; b < nb; ++b) sr += p[b];
+                    } else {
+                        for (int b = 0; b < nb; b += ch) {
+                            sr += p[b];
+                            sg += p[b + 1];
+                            sb += p[b + 2];
+                        }
+                    }
+                }
+                if (ch == 1) sg = sb = sr;
+            }
             const float R = (float)sr, G = (float)sg, B = (float)sb;
             base[(im ? 1 : 0) * N + idx] = (R * 0.299f + G * 0.587f + B * 0.114f) * inv;
             base[(im ? 4 : 2) * N + idx] = (R * 0.596f - G * 0.274f - B * 0.322f) * inv;
@@ -302,14 +367,16 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
     }
 }
 
+/* FT = F for the common factors 1..8 (compile-time box width), 0 otherwise */
+template <int FT>
 __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
                                                            const uint8_t* __restrict__ ref,
                                                            long pair_stride, int w, int h,
                                                            int ch, int F, int wd, int hd, int cols, int prb,
                                                            float* __restrict__ yiq,
                                                            unsigned* __restrict__ hist) {
-    fsim_prep_body(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist, blockIdx.x, blockIdx.y,
-                   blockIdx.z);
+    fsim_prep_body<FT>(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist, blockIdx.x, blockIdx.y,
+                       blockIdx.z);
 }
 
 /* Row half of F5 on the packed rows Y_in + i*Y_ref of `rb` rows; the two
@@ -410,11 +477,20 @@ __global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* _
 
 /* Row half of F11 (normalised by 1/N) fused with F12
  * (fsim_sum_filter_responses.glsl:33-83) and the F13 packing
- * (fsim_pack_for_median.glsl:33-44) for `rb` rows of one image and
- * orientation; T planes img*16 + 4*o + s. */
+ * (fsim_pack_for_median.glsl:33-44) for `rb` rows of one image, one
+ * orientation after the other; T planes img*16 + 4*o + s.  Writes per image
+ * the plane sum_o An (summed in the order o = 0..3, as the PC step
+ * (fsim_phase_congruency.glsl:55-60) adds them) and E_o, plus the keys and
+ * their level-1 histogram per orientation. */
+__host__ __device__ __forceinline__ size_t rowpass_lds_bytes(const LinePlan& P, int rb, int wd) {
+    const size_t bufbytes = (size_t)4 * rb * lds_pitch(P.lds_len) * sizeof(float2);
+    const size_t hist = bufbytes >= kHistBins * sizeof(unsigned) ? 0 : kHistBins * sizeof(unsigned);
+    return 2 * bufbytes + (size_t)lds_twiddles(P) * sizeof(float2) + hist + sizeof(float) * (size_t)rb * wd;
+}
+
 __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin, float* __restrict__ an_e,
                                                   float* __restrict__ keys, unsigned* __restrict__ hist, int wd,
-                                                  int hd, int rb, const LinePlan& P, float invN, int bx, int io,
+                                                  int hd, int rb, const LinePlan& P, float invN, int bx, int img,
                                                   int pair) {
     extern __shared__ float2 lds[];
     const int ld = lds_pitch(P.lds_len);
@@ -423,56 +499,96 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
     float2* b1 = lds + nl * ld;
     const int ntw = lds_twiddles(P);
     const float2* tw = stage_twiddles(P, lds + 2 * nl * ld);
+    const size_t bufbytes = (size_t)nl * ld * sizeof(float2);
+    const bool hist_aliased = bufbytes >= kHistBins * sizeof(unsigned);
+    unsigned* hsep = (unsigned*)(lds + 2 * nl * ld + ntw);
+    float* sa = (float*)(hsep + (hist_aliased ? 0 : kHistBins)); /* rb * wd running sums of An */
     const long N = (long)wd * hd;
     const int y0 = bx * rb;
     const int nr = min(rb, hd - y0);
-    for (int idx = threadIdx.x; idx < nl * wd; idx += blockDim.x) {
-        const int l = idx / wd, x = idx - l * wd;
-        const int s = l / rb, r = l - s * rb;
-        float2 v = make_float2(0.f, 0.f);
-        if (r < nr) v = Tin[((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * wd + x];
-        b0[l * ld + x] = v;
-    }
-    __syncthreads();
-    float2* R = line_fft<1>(b0, b1, ld, nl, P, tw);
-    const size_t bufbytes = (size_t)nl * ld * sizeof(float2);
-    unsigned* hl = (bufbytes >= kHistBins * sizeof(unsigned))
-                       ? (unsigned*)((R == b0) ? b1 : b0)
-                       : (unsigned*)(lds + 2 * nl * ld + ntw);
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
-    __syncthreads();
-    float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
-    float* enp = anp + N;
-    float* kp = keys + ((long)pair * 8 + io) * N;
-    for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
-        const int r = idx / wd, x = idx - r * wd;
-        float2 eo[4];
-        float amp = 0.f, sr = 0.f, si = 0.f;
+    float* ap = an_e + ((long)pair * 2 + img) * kAnePlanes * N;
+    /* when the 4 * rb lines fit kRowPrefetch elements per thread, the next
+     * orientation's lines are loaded into registers while the current one is
+     * transformed (the transform's barriers order LDS only) */
+    constexpr int kRowPrefetch = 8;
+    const int tot = nl * wd;
+    const bool pf = tot <= kRowPrefetch * (int)blockDim.x;
+    float2 pre[kRowPrefetch];
+    auto fetch = [&](int o) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            eo[s] = cscale(R[(s * rb + r) * ld + x], invN);
-            amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
-            sr += eo[s].x;
-            si += eo[s].y;
+        for (int q = 0; q < kRowPrefetch; ++q) {
+            const int idx = threadIdx.x + q * blockDim.x;
+            pre[q] = make_float2(0.f, 0.f);
+            if (idx < tot) {
+                const int l = idx / wd, x = idx - l * wd;
+                const int s = l / rb, r = l - s * rb;
+                if (r < nr) pre[q] = Tin[((long)pair * 32 + (img * 4 + o) * 4 + s) * N + (long)(y0 + r) * wd + x];
+            }
         }
-        const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
-        const float me = sr / xe, mo = si / xe;
-        float energy = 0.f;
+    };
+    if (pf) fetch(0);
+    for (int o = 0; o < 4; ++o) {
+        const int io = img * 4 + o;
+        if (pf) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
-        const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
-        const long g = (long)(y0 + r) * wd + x;
-        anp[g] = amp;
-        enp[g] = energy;
-        kp[g] = key;
-        atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
-    }
-    __syncthreads();
-    unsigned* hg = hist + ((long)pair * 8 + io) * kHistBins;
-    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) {
-        const unsigned c = hl[i];
-        if (c) atomicAdd(&hg[i], c);
+            for (int q = 0; q < kRowPrefetch; ++q) {
+                const int idx = threadIdx.x + q * blockDim.x;
+                if (idx < tot) {
+                    const int l = idx / wd, x = idx - l * wd;
+                    b0[l * ld + x] = pre[q];
+                }
+            }
+            if (o < 3) fetch(o + 1);
+        } else {
+            for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+                const int l = idx / wd, x = idx - l * wd;
+                const int s = l / rb, r = l - s * rb;
+                float2 v = make_float2(0.f, 0.f);
+                if (r < nr) v = Tin[((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * wd + x];
+                b0[l * ld + x] = v;
+            }
+        }
+        lds_barrier(); /* also orders the twiddles staged before the loop */
+        float2* R = line_fft<1>(b0, b1, ld, nl, P, tw);
+        unsigned* hl = hist_aliased ? (unsigned*)((R == b0) ? b1 : b0) : hsep;
+        for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
+        lds_barrier();
+        float* enp = ap + (long)(1 + o) * N;
+        float* kp = keys + ((long)pair * 8 + io) * N;
+        for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
+            const int r = idx / wd, x = idx - r * wd;
+            float2 eo[4];
+            float amp = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                eo[s] = cscale(R[(s * rb + r) * ld + x], invN);
+                amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
+                sr += eo[s].x;
+                si += eo[s].y;
+            }
+            const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
+            const float me = sr / xe, mo = si / xe;
+            float energy = 0.f;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+            const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+            const long g = (long)(y0 + r) * wd + x;
+            /* sum over o, in the order o = 0..3 (each idx stays with one thread) */
+            if (o == 0) sa[idx] = amp;
+            else if (o < 3) sa[idx] += amp;
+            else ap[g] = sa[idx] + amp;
+            enp[g] = energy;
+            kp[g] = key;
+            atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
+        }
+        lds_barrier(); /* histogram complete; line buffers free for the next orientation */
+        unsigned* hg = hist + ((long)pair * 8 + io) * kHistBins;
+        for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) {
+            const unsigned c = hl[i];
+            if (c) atomicAdd(&hg[i], c);
+        }
+        lds_barrier(); /* hl may alias b0/b1, which the next orientation loads into */
     }
 }
 
@@ -536,42 +652,81 @@ using NoiseArgs = FsimNoiseArgs;
  * (fsim_noise_power.cpp:198-199) by a 3-level radix select on the float bit
  * patterns (keys are >= 0, so the bits order like the values and bit 31 is
  * clear): level 1 uses the histogram built by fsim_rowpass (bits 30..21),
- * level 2 bits 20..10, level 3 bits 9..0. */
+ * level 2 bits 20..10, level 3 bits 9..0.  The level-2 pass also copies the
+ * keys of the level-1 bucket (about 5-8% of them on natural images) into
+ * LDS, so level 3 reads them from there; when more than kSelectCap keys fall
+ * in that bucket, level 3 reads the keys from memory again. */
+constexpr int kSelectCap = 16384; /* keys of the level-1 bucket kept in LDS */
+
 __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys, const unsigned* __restrict__ hist,
                                                  float* __restrict__ thr, double* __restrict__ stats, long N,
                                                  const NoiseArgs& na, int io, int pair) {
+    extern __shared__ unsigned cand[]; /* kSelectCap keys of bucket b1 */
     __shared__ unsigned h[2048];
     __shared__ unsigned wsum[16];
     __shared__ int bucket;
     __shared__ unsigned rest;
+    __shared__ unsigned ncand;
     const long slot = (long)pair * 8 + io;
     const unsigned* kb = (const unsigned*)(keys + slot * N);
     const unsigned* hg = hist + slot * kHistBins;
     const unsigned k = (unsigned)(N / 2);
+    const int lane = threadIdx.x & 63;
 
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) h[i] = hg[i];
-    if (threadIdx.x == 0) { bucket = 0; rest = 0; }
+    if (threadIdx.x == 0) { bucket = 0; rest = 0; ncand = 0u; }
     __syncthreads();
     rank_bucket(h, kHistBins, k, &bucket, &rest, wsum);
     const unsigned b1 = (unsigned)bucket, k1 = rest;
 
     for (int i = threadIdx.x; i < 2048; i += blockDim.x) h[i] = 0u;
     __syncthreads();
+    /* level 2 + compaction: every lane runs the same number of iterations so
+     * the wave-wide ballot sees all lanes */
+    auto level2 = [&](unsigned u, bool valid) {
+        const bool m = valid && ((u >> 21) & 1023u) == b1;
+        if (m) atomicAdd(&h[(u >> 10) & 2047u], 1u);
+        const unsigned long long mask = __ballot(m);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&ncand, (unsigned)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (m) {
+                const unsigned at = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+                if (at < (unsigned)kSelectCap) cand[at] = u;
+            }
+        }
+    };
     /* keys are read as uint4 when every plane starts 16-byte aligned */
     const bool vec = (N % 4) == 0;
+    const long stride = blockDim.x;
     if (vec) {
+        /* 4 loads in flight per thread */
         const uint4* k4 = (const uint4*)kb;
-        for (long i = threadIdx.x; i < N / 4; i += blockDim.x) {
-            const uint4 q = k4[i];
-            const unsigned u[4] = {q.x, q.y, q.z, q.w};
+        const long n4 = N / 4;
+        for (long i0 = 0; i0 < n4; i0 += 4 * stride) {
+            uint4 q[4];
+            bool ok[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (((u[j] >> 21) & 1023u) == b1) atomicAdd(&h[(u[j] >> 10) & 2047u], 1u);
+            for (int u = 0; u < 4; ++u) {
+                const long i = i0 + u * stride + threadIdx.x;
+                ok[u] = i < n4;
+                q[u] = ok[u] ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                level2(q[u].x, ok[u]);
+                level2(q[u].y, ok[u]);
+                level2(q[u].z, ok[u]);
+                level2(q[u].w, ok[u]);
+            }
         }
     } else {
-        for (long i = threadIdx.x; i < N; i += blockDim.x) {
-            const unsigned u = kb[i];
-            if (((u >> 21) & 1023u) == b1) atomicAdd(&h[(u >> 10) & 2047u], 1u);
+        for (long i0 = 0; i0 < N; i0 += stride) {
+            const long i = i0 + threadIdx.x;
+            const bool ok = i < N;
+            level2(ok ? kb[i] : 0u, ok);
         }
     }
     __syncthreads();
@@ -581,7 +736,13 @@ __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys,
     for (int i = threadIdx.x; i < 1024; i += blockDim.x) h[i] = 0u;
     __syncthreads();
     const unsigned pre = (b1 << 11) | b2;
-    if (vec) {
+    const unsigned nc = ncand;
+    if (nc <= (unsigned)kSelectCap) {
+        for (unsigned i = threadIdx.x; i < nc; i += blockDim.x) {
+            const unsigned u = cand[i];
+            if (((u >> 10) & 0x1FFFFFu) == pre) atomicAdd(&h[u & 1023u], 1u);
+        }
+    } else if (vec) {
         const uint4* k4 = (const uint4*)kb;
         for (long i = threadIdx.x; i < N / 4; i += blockDim.x) {
             const uint4 q = k4[i];
@@ -686,17 +847,12 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
         float pc[2];
         for (int im = 0; im < 2 && pc_src; ++im) pc[im] = pc_src[((long)pair * 2 + im) * N + idx];
         for (int im = 0; im < 2 && !pc_src; ++im) {
-            const float* ae = an_e + (long)pair * 16 * N;
-            const float* T = thr + (long)pair * 8;
-            float et = 0.f, at = 0.f;
-            for (int o = 0; o < 4; ++o) {
-                const int io = im * 4 + o;
-                const float amp = ae[(2 * io) * N + idx];
-                const float en = ae[(2 * io + 1) * N + idx];
-                et += fmaxf(en - T[io], 0.0f);
-                at += amp;
-            }
-            pc[im] = et / at; /* no epsilon: NaN where sum An = 0 */
+            /* plane 0: sum_o An (added in the order o = 0..3 by the row pass) */
+            const float* ae = an_e + ((long)pair * 2 + im) * kAnePlanes * N;
+            const float* T = thr + (long)pair * 8 + im * 4;
+            float et = 0.f;
+            for (int o = 0; o < 4; ++o) et += fmaxf(ae[(1 + o) * N + idx] - T[o], 0.0f);
+            pc[im] = et / ae[idx]; /* no epsilon: NaN where sum An = 0 */
         }
         const float T1 = 0.85f, T2 = 160.0f, T3 = 200.0f, T4 = 200.0f, lambda = 0.03f;
         const float p1 = pc[0], p2 = pc[1];
@@ -806,7 +962,7 @@ FsimRegions fsim_regions(long N, int n, int chunk, int nblk) {
     r.grad = take(sizeof(float) * 2 * N * n);
     r.A = take(sizeof(float2) * 2 * N * n);
     r.T = take(sizeof(float2) * 32 * N * chunk);
-    r.an_e = take(sizeof(float) * 16 * N * n);
+    r.an_e = take(sizeof(float) * 2 * kAnePlanes * N * n);
     r.keys = take(sizeof(float) * 8 * N * n);
     r.hist = take(sizeof(unsigned) * 8 * kHistBins * n);
     r.thr = take(sizeof(float) * 8 * n);
@@ -839,7 +995,19 @@ FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long 
     };
     t.grad_rb = fit(Px, 2, env_int("IQM_FSIM_GRB", tune.fft_rb), 1);
     t.col_bx = fit(Py, 3, env_int("IQM_FSIM_BX", tune.col_bx), 1);
-    t.row_rb = fit(Px, 2, env_int("IQM_FSIM_RB", tune.row_rb), 4); /* 4 scales per row */
+    /* row pass: 4 scales per row, plus the histogram and the An sums */
+    auto fit_row = [&](int cap) {
+        auto best = [&](size_t budget) {
+            int b = 0;
+            for (int r = 1; r <= cap; ++r)
+                if (rowpass_lds_bytes(Px, r, Px.n) <= budget) b = r;
+            return b;
+        };
+        const int l = best(std::min<size_t>(c.lds_per_block(), 65536));
+        if (l >= std::min(cap, 4)) return l;
+        return std::max(l, best(c.lds_per_block()));
+    };
+    t.row_rb = fit_row(env_int("IQM_FSIM_RB", tune.row_rb));
     /* pairs per column/row-pass launch: one launch over the whole batch
      * measured fastest on MI355X (larger grids beat cache residency of the
      * response planes); the cap bounds the response-plane workspace */
@@ -872,8 +1040,20 @@ void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stri
     const int cols = prep_cols(F, ch, prb);
     const size_t bytes = prep_lds_bytes(F, ch, cols, prb);
     IQM_CHECK(bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-    launch("fsim_prep", fsim_prep_kernel, dim3(ceil_div(wd, cols), ceil_div(hd, prb), n), dim3(kBlock), bytes,
-           d_img, d_ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist);
+    const dim3 grid(ceil_div(wd, cols), ceil_div(hd, prb), n);
+    switch (F) {
+#define IQM_PREP_CASE(f)                                                                                      \
+    case f:                                                                                                   \
+        launch("fsim_prep", fsim_prep_kernel<f>, grid, dim3(kBlock), bytes, d_img, d_ref, pair_stride, w, h, ch, \
+               F, wd, hd, cols, prb, yiq, hist);                                                              \
+        break;
+        IQM_PREP_CASE(1) IQM_PREP_CASE(2) IQM_PREP_CASE(3) IQM_PREP_CASE(4)
+        IQM_PREP_CASE(5) IQM_PREP_CASE(6) IQM_PREP_CASE(7) IQM_PREP_CASE(8)
+#undef IQM_PREP_CASE
+        default:
+            launch("fsim_prep", fsim_prep_kernel<0>, grid, dim3(kBlock), bytes, d_img, d_ref, pair_stride, w, h, ch,
+                   F, wd, hd, cols, prb, yiq, hist);
+    }
 }
 
 double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
@@ -923,9 +1103,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const int nbx = ceil_div(wd, t.col_bx);
     const int nbx_pad = ceil_div(nbx, xcds_) * xcds_;
     const float invN = (float)(1.0 / (double)N);
-    size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);
-    if ((size_t)4 * t.row_rb * lds_pitch(Px.lds_len) * sizeof(float2) < kHistBins * sizeof(unsigned))
-        row_sh += kHistBins * sizeof(unsigned);
+    const size_t row_sh = rowpass_lds_bytes(Px, t.row_rb, wd);
     const bool tl_col = tl && tl_supported(hd);
     const bool tl_row = tl && tl_supported(wd);
     for (int p0 = 0; p0 < n; p0 += chunk) {
@@ -940,13 +1118,13 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
                    t.col_bx, nbx_pad, Py);
         }
         if (tl_row) {
-            TlRowArgs a{T, an_e + (long)p0 * 16 * N, keys + (long)p0 * 8 * N,
+            TlRowArgs a{T, an_e + (long)p0 * 2 * kAnePlanes * N, keys + (long)p0 * 8 * N,
                         hist + (long)p0 * 8 * kHistBins, Px.tw, hd, np, env_int("IQM_FSIM_TL_RB", tune.row_rb),
                         invN, tl_budget};
             tl_rowpass(*this, wd, a);
         } else {
-            launch("fsim_rowpass", fsim_rowpass_kernel, dim3(ceil_div(hd, t.row_rb), 8, np),
-                   dim3(kBlockFft), row_sh, (const float2*)T, an_e + (long)p0 * 16 * N,
+            launch("fsim_rowpass", fsim_rowpass_kernel, dim3(ceil_div(hd, t.row_rb), 2, np),
+                   dim3(kBlockFft), row_sh, (const float2*)T, an_e + (long)p0 * 2 * kAnePlanes * N,
                    keys + (long)p0 * 8 * N, hist + (long)p0 * 8 * kHistBins, wd, hd, t.row_rb, Px,
                    invN);
         }
@@ -955,7 +1133,8 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     NoiseArgs na;
     for (int i = 0; i < 4; ++i) na.em_n[i] = plan.em_n[i];
     for (int i = 0; i < 8; ++i) na.est[i] = plan.est[i];
-    launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), 0, (const float*)keys,
+    launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), sizeof(unsigned) * kSelectCap,
+           (const float*)keys,
            (const unsigned*)hist, thr, stats, N, na);
     /* F15 + F16 + F17 */
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
@@ -1013,8 +1192,19 @@ __global__ __launch_bounds__(kBlock) void mix_prep_kernel(const MixJob* __restri
     const MixJob& J = jobs[j];
     const int local = blockIdx.x - st[j];
     const int gx = (J.wd + J.cols - 1) / J.cols;
-    fsim_prep_body(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx,
-                   local / gx, 0);
+    switch (J.F) {
+#define IQM_MIXPREP_CASE(f)                                                                                   \
+    case f:                                                                                                   \
+        fsim_prep_body<f>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx, \
+                          local / gx, 0);                                                                     \
+        break;
+        IQM_MIXPREP_CASE(1) IQM_MIXPREP_CASE(2) IQM_MIXPREP_CASE(3) IQM_MIXPREP_CASE(4)
+        IQM_MIXPREP_CASE(5) IQM_MIXPREP_CASE(6) IQM_MIXPREP_CASE(7) IQM_MIXPREP_CASE(8)
+#undef IQM_MIXPREP_CASE
+        default:
+            fsim_prep_body<0>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx,
+                              local / gx, 0);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void mix_rowfft_kernel(const MixJob* __restrict__ jobs,
@@ -1122,11 +1312,8 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     add(MIX_PREP, ceil_div(wd, mj.cols) * hd, prep_lds_bytes(F, pj.ch, mj.cols, 1));
     add(MIX_ROWFFT, ceil_div(hd, t.grad_rb), line_lds_bytes(Px, t.grad_rb, 2));
     add(MIX_COL, mj.nbx * 2, line_lds_bytes(Py, t.col_bx, 3));
-    size_t row_sh = line_lds_bytes(Px, 4 * t.row_rb, 2);
-    if ((size_t)4 * t.row_rb * lds_pitch(Px.lds_len) * sizeof(float2) < kHistBins * sizeof(unsigned))
-        row_sh += kHistBins * sizeof(unsigned);
-    add(MIX_ROW, ceil_div(hd, t.row_rb) * 8, row_sh);
-    add(MIX_SELECT, 8, 0);
+    add(MIX_ROW, ceil_div(hd, t.row_rb) * 2, rowpass_lds_bytes(Px, t.row_rb, wd));
+    add(MIX_SELECT, 8, sizeof(unsigned) * kSelectCap);
     add(MIX_PCF, mj.pc_gx * mj.pc_gy, 0);
     add(MIX_FIN, 1, 0);
 }

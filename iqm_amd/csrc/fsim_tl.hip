@@ -95,9 +95,20 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
     }
 }
 
-/* Row pass: inverse row FFT of the 4 scales of `rb` rows of one image and
- * orientation, normalised by 1/N, then the response epilogue, the median
- * keys and their top-bit histogram (as fsim_rowpass_kernel). */
+/* Row pass: for each orientation o = 0..3 in turn, the inverse row FFT of
+ * the 4 scales of `rb` rows of one image, normalised by 1/N, then the
+ * response epilogue, the median keys and their top-bit histogram (as
+ * fsim_rowpass_kernel).  Each thread owns one step-1 item (the launch keeps
+ * 4 * rb * max(N1, N2) <= blockDim), whose inputs for the next orientation are loaded
+ * into registers before the current one's step 2 and epilogue, so the loads
+ * overlap that work (the barriers in the loop order LDS only).  The
+ * amplitudes are summed over o in registers, in the order o = 0..3, and
+ * written once as the plane sum_o An. */
+/* epilogue pixels per thread: the launch keeps blockDim >= 4 * rb * max(N1, N2),
+ * so rb * N1 * N2 <= ceil(min(N1, N2) / 4) * blockDim */
+template <int N1, int N2>
+constexpr int row_slots() { return ((N1 < N2 ? N1 : N2) + 3) / 4; }
+
 template <int N1, int N2>
 __global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
                                                                float* __restrict__ an_e,
@@ -114,82 +125,100 @@ __global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __re
     float2* E = lds; /* aliases Z, see step 2 */
     float2* tw = Z + nl * ZL;
     unsigned* hl = (unsigned*)(tw + n);
-    const int io = blockIdx.y, pair = blockIdx.z;
+    const int img = blockIdx.y, pair = blockIdx.z;
     const long N = (long)n * hd;
     const int y0 = blockIdx.x * rb;
     const int nr = min(rb, hd - y0);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
-    for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) hl[i] = 0u;
-    __syncthreads();
-    /* step 1, items (line, j2), j2 fastest; line = s * rb + r */
-    for (int it = threadIdx.x; it < nl * N2; it += blockDim.x) {
-        const int l = it / N2, j2 = it - l * N2;
-        const int s = l / rb, r = l - s * rb;
-        float2 x[N1];
-        if (r < nr) {
-            const float2* src = Tin + ((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * n;
+    /* this thread's step-1 item: line l1 = s1 * rb + r1, column j21 */
+    const int it1 = threadIdx.x;
+    const bool has1 = it1 < nl * N2;
+    const int l1 = it1 / N2, j21 = it1 - l1 * N2;
+    const int s1 = l1 / rb, r1 = l1 - s1 * rb;
+    const bool ld1 = has1 && r1 < nr;
+    float2 x[N1];
+    auto fetch = [&](int o) {
+        if (ld1) {
+            const float2* src = Tin + ((long)pair * 32 + (img * 4 + o) * 4 + s1) * N + (long)(y0 + r1) * n;
 #pragma unroll
-            for (int j1 = 0; j1 < N1; ++j1) x[j1] = nt_load(&src[N2 * j1 + j2]);
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = nt_load(&src[N2 * j1 + j21]);
         } else {
 #pragma unroll
             for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
         }
-        two_level_step1<N1, N2, 1>(x, j2, tw, Z + l * ZL);
-    }
+    };
+    fetch(0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) hl[i] = 0u;
     lds_barrier();
-    /* step 2 in rounds of blockDim items: every item of a round reads its
-     * exchange row into registers before any item writes outputs over the
-     * same LDS (E aliases Z) */
-    const int items2 = nl * N1;
-    for (int base = 0; base < items2; base += blockDim.x) {
-        const int it = base + threadIdx.x;
-        float2 y[N2];
-        const bool act = it < items2;
-        const int l = act ? it / N1 : 0, k1 = act ? it - l * N1 : 0;
-        if (act) {
-#pragma unroll
-            for (int j2 = 0; j2 < N2; ++j2) y[j2] = Z[l * ZL + k1 * Exch<N1, N2>::pitch + j2];
-        }
+    float* ap = an_e + ((long)pair * 2 + img) * kAnePlanes * N;
+    constexpr int kSlots = row_slots<N1, N2>();
+    float sa[kSlots];
+    for (int o = 0; o < 4; ++o) {
+        const int io = img * 4 + o;
+        if (has1) two_level_step1<N1, N2, 1>(x, j21, tw, Z + l1 * ZL);
+        if (o < 3) fetch(o + 1);
         lds_barrier();
-        if (act) {
-            RegDft<N2, 1>::run(y);
+        /* step 2 in rounds of blockDim items: every item of a round reads its
+         * exchange row into registers before any item writes outputs over the
+         * same LDS (E aliases Z) */
+        const int items2 = nl * N1;
+        for (int base = 0; base < items2; base += blockDim.x) {
+            const int it = base + threadIdx.x;
+            float2 y[N2];
+            const bool act = it < items2;
+            const int l = act ? it / N1 : 0, k1 = act ? it - l * N1 : 0;
+            if (act) {
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) E[l * LE + k1 + N1 * k2] = y[k2];
+                for (int j2 = 0; j2 < N2; ++j2) y[j2] = Z[l * ZL + k1 * Exch<N1, N2>::pitch + j2];
+            }
+            lds_barrier();
+            if (act) {
+                RegDft<N2, 1>::run(y);
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) E[l * LE + k1 + N1 * k2] = y[k2];
+            }
+            lds_barrier();
         }
-        lds_barrier();
-    }
-    float* anp = an_e + ((long)pair * 8 + io) * 2 * N;
-    float* enp = anp + N;
-    float* kp = keys + ((long)pair * 8 + io) * N;
-    for (int idx = threadIdx.x; idx < nr * n; idx += blockDim.x) {
-        const int r = idx / n, x = idx - r * n;
-        float2 eo[4];
-        float amp = 0.f, sr = 0.f, si = 0.f;
+        float* enp = ap + (long)(1 + o) * N;
+        float* kp = keys + ((long)pair * 8 + io) * N;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            eo[s] = cscale(E[(s * rb + r) * LE + x], invN);
-            amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
-            sr += eo[s].x;
-            si += eo[s].y;
+        for (int q = 0; q < kSlots; ++q) {
+            const int idx = threadIdx.x + q * blockDim.x;
+            if (idx < nr * n) {
+                const int r = idx / n, xx = idx - r * n;
+                float2 eo[4];
+                float amp = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    eo[s] = cscale(E[(s * rb + r) * LE + xx], invN);
+                    amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
+                    sr += eo[s].x;
+                    si += eo[s].y;
+                }
+                const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
+                const float me = sr / xe, mo = si / xe;
+                float energy = 0.f;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+                const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+                const long g = (long)(y0 + r) * n + xx;
+                /* sum over o, in the order o = 0..3 */
+                if (o == 0) sa[q] = amp;
+                else if (o < 3) sa[q] += amp;
+                else ap[g] = sa[q] + amp;
+                enp[g] = energy;
+                kp[g] = key;
+                atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
+            }
         }
-        const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
-        const float me = sr / xe, mo = si / xe;
-        float energy = 0.f;
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-            energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
-        const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
-        const long g = (long)(y0 + r) * n + x;
-        anp[g] = amp;
-        enp[g] = energy;
-        kp[g] = key;
-        atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
-    }
-    __syncthreads();
-    unsigned* hg = hist + ((long)pair * 8 + io) * kHistBinsTl;
-    for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) {
-        const unsigned c = hl[i];
-        if (c) atomicAdd(&hg[i], c);
+        lds_barrier(); /* histogram complete; E read (the next step 1 overwrites it) */
+        unsigned* hg = hist + ((long)pair * 8 + io) * kHistBinsTl;
+        for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) {
+            const unsigned c = hl[i];
+            if (c) atomicAdd(&hg[i], c);
+            hl[i] = 0u; /* the next orientation's atomics follow several barriers */
+        }
     }
 }
 
@@ -291,9 +320,14 @@ void col_launch(Ctx& c, const TlColArgs& a) {
 template <int N1, int N2>
 void row_launch(Ctx& c, const TlRowArgs& a) {
     int rb = a.rb;
-    while (rb > 1 && row_lds<N1, N2>(rb) > a.lds_budget) --rb;
-    const int thr = threads_for(4 * rb * (N1 > N2 ? N1 : N2));
-    c.launch("fsim_rowpass", fsim_rowpass_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), 8, a.npairs),
+    /* one step-1 item per thread and at most row_slots() epilogue pixels per thread */
+    constexpr int mx = N1 > N2 ? N1 : N2;
+    while (rb > 1 && (row_lds<N1, N2>(rb) > a.lds_budget || 4 * rb * mx > 512)) --rb;
+    const int thr = threads_for(4 * rb * mx);
+    constexpr int slots = row_slots<N1, N2>();
+    IQM_CHECK(4 * rb * N2 <= thr && rb * N1 * N2 <= slots * thr, IQM_ERR_UNSUPPORTED,
+              "row pass tile does not fit its thread mapping");
+    c.launch("fsim_rowpass", fsim_rowpass_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), 2, a.npairs),
              dim3(thr), row_lds<N1, N2>(rb), a.T, a.an_e, a.keys, a.hist, a.tw, a.hd, rb, a.invN);
 }
 

@@ -109,9 +109,15 @@ def test_fsim_stages_small(ctx):
     for n in ("yiq_in", "yiq_ref", "grad_in", "grad_ref"):
         scale = max(1.0, float(np.max(np.abs(o.maps[n]))))
         assert _maxabs(d[n], o.maps[n]) / scale <= 1e-5, n
+    # the oracle keeps (An, E) per [img*4 + o]; the library stores per image
+    # sum_o An, then E_0..E_3
+    ae = o.maps["an_e"].reshape(2, 4, 2, *o.maps["an_e"].shape[1:])
+    expect = {"an_e": np.concatenate([np.concatenate([ae[i, :, 0].sum(axis=0)[None], ae[i, :, 1]])
+                                      for i in range(2)]),
+              "keys": o.maps["keys"]}
     for n in ("an_e", "keys"):
-        scale = float(np.max(np.abs(o.maps[n])))
-        err = _maxabs(d[n], o.maps[n]) / scale
+        scale = float(np.max(np.abs(expect[n])))
+        err = _maxabs(d[n], expect[n]) / scale
         _report(f"stage {n}", rel=err)
         assert err <= 1e-4, n
 
