@@ -213,6 +213,264 @@ __global__ __launch_bounds__(kBlock) void ssim_kernel(const uint8_t* __restrict_
                          gridDim.x, gridDim.y, blockIdx.z);
 }
 
+/* ---- row-walker SSIM (window radius R <= kRwMaxRadius) -------------------
+ *
+ * A workgroup of kRwThreads threads produces a kRwTX x kRwTY tile; thread t
+ * owns output columns 2t and 2t+1.  It walks down the kRwTY + 2R window rows
+ * once:
+ *   1. the bytes of both images' window rows arrive in LDS in chunks of
+ *      K = 2R+1 rows, loaded straight into LDS (global_load_lds_dwordx4) one
+ *      chunk ahead of their use;
+ *   2. per window row, the shifted lumas a = x - sx and b = y - sy of the
+ *      row's kRwTX + 2R columns give u = a + b, v = a - b, u^2 and v^2 in a
+ *      double-buffered LDS row;
+ *   3. each thread forms the four horizontal weighted sums of its two
+ *      columns from that row and keeps them in a ring of the last K rows held
+ *      in registers (slot = row mod K, so every ring index is a compile-time
+ *      constant);
+ *   4. once K rows are in the ring, the vertical sums give one output row.
+ * SSIM needs only sigma_x^2 + sigma_y^2 and sigma_xy, and with
+ * Var(u) = E[u^2] - E[u]^2 (likewise v) these are (Var u + Var v) / 2 and
+ * (Var u - Var v) / 4, so four moments replace the five of ssim_body.  The
+ * vertical pass reads no LDS and recomputes no halo rows; the tile's halo
+ * costs (kRwTY + 2R) / kRwTY in rows.  Same shift and renormalisation as
+ * ssim_body. */
+constexpr int kRwThreads = 128;
+constexpr int kRwTX = 2 * kRwThreads;
+constexpr int kRwTY = 64;
+constexpr int kRwMaxRadius = 6;
+
+__host__ __device__ __forceinline__ int rw_row_chunks(int R, int ch) { return ((kRwTX + 2 * R) * ch) / 16 + 2; }
+/* dynamic LDS: two chunk buffers of 2 images x K rows, then the rows of u,
+ * v, u^2, v^2, double-buffered */
+static size_t rw_lds_bytes(int R, int ch) {
+    return (size_t)2 * 2 * (2 * R + 1) * rw_row_chunks(R, ch) * 16 + (size_t)2 * 4 * (kRwTX + 2 * R) * sizeof(float);
+}
+
+__device__ __forceinline__ float luma_rw(const uint8_t* p, int ch) {
+    float r, g, b;
+    if (ch == 1) {
+        r = g = b = (float)p[0];
+    } else {
+        r = (float)p[0];
+        g = (float)p[1];
+        b = (float)p[2];
+    }
+    return (0.299f * r + 0.581f * g + 0.114f * b) * (1.0f / 255.0f);
+}
+
+template <int R>
+__device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
+                                             long pair_stride, int w, int h, int ch, const SsimWeights& wt,
+                                             float c1, float c2, float* __restrict__ map,
+                                             double* __restrict__ partial, int bx, int by, int gx, int gy,
+                                             int pair) {
+    constexpr int K = 2 * R + 1;
+    constexpr int LW = kRwTX + 2 * R; /* window columns (even) */
+    extern __shared__ uint4 rw_smem[];
+    __shared__ double red[kRwThreads / 64];
+
+    const uint8_t* A = img + (long)pair * pair_stride;
+    const uint8_t* B = ref + (long)pair * pair_stride;
+    const int tid = threadIdx.x, wv = tid >> 6;
+    const int x0 = bx * kRwTX, y0 = by * kRwTY;
+    const int c0 = max(x0 - R, 0), cend = min(x0 + kRwTX + R, w);
+    const int seg = (cend - c0) * ch;
+    const int rc = rw_row_chunks(R, ch);
+    const int slots = 2 * K * rc;
+    float* lum = (float*)(rw_smem + 2 * slots); /* [buf][u, v, u^2, v^2][LW] */
+    const int nwin = kRwTY + 2 * R;
+    const int nch = (nwin + K - 1) / K;
+
+    /* shift: luma of the tile centre (read by every lane from one address) */
+    const int cx = min(x0 + kRwTX / 2, w - 1), cy = min(y0 + kRwTY / 2, h - 1);
+    const float sx = luma_rw(A + ((long)cy * w + cx) * ch, ch);
+    const float sy = luma_rw(B + ((long)cy * w + cx) * ch, ch);
+
+    /* chunk c -> stage buffer: slot (im * K + r) * rc + k holds the aligned
+     * 16-byte block k of window row c*K + r of image im; lane l of a wave
+     * writes slot (wave base + l) */
+    auto issue = [&](int c, uint4* st) {
+        for (int base = 0; base < slots; base += kRwThreads) {
+            const int i = base + tid;
+            if (i < slots) {
+                const int rr = i / rc, k = i - rr * rc;
+                const int im = rr >= K, r = rr - im * K;
+                const int wr = c * K + r, gyy = y0 - R + wr;
+                if (wr < nwin && gyy >= 0 && gyy < h) {
+                    const uintptr_t a0 = (uintptr_t)((im ? B : A) + ((long)gyy * w + c0) * ch);
+                    const uintptr_t alast = (a0 + seg - 1) & ~(uintptr_t)15;
+                    uintptr_t a = (a0 & ~(uintptr_t)15) + (uintptr_t)k * 16;
+                    if (a > alast) a = alast; /* spare slot: a block of this row, never read */
+                    __builtin_amdgcn_global_load_lds((const void*)a,
+                                                     (__attribute__((address_space(3))) void*)(st + base + wv * 64),
+                                                     16, 0, 0);
+                }
+            }
+        }
+    };
+
+    /* reciprocal in-bounds weight mass per column, and of the full window */
+    float rgx[2];
+#pragma unroll
+    for (int col = 0; col < 2; ++col) {
+        const int x = x0 + 2 * tid + col;
+        float sgx = 0.f;
+#pragma unroll
+        for (int d = -R; d <= R; ++d)
+            if (x + d >= 0 && x + d < w) sgx += wt.g[d + R];
+        rgx[col] = 1.0f / sgx; /* infinite only for columns past the image, which are not stored */
+    }
+    float gfull = 0.f;
+#pragma unroll
+    for (int d = 0; d <= 2 * R; ++d) gfull += wt.g[d];
+    const float rgfull = 1.0f / gfull;
+
+    float ring[K][2][4];
+    double crop = 0.0;
+
+    issue(0, rw_smem);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        if (c + 1 < nch) issue(c + 1, rw_smem + ((c + 1) & 1) * slots);
+        const uint8_t* sb = (const uint8_t*)(rw_smem + (c & 1) * slots);
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            const int i = c * K + p; /* window row */
+            if (i < nwin) {
+                /* 2. luma of window row i */
+                const int gyy = y0 - R + i;
+                const bool rowin = gyy >= 0 && gyy < h;
+                float* lq = lum + (i & 1) * 4 * LW; /* u, v, u^2, v^2 rows */
+                int off[2] = {0, 0};
+                if (rowin) {
+                    off[0] = (p * rc) * 16 + (int)(((uintptr_t)(A + ((long)gyy * w + c0) * ch)) & 15);
+                    off[1] = ((K + p) * rc) * 16 + (int)(((uintptr_t)(B + ((long)gyy * w + c0) * ch)) & 15);
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int j = q == 0 ? 2 * tid : kRwTX + 2 * tid;
+                    if (q == 0 || tid < R) {
+                        float uv[2], vv[2];
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) {
+                            const int gxx = x0 - R + j + e;
+                            uv[e] = vv[e] = 0.f;
+                            if (rowin && gxx >= 0 && gxx < w) {
+                                const int bo = (gxx - c0) * ch;
+                                const float a = luma_rw(sb + off[0] + bo, ch) - sx;
+                                const float b = luma_rw(sb + off[1] + bo, ch) - sy;
+                                uv[e] = a + b;
+                                vv[e] = a - b;
+                            }
+                        }
+                        *(float2*)(lq + j) = make_float2(uv[0], uv[1]);
+                        *(float2*)(lq + LW + j) = make_float2(vv[0], vv[1]);
+                        *(float2*)(lq + 2 * LW + j) = make_float2(uv[0] * uv[0], uv[1] * uv[1]);
+                        *(float2*)(lq + 3 * LW + j) = make_float2(vv[0] * vv[0], vv[1] * vv[1]);
+                    }
+                }
+                lds_barrier();
+                /* 3. horizontal sums of columns 2t, 2t+1 into ring slot p */
+                float m[2][4];
+#pragma unroll
+                for (int col = 0; col < 2; ++col)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) m[col][k] = 0.f;
+#pragma unroll
+                for (int k2 = 0; k2 < 2 * R + 2; k2 += 2) {
+                    float2 q2[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) q2[k] = *(const float2*)(lq + k * LW + 2 * tid + k2);
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int kk = k2 + e;
+#pragma unroll
+                        for (int col = 0; col < 2; ++col) {
+                            const int d = kk - col;
+                            if (d >= 0 && d <= 2 * R) {
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) m[col][k] += wt.g[d] * (e ? q2[k].y : q2[k].x);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int col = 0; col < 2; ++col)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) ring[p][col][k] = m[col][k];
+                /* 4. output row i - 2R */
+                if (i >= 2 * R) {
+                    const int y = y0 + i - 2 * R;
+                    if (y < h) {
+                        float rgy = rgfull;
+                        if (y < R || y + R >= h) {
+                            float gys = 0.f;
+#pragma unroll
+                            for (int d = 0; d <= 2 * R; ++d) {
+                                const int yy = y + d - R;
+                                if (yy >= 0 && yy < h) gys += wt.g[d];
+                            }
+                            rgy = 1.0f / gys;
+                        }
+                        float out[2];
+#pragma unroll
+                        for (int col = 0; col < 2; ++col) {
+                            float sm[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int d = 0; d <= 2 * R; ++d) {
+                                const int slot = (p + 1 + d) % K;
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) sm[k] += wt.g[d] * ring[slot][col][k];
+                            }
+                            const float inv = rgx[col] * rgy;
+                            const float mu = sm[0] * inv, mv = sm[1] * inv;
+                            const float vu = sm[2] * inv - mu * mu; /* Var u = var_x + var_y + 2 cov */
+                            const float vv = sm[3] * inv - mv * mv; /* Var v = var_x + var_y - 2 cov */
+                            const float mi = 0.5f * (mu + mv) + sx, mr = 0.5f * (mu - mv) + sy;
+                            const float num = (2.0f * mi * mr + c1) * (0.5f * (vu - vv) + c2);
+                            const float den = (mi * mi + mr * mr + c1) * (0.5f * (vu + vv) + c2);
+                            out[col] = num * __builtin_amdgcn_rcpf(den);
+                        }
+                        const int x = x0 + 2 * tid;
+                        float* mrow = map ? map + (long)pair * w * h + (long)y * w : nullptr;
+#pragma unroll
+                        for (int col = 0; col < 2; ++col) {
+                            if (x + col < w) {
+                                if (mrow) mrow[x + col] = out[col];
+                                if (x + col >= R && x + col <= w - R && y >= R && y <= h - R)
+                                    crop += (double)out[col];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        /* chunk c + 1 has landed in LDS for every wave */
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+    for (int d = 32; d > 0; d >>= 1) crop += __shfl_xor(crop, d, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = crop;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int i = 0; i < kRwThreads / 64; ++i) sum += red[i];
+        partial[((long)pair * gy + by) * gx + bx] = sum;
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kRwThreads) void ssim_rw_kernel(const uint8_t* __restrict__ img,
+                                                             const uint8_t* __restrict__ ref, long pair_stride,
+                                                             int w, int h, int ch, SsimWeights wt, float c1,
+                                                             float c2, float* __restrict__ map,
+                                                             double* __restrict__ partial) {
+    ssim_rw_body<R>(img, ref, pair_stride, w, h, ch, wt, c1, c2, map, partial, blockIdx.x, blockIdx.y, gridDim.x,
+                    gridDim.y, blockIdx.z);
+}
+
 __device__ __forceinline__ void ssim_finalize_body(const double* __restrict__ partial, int nblk, double count,
                                                    double* __restrict__ out, int pair) {
     __shared__ double sh[kBlock];
@@ -231,6 +489,17 @@ __global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __r
                                                                int nblk, double count,
                                                                double* __restrict__ out) {
     ssim_finalize_body(partial, nblk, count, out, blockIdx.x);
+}
+
+/* output tile of the kernel used for radius R */
+static void ssim_tile(int R, int& TX, int& TY) {
+    if (R <= kRwMaxRadius) {
+        TX = kRwTX;
+        TY = kRwTY;
+    } else {
+        TX = R <= 7 ? 64 : 32;
+        TY = R <= 7 ? 16 : 8;
+    }
 }
 
 template <int R>
@@ -256,13 +525,27 @@ double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     SsimWeights wt{};
     const double sg = (double)sigma;
     for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
-    const int TX = R <= 7 ? 64 : 32, TY = R <= 7 ? 16 : 8;
+    int TX, TY;
+    ssim_tile(R, TX, TY);
     dim3 grid(ceil_div(w, TX), ceil_div(h, TY), n);
     const int nblk = grid.x * grid.y;
     double* partial = (double*)workspace(WS_SSIM, sizeof(double) * ((size_t)nblk * n + n));
     double* out = partial + (size_t)nblk * n;
     const float c1 = k1 * k1, c2 = k2 * k2;
-    switch (R) {
+    if (R <= kRwMaxRadius) {
+        const size_t lds = rw_lds_bytes(R, ch);
+        switch (R) {
+#define IQM_SSIM_RW_CASE(r)                                                                                     \
+    case r:                                                                                                     \
+        launch("ssim", ssim_rw_kernel<r>, grid, dim3(kRwThreads), lds, d_img, d_ref, pair_stride, w, h, ch, wt, \
+               c1, c2, d_maps, partial);                                                                        \
+        break;
+            IQM_SSIM_RW_CASE(1) IQM_SSIM_RW_CASE(2) IQM_SSIM_RW_CASE(3)
+            IQM_SSIM_RW_CASE(4) IQM_SSIM_RW_CASE(5) IQM_SSIM_RW_CASE(6)
+#undef IQM_SSIM_RW_CASE
+            default: break;
+        }
+    } else switch (R) {
 #define IQM_SSIM_CASE(r) \
     case r: launch_ssim<r>(*this, grid, d_img, d_ref, pair_stride, w, h, ch, wt, c1, c2, d_maps, partial); break;
         IQM_SSIM_CASE(1) IQM_SSIM_CASE(2) IQM_SSIM_CASE(3) IQM_SSIM_CASE(4) IQM_SSIM_CASE(5)
@@ -303,6 +586,17 @@ __global__ __launch_bounds__(kBlock) void mix_ssim_kernel(const MixJob* __restri
                          local % J.ssim_gx, local / J.ssim_gx, J.ssim_gx, J.ssim_gy, 0);
 }
 
+template <int R>
+__global__ __launch_bounds__(kRwThreads) void mix_ssim_rw_kernel(const MixJob* __restrict__ jobs,
+                                                                 const int* __restrict__ st, int n, SsimWeights wt,
+                                                                 float c1, float c2) {
+    const int j = ssim_mix_find(st, n, blockIdx.x);
+    const MixJob& J = jobs[j];
+    const int local = blockIdx.x - st[j];
+    ssim_rw_body<R>(J.img, J.ref, 0, J.w, J.h, J.ch, wt, c1, c2, J.ssim_map, J.ssim_partial, local % J.ssim_gx,
+                    local / J.ssim_gx, J.ssim_gx, J.ssim_gy, 0);
+}
+
 /* finalize, then the pair's MSSIM into its row of the batch result */
 __global__ __launch_bounds__(kBlock) void mix_ssim_finalize_kernel(const MixJob* __restrict__ jobs,
                                                                    const int* __restrict__ st, int n) {
@@ -312,10 +606,6 @@ __global__ __launch_bounds__(kBlock) void mix_ssim_finalize_kernel(const MixJob*
     if (threadIdx.x == 0) J.out[0] = J.ssim_out[0];
 }
 
-static void ssim_tile(int R, int& TX, int& TY) {
-    TX = R <= 7 ? 64 : 32;
-    TY = R <= 7 ? 16 : 8;
-}
 
 void Ctx::mix_ssim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, MixCounts& cnt, int* blk,
                         int* cls, const SsimArgs& sp) {
@@ -341,7 +631,10 @@ void Ctx::mix_ssim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
 template <int R>
 static void launch_ssim_mix(Ctx& c, int nblocks, const MixJob* jobs, const int* st, int n, const SsimWeights& wt,
                             float c1, float c2) {
-    if constexpr (R <= 7)
+    if constexpr (R <= kRwMaxRadius)
+        c.launch("ssim", mix_ssim_rw_kernel<R>, dim3(nblocks), dim3(kRwThreads), rw_lds_bytes(R, 4), jobs, st, n,
+                 wt, c1, c2); /* sized for 4 channels, the widest input */
+    else if constexpr (R <= 7)
         c.launch("ssim", mix_ssim_kernel<R, 64, 16>, dim3(nblocks), dim3(kBlock), 0, jobs, st, n, wt, c1, c2);
     else
         c.launch("ssim", mix_ssim_kernel<R, 32, 8>, dim3(nblocks), dim3(kBlock), 0, jobs, st, n, wt, c1, c2);
