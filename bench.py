@@ -98,25 +98,31 @@ def kernel_bytes_per_pair(w: int, h: int, c_in: int, wd: int, hd: int) -> dict:
         "fsim_prep": 2 * c_in * w * h + 6 * N * 4,          # u8 pair in, 6 YIQ planes out
         "fsim_rowfft": 2 * N * 4 + 2 * N * 8,               # Y_in, Y_ref in; 2 row spectra out
         "fsim_colpass": 2 * N * 8 + 32 * N * 8,             # spectra in; 32 response planes out
-        "fsim_rowpass": 32 * N * 8 + 24 * N * 4,            # 32 planes in; An, E (16) + keys (8) out
-        "fsim_select": 2 * 8 * N * 4,                       # keys, two radix-select passes
-        "fsim_pcfinal": 6 * N * 4 + 16 * N * 4,             # YIQ + An/E in
+        "fsim_rowpass": 32 * N * 8 + 18 * N * 4,            # 32 planes in; sum An (2), E (8), keys (8) out
+        "fsim_select": 8 * N * 4,                           # keys read once (level 3 reads LDS)
+        "fsim_pcfinal": 6 * N * 4 + 10 * N * 4,             # YIQ + sum An / E in
         "fsim_finalize": 0,
     }
 
 
-def read_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (two --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated against a
-    device copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "fsim_rowpass_1080p_pmc.md")
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 PMC summaries under profiles/ (two --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, calibrated against a device copy of known size), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    key = f"traffic_bytes_per_launch[{workload}:{kernel}] ="
     try:
+        names = sorted(os.listdir(pdir))
+    except OSError:
+        return None, None
+    for name in names:
+        if not name.endswith(".md"):
+            continue
+        path = os.path.join(pdir, name)
         with open(path) as f:
             for line in f:
-                if line.startswith(f"traffic_bytes_per_launch[{kernel}] ="):
+                if line.startswith(key):
                     return int(line.split("=")[1].split()[0]), os.path.relpath(path, ROOT)
-    except OSError:
-        pass
     return None, None
 
 
@@ -298,13 +304,15 @@ def run_ssim(args, ctx) -> dict:
         ctx.free(p)
     per_pair = 2 * 1 * w * h + 4 * w * h + 8
     k_us = stages["ssim"]["avg_us"]
+    traffic, traffic_src = read_traffic("ssim", "ssim_512x512_gray_batch256")
     out = {"metric": "512x512 gray SSIM pairs/s (map + MSSIM)", "value": B / (ms * 1e-3), "unit": "pairs/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "ssim_512x512_gray_batch256", "pairs_per_step": B, "mssim": float(m[0])},
            "roofline": {"bound": "hbm", "achieved": per_pair * B / (k_us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9,
-                        "unit": "GB/s", "frac": per_pair * B / (k_us * 1e-6) / HBM_PEAK, "traffic": None,
-                        "kernel": "ssim", "alg_bytes_per_launch": per_pair * B, "avg_launch_us": k_us},
+                        "unit": "GB/s", "frac": per_pair * B / (k_us * 1e-6) / HBM_PEAK, "traffic": traffic,
+                        "kernel": "ssim", "alg_bytes_per_launch": per_pair * B, "avg_launch_us": k_us,
+                        "traffic_source": traffic_src},
            "stages_us": stages, "device": ctx.device_name}
     if not args.no_cpu:
         from oracle import oracle
@@ -481,6 +489,14 @@ def main() -> int:
     from iqm_amd import synth
 
     ctx = iqm_amd.Context(0)
+    if args.copy_calib_mb:  # one device copy of known size: the PMC counters' calibration
+        nb = args.copy_calib_mb << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        hip = _hip()
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), nb, 3) == 0  # device to device
+        ctx.free(a)
+        ctx.free(b)
     if args.workload in ("mixed", "stream", "ssim"):
         out = {"mixed": run_mixed, "stream": run_stream, "ssim": run_ssim}[args.workload](args, ctx)
         ctx.close()
@@ -496,14 +512,6 @@ def main() -> int:
     ctx.upload(d_img, imgs)
     ctx.upload(d_ref, refs)
     fsim = iqm_amd.FSIM()
-    if args.copy_calib_mb:
-        nb = args.copy_calib_mb << 20
-        a, b = ctx.alloc(nb), ctx.alloc(nb)
-        hip = _hip()
-        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        assert hip.hipMemcpy(ctypes.c_void_p(b), ctypes.c_void_p(a), nb, 3) == 0  # device to device
-        ctx.free(a)
-        ctx.free(b)
 
     def step():
         return fsim.compute_batch_dev(ctx, d_img, d_ref, W, H, CH, pair_bytes, B)
@@ -552,7 +560,7 @@ def main() -> int:
     if dominant:
         dom_bytes = kbytes[dominant] * B  # one launch covers the whole batch
         dom_us = stages[dominant]["avg_us"]
-        traffic, traffic_src = read_traffic(dominant)
+        traffic, traffic_src = read_traffic(dominant, "fsimc_exact_1920x1080_rgb")
         roofline = {"bound": "hbm", "achieved": dom_bytes / (dom_us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9,
                     "unit": "GB/s", "frac": dom_bytes / (dom_us * 1e-6) / HBM_PEAK, "traffic": traffic,
                     "kernel": dominant, "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_us,
