@@ -49,7 +49,7 @@ struct FsimTune {
     int row_rb = 2;    /* rows per row-pass workgroup */
     int fft_rb = 8;    /* rows per row-FFT workgroup */
     int prep_rb = 1;   /* downscaled rows per prep workgroup */
-    double us_per_pair = 0; /* measured by the autotuner (0 = default, not measured) */
+    double us_per_pair = 0; /* measured by the autotuner; 0: built-in defaults, < 0: explicit tiles */
 };
 
 /* One pair of a mixed batch (device pointers, packed rows). */

@@ -1,8 +1,10 @@
 // Generic batched 2-D FFT kernels (rows, then columns) and the host planner.
 // They back iqm_fft2d and the per-size constants of FSIM; the FSIM hot path
 // uses fused variants of the same LDS line transform (fsim.hip).
-#include "fft.hpp"
+#include "fft_line.hpp"
 #include "ctx.hpp"
+
+#include <algorithm>
 
 #include <cmath>
 #include <complex>
@@ -50,8 +52,13 @@ void host_dft(std::vector<std::complex<double>>& a) {
         }
 }
 
-/* smallest length >= n that the direct Stockham path plans */
+/* smallest length >= n that the direct transforms plan: the smallest with a
+ * two-level split when one exists up to 32 * 32, else the smallest the
+ * Stockham schedule plans */
 int next_plannable(int n) {
+    int a, b;
+    for (int m = n; m <= 32 * 32; ++m)
+        if (tl2_split(m, a, b)) return m;
     std::vector<int> rad;
     for (int m = n;; ++m)
         if (plan_radices(m, rad)) return m;
@@ -92,6 +99,7 @@ const LinePlanHost& Ctx::line_plan(int n) {
         float2* tw = upload_table(twiddle_table(n));
         P.tw = tw;
         P.lds_len = n;
+        if (tl2_split(n, P.tn1, P.tn2)) P.lds_len = std::max(n, P.tn1 * (P.tn2 + 1));
         lp.owned = {tw};
     } else {
         /* any plannable m >= 2n - 1 holds the linear convolution */
@@ -137,6 +145,7 @@ const LinePlanHost& Ctx::line_plan(int n) {
         P.gf = g1;
         P.gi = g2;
         P.lds_len = m;
+        if (tl2_split(m, P.tm1, P.tm2)) P.lds_len = std::max(m, P.tm1 * (P.tm2 + 1));
         lp.owned = {mtw, ch, g1, g2};
     }
     return line_plans_.emplace(n, lp).first->second;

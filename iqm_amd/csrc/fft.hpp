@@ -11,7 +11,8 @@
 // Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies.  A
 // length with a larger prime factor goes through Bluestein's chirp-z
 // identity X_k = c_k * sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(S*pi*i*j^2/n),
-// evaluated as a circular convolution of power-of-two length m >= 2n-1.
+// evaluated as a circular convolution of length m >= 2n-1 (the smallest such
+// m the line transforms handle directly).
 // Twiddles and chirps are computed in double on the host and stored as float.
 #pragma once
 
@@ -38,8 +39,40 @@ struct LinePlan {
     const float2* chirp;   /* exp(-pi i (j^2 mod 2n) / n), j < n                */
     const float2* gf;      /* FFT_m of the forward convolution kernel           */
     const float2* gi;      /* FFT_m of the inverse convolution kernel           */
-    int lds_len;           /* elements a line occupies in LDS (n or m)          */
+    int lds_len;           /* elements a line occupies in LDS (n or m, or the
+                              two-level exchange area when that is longer)      */
+    int tn1, tn2;          /* runtime two-level split n = tn1 * tn2 (0: none)   */
+    int tm1, tm2;          /* the same for the Bluestein length m               */
 };
+
+/* Sizes of the register DFTs the runtime two-level transform dispatches to
+ * (fft_line.hpp); a length n = n1 * n2 with both in this set is transformed
+ * with one LDS exchange instead of one LDS pass per radix stage. */
+#define IQM_TL2_SIZES(X)                                                                                    \
+    X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(18) X(20) X(21) X(22) \
+    X(24) X(25) X(26) X(27) X(28) X(30) X(32)
+
+inline bool tl2_size(int m) {
+    switch (m) {
+#define IQM_TL2_CASE(k) case k:
+        IQM_TL2_SIZES(IQM_TL2_CASE)
+#undef IQM_TL2_CASE
+        return true;
+        default: return false;
+    }
+}
+
+/* Most balanced split n = n1 * n2 with both factors in IQM_TL2_SIZES
+ * (n1 <= n2); false when there is none. */
+inline bool tl2_split(int n, int& n1, int& n2) {
+    n1 = n2 = 0;
+    for (int a = 2; a * a <= n; ++a)
+        if (n % a == 0 && tl2_size(a) && tl2_size(n / a)) {
+            n1 = a;
+            n2 = n / a;
+        }
+    return n1 > 0;
+}
 
 /* ---- small-radix DFTs (y_t = sum_q x_q exp(S*2*pi*i*q*t/R)) -------------- */
 
@@ -267,45 +300,6 @@ __device__ __forceinline__ const float2* stage_twiddles(const LinePlan& P, float
     if (lds_twiddles(P) == 0) return P.blue ? P.mtw : P.tw;
     for (int i = threadIdx.x; i < P.n; i += blockDim.x) tw_lds[i] = P.tw[i];
     return tw_lds;
-}
-
-/* Full line transform of nlines lines whose first n elements hold the input;
- * lines are ld >= P.lds_len apart.  Returns the buffer holding the result
- * (first n elements of each line).  Starts and ends at a barrier. */
-template <int S>
-__device__ float2* line_fft(float2* b0, float2* b1, int ld, int nlines, const LinePlan& P,
-                            const float2* tw) {
-    if (!P.blue) return lds_fft_direct<S>(b0, b1, ld, nlines, P.n, P.nst, P.rad, tw);
-    const int n = P.n, m = P.m;
-    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-        const int l = idx / m, j = idx - l * m;
-        float2 v = make_float2(0.f, 0.f);
-        if (j < n) {
-            float2 c = P.chirp[j];
-            if (S > 0) c.y = -c.y;
-            v = cmul(b0[l * ld + j], c);
-        }
-        b0[l * ld + j] = v;
-    }
-    lds_barrier();
-    float2* r = lds_fft_direct<-1>(b0, b1, ld, nlines, m, P.mst, P.mrad, P.mtw);
-    float2* other = (r == b0) ? b1 : b0;
-    const float2* g = (S < 0) ? P.gf : P.gi;
-    for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-        const int l = idx / m, j = idx - l * m;
-        r[l * ld + j] = cmul(r[l * ld + j], g[j]);
-    }
-    lds_barrier();
-    float2* q = lds_fft_direct<1>(r, other, ld, nlines, m, P.mst, P.mrad, P.mtw);
-    const float inv_m = 1.0f / (float)m;
-    for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-        const int l = idx / n, k = idx - l * n;
-        float2 c = P.chirp[k];
-        if (S > 0) c.y = -c.y;
-        q[l * ld + k] = cscale(cmul(q[l * ld + k], c), inv_m);
-    }
-    lds_barrier();
-    return q;
 }
 
 /* Host-side planner: radices for n (largest power-of-two radices first,

@@ -17,7 +17,7 @@
 // Per-size constants (filter planes, EM_n of F10, est of F14) are computed
 // once per (wd, hd) by Ctx::fsim_plan and cached.
 #include "ctx.hpp"
-#include "fft.hpp"
+#include "fft_line.hpp"
 #include "fsim_tl.hpp"
 
 #include <algorithm>
@@ -988,13 +988,20 @@ FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long 
     /* 64 KiB per workgroup keeps two workgroups per CU; when that holds
      * fewer than 4 lines (or the cap), long (Bluestein) lines take the
      * device's whole LDS for more work per barrier */
+    /* The generic transforms (two-level with a run-time split, Stockham,
+     * Bluestein) fill a workgroup's threads only with enough lines: without
+     * a measured tune they take up to 32 columns, 4 x 4 rows and 16 rows per
+     * workgroup, in up to the whole LDS (mixed 256-pair batch: 39.6 ms
+     * against 49.3 ms with 16 / 2 x 4 / 8 in 64 KiB) */
+    const bool measured = tune.us_per_pair != 0; /* tuned or explicitly set tiles */
+    const size_t pref = (size_t)env_int("IQM_FSIM_LDS_PREF", (int)c.lds_per_block());
     auto fit = [&](const LinePlan& P, int bufs, int cap, int group) {
-        const int l = lines_that_fit(P, bufs, std::min<size_t>(c.lds_per_block(), 65536), group * cap) / group;
+        const int l = lines_that_fit(P, bufs, std::min<size_t>(c.lds_per_block(), pref), group * cap) / group;
         if (l >= std::min(cap, 4)) return l;
         return std::max(l, lines_that_fit(P, bufs, c.lds_per_block(), group * cap) / group);
     };
-    t.grad_rb = fit(Px, 2, env_int("IQM_FSIM_GRB", tune.fft_rb), 1);
-    t.col_bx = fit(Py, 3, env_int("IQM_FSIM_BX", tune.col_bx), 1);
+    t.grad_rb = fit(Px, 2, env_int("IQM_FSIM_GRB", measured ? tune.fft_rb : 16), 1);
+    t.col_bx = fit(Py, 3, env_int("IQM_FSIM_BX", measured ? tune.col_bx : 32), 1);
     /* row pass: 4 scales per row, plus the histogram and the An sums */
     auto fit_row = [&](int cap) {
         auto best = [&](size_t budget) {
@@ -1003,11 +1010,11 @@ FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py, long 
                 if (rowpass_lds_bytes(Px, r, Px.n) <= budget) b = r;
             return b;
         };
-        const int l = best(std::min<size_t>(c.lds_per_block(), 65536));
+        const int l = best(std::min<size_t>(c.lds_per_block(), pref));
         if (l >= std::min(cap, 4)) return l;
         return std::max(l, best(c.lds_per_block()));
     };
-    t.row_rb = fit_row(env_int("IQM_FSIM_RB", tune.row_rb));
+    t.row_rb = fit_row(env_int("IQM_FSIM_RB", measured ? tune.row_rb : 4));
     /* pairs per column/row-pass launch: one launch over the whole batch
      * measured fastest on MI355X (larger grids beat cache residency of the
      * response planes); the cap bounds the response-plane workspace */

@@ -46,10 +46,14 @@ FsimTune Ctx::autotune_fsim(int w, int h, int ch, int n) {
 
     std::vector<FsimTune> cand;
     const bool split = tl_supported(wd) || tl_supported(hd);
+    /* two-level kernels: small tiles; generic transforms: tiles that fill
+     * the workgroup's threads */
+    const std::vector<int> bx_tl{8, 16}, rb_tl{1, 2, 4}, fr_tl{2, 4, 8};
+    const std::vector<int> bx_g{16, 24, 32}, rb_g{2, 3, 4}, fr_g{8, 16};
     for (int tl = split ? 1 : 0; tl >= 0; --tl)
-        for (int bx : {8, 16})
-            for (int rb : {1, 2, 4})
-                for (int fr : {2, 4, 8})
+        for (int bx : tl ? bx_tl : bx_g)
+            for (int rb : tl ? rb_tl : rb_g)
+                for (int fr : tl ? fr_tl : fr_g)
                     for (int pr : {1, 2}) {
                         FsimTune t;
                         t.two_level = tl;
@@ -57,6 +61,7 @@ FsimTune Ctx::autotune_fsim(int w, int h, int ch, int n) {
                         t.row_rb = rb;
                         t.fft_rb = fr;
                         t.prep_rb = pr;
+                        t.us_per_pair = -1; /* explicit tiles (not the built-in defaults) */
                         cand.push_back(t);
                     }
     const bool was_timing = timing_;

@@ -1,6 +1,7 @@
 // Host-side check of the FFT index math (no GPU needed): runs the same
 // __host__ __device__ building blocks the kernels use -- Stockham stages,
-// register DFTs and the two-level line split -- sequentially on the CPU and
+// register DFTs, the compile-time two-level line split and the run-time
+// two-level split with its size dispatch -- sequentially on the CPU and
 // prints the largest error against a direct double-precision DFT.
 #include <cmath>
 #include <complex>
@@ -8,7 +9,7 @@
 #include <random>
 #include <vector>
 
-#include "fft_reg.hpp"
+#include "fft_line.hpp"
 
 using namespace iqm;
 using cd = std::complex<double>;
@@ -102,6 +103,21 @@ static double check_two_level() {
     return relerr(out, dft(x, S));
 }
 
+/* the run-time split as tl2_lines runs it: every step-1 item, then every
+ * step-2 item, through an exchange area of n1 * (n2 + 1) points */
+template <int S>
+static double check_tl2(int n) {
+    int n1, n2;
+    if (!tl2_split(n, n1, n2)) return -1;
+    auto x = rnd(n, 11 * n);
+    auto tw = twiddles(n);
+    std::vector<float2> in(n), z(n1 * (n2 + 1));
+    for (int i = 0; i < n; ++i) in[i] = make_float2((float)x[i].real(), (float)x[i].imag());
+    for (int j2 = 0; j2 < n2; ++j2) tl2_item1<S>(n1, in.data(), j2, n2, tw.data(), z.data());
+    for (int k1 = 0; k1 < n1; ++k1) tl2_item2<S>(n2, z.data(), k1, n1, in.data());
+    return relerr(in, dft(x, S));
+}
+
 int main() {
     double worst = 0;
     auto rep = [&](const char* what, int n, double e) {
@@ -123,6 +139,16 @@ int main() {
     rep("two_level_inv", 270, check_two_level<15, 18, 1>());
     rep("two_level_fwd", 480, check_two_level<20, 24, -1>());
     rep("two_level_inv", 480, check_two_level<20, 24, 1>());
+    int nsplit = 0;
+    for (int n = 4; n <= 1024; ++n) {
+        int a, b;
+        if (!tl2_split(n, a, b)) continue;
+        if (n % 3 != 0 && n % 7 != 1 && n != 1024 && n != 338) continue; /* a spread of lengths and splits */
+        ++nsplit;
+        rep("tl2_fwd", n, check_tl2<-1>(n));
+        rep("tl2_inv", n, check_tl2<1>(n));
+    }
+    std::printf("run-time splits checked: %d\n", nsplit);
     std::printf("worst %.3e\n", worst);
     return worst < 2e-6 ? 0 : 1;
 }
