@@ -507,6 +507,7 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
     const int y0 = bx * rb;
     const int nr = min(rb, hd - y0);
     float* ap = an_e + ((long)pair * 2 + img) * kAnePlanes * N;
+    const float epsN = 0.0001f / invN;
     /* when the 4 * rb lines fit kRowPrefetch elements per thread, the next
      * orientation's lines are loaded into registers while the current one is
      * transformed (the transform's barriers order LDS only) */
@@ -558,21 +559,10 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
         for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
             const int r = idx / wd, x = idx - r * wd;
             float2 eo[4];
-            float amp = 0.f, sr = 0.f, si = 0.f;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                eo[s] = cscale(R[(s * rb + r) * ld + x], invN);
-                amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
-                sr += eo[s].x;
-                si += eo[s].y;
-            }
-            const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
-            const float me = sr / xe, mo = si / xe;
-            float energy = 0.f;
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
-            const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+            for (int s = 0; s < 4; ++s) eo[s] = R[(s * rb + r) * ld + x];
+            float amp, energy, key;
+            fsim_response_epilogue(eo, invN, epsN, amp, energy, key);
             const long g = (long)(y0 + r) * wd + x;
             /* sum over o, in the order o = 0..3 (each idx stays with one thread) */
             if (o == 0) sa[idx] = amp;

@@ -110,7 +110,7 @@ template <int N1, int N2>
 constexpr int row_slots() { return ((N1 < N2 ? N1 : N2) + 3) / 4; }
 
 template <int N1, int N2>
-__global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void fsim_rowpass_tl_kernel(const float2* __restrict__ Tin,
                                                                float* __restrict__ an_e,
                                                                float* __restrict__ keys,
                                                                unsigned* __restrict__ hist,
@@ -151,6 +151,7 @@ __global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __re
     for (int i = threadIdx.x; i < kHistBinsTl; i += blockDim.x) hl[i] = 0u;
     lds_barrier();
     float* ap = an_e + ((long)pair * 2 + img) * kAnePlanes * N;
+    const float epsN = 0.0001f / invN;
     constexpr int kSlots = row_slots<N1, N2>();
     float sa[kSlots];
     for (int o = 0; o < 4; ++o) {
@@ -187,21 +188,10 @@ __global__ __launch_bounds__(512) void fsim_rowpass_tl_kernel(const float2* __re
             if (idx < nr * n) {
                 const int r = idx / n, xx = idx - r * n;
                 float2 eo[4];
-                float amp = 0.f, sr = 0.f, si = 0.f;
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    eo[s] = cscale(E[(s * rb + r) * LE + xx], invN);
-                    amp += sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
-                    sr += eo[s].x;
-                    si += eo[s].y;
-                }
-                const float xe = sqrtf(sr * sr + si * si) + 0.0001f;
-                const float me = sr / xe, mo = si / xe;
-                float energy = 0.f;
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    energy += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
-                const float key = eo[0].x * eo[0].x + eo[0].y * eo[0].y;
+                for (int s = 0; s < 4; ++s) eo[s] = E[(s * rb + r) * LE + xx];
+                float amp, energy, key;
+                fsim_response_epilogue(eo, invN, epsN, amp, energy, key);
                 const long g = (long)(y0 + r) * n + xx;
                 /* sum over o, in the order o = 0..3 */
                 if (o == 0) sa[q] = amp;

@@ -33,6 +33,31 @@ struct TlGradArgs {
     size_t lds_budget;
 };
 
+/* F12 (fsim_sum_filter_responses.glsl:33-83) for one pixel and orientation
+ * from the 4 scales' responses before the inverse transform's 1/N: with
+ * eo = invN * eo_u, the energy normaliser |sum eo| + 1e-4 is invN * (|sum eo_u|
+ * + 1e-4 * N), so the unit vector (me, mo) is the same and An, E scale by
+ * invN and the median key |eo_0|^2 by invN^2.  One reciprocal replaces the two
+ * divisions; square roots are the hardware's (1 ulp). */
+__device__ __forceinline__ void fsim_response_epilogue(const float2 (&eo)[4], float invN, float epsN, float& amp,
+                                                       float& energy, float& key) {
+    float a = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        a += __builtin_amdgcn_sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
+        sr += eo[s].x;
+        si += eo[s].y;
+    }
+    const float rx = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(sr * sr + si * si) + epsN);
+    const float me = sr * rx, mo = si * rx;
+    float e = 0.f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) e += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+    amp = a * invN;
+    energy = e * invN;
+    key = (eo[0].x * eo[0].x + eo[0].y * eo[0].y) * (invN * invN);
+}
+
 bool tl_supported(int n);
 void tl_rowfft(Ctx& c, int wd, const TlGradArgs& a);
 void tl_colpass(Ctx& c, int hd, const TlColArgs& a);
