@@ -367,6 +367,120 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
     }
 }
 
+/* F2 for wide boxes (F >= kPrep2MinF) on inputs whose rows are a multiple of
+ * 16 bytes apart (so every row of a box has the same alignment): a
+ * workgroup produces kPrep2Cols downscaled pixels of one row.  Phase 1: each
+ * thread takes aligned 16-byte chunks of the box's column range and adds the
+ * chunk over the F rows in registers, bytes in 16-bit lanes (even bytes, odd
+ * bytes), then writes the 16 column sums to LDS as u16.  Phase 2: each
+ * thread adds the column sums of its pixel's F x C bytes per channel.  The
+ * integer sums, and so the planes, are those of fsim_prep_body; LDS holds one
+ * row of sums instead of F rows of bytes. */
+constexpr int kPrep2Cols = 256;
+constexpr int kPrep2MinF = 6;
+__host__ __device__ __forceinline__ int prep2_chunks(int F, int ch, int cols) { return (cols * F * ch) / 16 + 2; }
+static size_t prep2_lds_bytes(int F, int ch, int cols) {
+    return (size_t)2 * prep2_chunks(F, ch, cols) * 16 * sizeof(uint16_t);
+}
+
+template <int FT>
+__device__ __forceinline__ void fsim_prep2_body(const uint8_t* __restrict__ img, const uint8_t* __restrict__ ref,
+                                                long pair_stride, int w, int h, int ch, int wd, int hd, int cols,
+                                                float* __restrict__ yiq, unsigned* __restrict__ hist, int bx,
+                                                int y, int pair) {
+    extern __shared__ uint4 p2_smem[];
+    uint16_t* cs = (uint16_t*)p2_smem; /* [image][chunk * 16 + byte] */
+    constexpr int F = FT;
+    const int tid = threadIdx.x;
+    if (bx == 0 && y == 0) {
+        unsigned* hp = hist + (long)pair * 8 * kHistBins;
+        for (int i = tid; i < 8 * kHistBins; i += blockDim.x) hp[i] = 0u;
+    }
+    const int xd0 = bx * cols;
+    const int ncols = min(cols, wd - xd0);
+    const int cc0 = max(xd0 * F - F / 2, 0), cc1 = min((xd0 + ncols) * F - F / 2, w);
+    const int ys = y * F - F / 2;
+    const int k0 = max(ys, 0), nk = min(ys + F, h) - k0;
+    const int seg = max(cc1 - cc0, 0) * ch;
+    const uint8_t* src[2] = {img + (long)pair * pair_stride, ref + (long)pair * pair_stride};
+    /* the box rows share their alignment d */
+    const uintptr_t a00 = (uintptr_t)(src[0] + ((long)k0 * w + cc0) * ch);
+    const uintptr_t a01 = (uintptr_t)(src[1] + ((long)k0 * w + cc0) * ch);
+    const int d0 = (int)(a00 & 15), d1 = (int)(a01 & 15);
+    const int K0 = (d0 + seg + 15) / 16, K1 = (d1 + seg + 15) / 16;
+    const int KS = prep2_chunks(F, ch, cols); /* chunk slots per image */
+    const long pitch = (long)w * ch;
+    for (int t = tid; t < K0 + K1; t += blockDim.x) {
+        const int im = t >= K0;
+        const int k = t - im * K0;
+        const uint4* p = (const uint4*)(((im ? a01 : a00) & ~(uintptr_t)15) + (uintptr_t)k * 16);
+        uint4 v[FT];
+#pragma unroll
+        for (int r = 0; r < FT; ++r)
+            v[r] = r < nk ? *(const uint4*)((const uint8_t*)p + r * pitch) : make_uint4(0u, 0u, 0u, 0u);
+        unsigned e[4] = {0u, 0u, 0u, 0u}, o[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int r = 0; r < FT; ++r) {
+            const unsigned q[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                e[i] += q[i] & 0x00FF00FFu;
+                o[i] += (q[i] >> 8) & 0x00FF00FFu;
+            }
+        }
+        /* u16 sums of bytes 4i .. 4i+3: (e lo, o lo, e hi, o hi) */
+        uint4 lo, hi;
+        lo.x = (e[0] & 0xFFFFu) | (o[0] << 16);
+        lo.y = (e[0] >> 16) | (o[0] & 0xFFFF0000u);
+        lo.z = (e[1] & 0xFFFFu) | (o[1] << 16);
+        lo.w = (e[1] >> 16) | (o[1] & 0xFFFF0000u);
+        hi.x = (e[2] & 0xFFFFu) | (o[2] << 16);
+        hi.y = (e[2] >> 16) | (o[2] & 0xFFFF0000u);
+        hi.z = (e[3] & 0xFFFFu) | (o[3] << 16);
+        hi.w = (e[3] >> 16) | (o[3] & 0xFFFF0000u);
+        uint4* dst = p2_smem + ((long)im * KS + k) * 2;
+        dst[0] = lo;
+        dst[1] = hi;
+    }
+    __syncthreads();
+    const long N = (long)wd * hd;
+    float* base = yiq + (long)pair * 6 * N;
+    const float inv = 1.0f / ((float)F * (float)F);
+    for (int xc = tid; xc < ncols; xc += blockDim.x) {
+        const int x = xd0 + xc;
+        const int xs = x * F - F / 2;
+        const int j0 = max(xs, cc0), j1 = min(xs + F, cc1);
+        const long idx = (long)y * wd + x;
+        for (int im = 0; im < 2; ++im) {
+            const uint16_t* c = cs + (long)im * KS * 16 + (im ? d1 : d0) + (j0 - cc0) * ch;
+            int sr = 0, sg = 0, sb = 0;
+            if (ch == 1) {
+                for (int j = 0; j < j1 - j0; ++j) sr += c[j];
+                sg = sb = sr;
+            } else {
+                for (int j = 0; j < j1 - j0; ++j) {
+                    sr += c[j * ch];
+                    sg += c[j * ch + 1];
+                    sb += c[j * ch + 2];
+                }
+            }
+            const float R = (float)sr, G = (float)sg, B = (float)sb;
+            base[(im ? 1 : 0) * N + idx] = (R * 0.299f + G * 0.587f + B * 0.114f) * inv;
+            base[(im ? 4 : 2) * N + idx] = (R * 0.596f - G * 0.274f - B * 0.322f) * inv;
+            base[(im ? 5 : 3) * N + idx] = (R * 0.211f - G * 0.523f + B * 0.312f) * inv;
+        }
+    }
+}
+
+template <int FT>
+__global__ __launch_bounds__(kBlock) void fsim_prep2_kernel(const uint8_t* __restrict__ img,
+                                                            const uint8_t* __restrict__ ref, long pair_stride,
+                                                            int w, int h, int ch, int wd, int hd, int cols,
+                                                            float* __restrict__ yiq, unsigned* __restrict__ hist) {
+    fsim_prep2_body<FT>(img, ref, pair_stride, w, h, ch, wd, hd, cols, yiq, hist, blockIdx.x, blockIdx.y,
+                        blockIdx.z);
+}
+
 /* FT = F for the common factors 1..8 (compile-time box width), 0 otherwise */
 template <int FT>
 __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __restrict__ img,
@@ -1037,6 +1151,19 @@ void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stri
     const int cols = prep_cols(F, ch, prb);
     const size_t bytes = prep_lds_bytes(F, ch, cols, prb);
     IQM_CHECK(bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+    /* wide boxes over rows a multiple of 16 bytes apart: the register
+     * column-sum kernel (each image's alignment is taken per workgroup) */
+    if (F >= kPrep2MinF && F <= 8 && ((long)w * ch) % 16 == 0 && env_int("IQM_FSIM_PREP2", 1)) {
+        const int c2 = std::min(kPrep2Cols, wd);
+        const dim3 g2(ceil_div(wd, c2), hd, n);
+        const size_t b2 = prep2_lds_bytes(F, ch, c2);
+        switch (F) {
+            case 6: launch("fsim_prep", fsim_prep2_kernel<6>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            case 7: launch("fsim_prep", fsim_prep2_kernel<7>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            default: launch("fsim_prep", fsim_prep2_kernel<8>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+        }
+        return;
+    }
     const dim3 grid(ceil_div(wd, cols), ceil_div(hd, prb), n);
     switch (F) {
 #define IQM_PREP_CASE(f)                                                                                      \

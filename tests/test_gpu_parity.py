@@ -4,6 +4,8 @@ Tolerances (FP32 path): scalars within 1e-5 absolute, maps within 1e-4
 max-abs, against both the FP32 mirror and the FP64 oracle; the errors
 against FP64 are printed for every case.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -135,6 +137,25 @@ def test_fsim_shapes(ctx, w, h, ch):
     elif ch == 4:
         img, ref = synth.rgba(img), synth.rgba(ref)
     _fsim_case(ctx, img, ref, f"fsim {w}x{h}x{ch}")
+
+
+@pytest.mark.parametrize("w,h,ch", [(3840, 2160, 3), (2560, 1536, 3), (3136, 1792, 4)])
+def test_fsim_wide_boxes(ctx, w, h, ch):
+    """Downscale factors 8, 6 and 7 take the register column-sum prep kernel
+    (rows 16-byte aligned); its planes must equal the LDS-staged kernel's bit
+    for bit (same integer box sums), and the result must match the oracle."""
+    img, ref = synth.rgb_pair(w, h, seed=h)
+    if ch == 4:
+        img, ref = synth.rgba(img), synth.rgba(ref)
+    r, _ = _fsim_case(ctx, img, ref, f"fsim {w}x{h}x{ch}", stages=True)
+    os.environ["IQM_FSIM_PREP2"] = "0"
+    try:
+        r0 = iqm_amd.FSIM().compute_metric(ctx, img, ref, maps=iqm_amd.FSIM.MAPS, detail=True)
+    finally:
+        del os.environ["IQM_FSIM_PREP2"]
+    for name in ("yiq_in", "yiq_ref"):
+        assert np.array_equal(r.detail[name], r0.detail[name]), name
+    assert r.fsim == r0.fsim and r.fsimc == r0.fsimc
 
 
 def test_fsim_identity(ctx):
