@@ -367,7 +367,7 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
     }
 }
 
-/* F2 for wide boxes (F >= kPrep2MinF) on inputs whose rows are a multiple of
+/* F2 for boxes of F >= kPrep2MinF on inputs whose rows are a multiple of
  * 16 bytes apart (so every row of a box has the same alignment): a
  * workgroup produces kPrep2Cols downscaled pixels of one row.  Phase 1: each
  * thread takes aligned 16-byte chunks of the box's column range and adds the
@@ -377,7 +377,7 @@ __device__ __forceinline__ void fsim_prep_body(const uint8_t* __restrict__ img, 
  * integer sums, and so the planes, are those of fsim_prep_body; LDS holds one
  * row of sums instead of F rows of bytes. */
 constexpr int kPrep2Cols = 256;
-constexpr int kPrep2MinF = 6;
+constexpr int kPrep2MinF = 4; /* measured: F = 8 507 -> 301 us, F = 4 99 -> 93 us per 30-32 pairs */
 __host__ __device__ __forceinline__ int prep2_chunks(int F, int ch, int cols) { return (cols * F * ch) / 16 + 2; }
 static size_t prep2_lds_bytes(int F, int ch, int cols) {
     return (size_t)2 * prep2_chunks(F, ch, cols) * 16 * sizeof(uint16_t);
@@ -761,6 +761,7 @@ using NoiseArgs = FsimNoiseArgs;
  * LDS, so level 3 reads them from there; when more than kSelectCap keys fall
  * in that bucket, level 3 reads the keys from memory again. */
 constexpr int kSelectCap = 16384; /* keys of the level-1 bucket kept in LDS */
+constexpr int kSelectLoads = 8;    /* 16-byte key loads in flight per thread */
 
 __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys, const unsigned* __restrict__ hist,
                                                  float* __restrict__ thr, double* __restrict__ stats, long N,
@@ -806,20 +807,20 @@ __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys,
     const bool vec = (N % 4) == 0;
     const long stride = blockDim.x;
     if (vec) {
-        /* 4 loads in flight per thread */
+        /* kSelectLoads loads in flight per thread */
         const uint4* k4 = (const uint4*)kb;
         const long n4 = N / 4;
-        for (long i0 = 0; i0 < n4; i0 += 4 * stride) {
-            uint4 q[4];
-            bool ok[4];
+        for (long i0 = 0; i0 < n4; i0 += kSelectLoads * stride) {
+            uint4 q[kSelectLoads];
+            bool ok[kSelectLoads];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kSelectLoads; ++u) {
                 const long i = i0 + u * stride + threadIdx.x;
                 ok[u] = i < n4;
                 q[u] = ok[u] ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kSelectLoads; ++u) {
                 level2(q[u].x, ok[u]);
                 level2(q[u].y, ok[u]);
                 level2(q[u].z, ok[u]);
@@ -1151,13 +1152,18 @@ void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stri
     const int cols = prep_cols(F, ch, prb);
     const size_t bytes = prep_lds_bytes(F, ch, cols, prb);
     IQM_CHECK(bytes <= (size_t)kMaxLdsBytes, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
-    /* wide boxes over rows a multiple of 16 bytes apart: the register
+    /* boxes of 4..8 over rows a multiple of 16 bytes apart: the register
      * column-sum kernel (each image's alignment is taken per workgroup) */
-    if (F >= kPrep2MinF && F <= 8 && ((long)w * ch) % 16 == 0 && env_int("IQM_FSIM_PREP2", 1)) {
+    if (F >= env_int("IQM_FSIM_PREP2_MINF", kPrep2MinF) && F <= 8 && ((long)w * ch) % 16 == 0 &&
+        env_int("IQM_FSIM_PREP2", 1)) {
         const int c2 = std::min(kPrep2Cols, wd);
         const dim3 g2(ceil_div(wd, c2), hd, n);
         const size_t b2 = prep2_lds_bytes(F, ch, c2);
         switch (F) {
+            case 2: launch("fsim_prep", fsim_prep2_kernel<2>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            case 3: launch("fsim_prep", fsim_prep2_kernel<3>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            case 4: launch("fsim_prep", fsim_prep2_kernel<4>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            case 5: launch("fsim_prep", fsim_prep2_kernel<5>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
             case 6: launch("fsim_prep", fsim_prep2_kernel<6>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
             case 7: launch("fsim_prep", fsim_prep2_kernel<7>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
             default: launch("fsim_prep", fsim_prep2_kernel<8>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;

@@ -139,11 +139,12 @@ def test_fsim_shapes(ctx, w, h, ch):
     _fsim_case(ctx, img, ref, f"fsim {w}x{h}x{ch}")
 
 
-@pytest.mark.parametrize("w,h,ch", [(3840, 2160, 3), (2560, 1536, 3), (3136, 1792, 4)])
+@pytest.mark.parametrize("w,h,ch", [(3840, 2160, 3), (2560, 1536, 3), (3136, 1792, 4), (1280, 1280, 4)])
 def test_fsim_wide_boxes(ctx, w, h, ch):
-    """Downscale factors 8, 6 and 7 take the register column-sum prep kernel
-    (rows 16-byte aligned); its planes must equal the LDS-staged kernel's bit
-    for bit (same integer box sums), and the result must match the oracle."""
+    """Downscale factors 8, 6, 7 and 5 take the register column-sum prep
+    kernel (rows 16-byte aligned); its planes must equal the LDS-staged
+    kernel's bit for bit (same integer box sums), and the result must match
+    the oracle."""
     img, ref = synth.rgb_pair(w, h, seed=h)
     if ch == 4:
         img, ref = synth.rgba(img), synth.rgba(ref)
@@ -151,6 +152,21 @@ def test_fsim_wide_boxes(ctx, w, h, ch):
     os.environ["IQM_FSIM_PREP2"] = "0"
     try:
         r0 = iqm_amd.FSIM().compute_metric(ctx, img, ref, maps=iqm_amd.FSIM.MAPS, detail=True)
+    finally:
+        del os.environ["IQM_FSIM_PREP2"]
+    for name in ("yiq_in", "yiq_ref"):
+        assert np.array_equal(r.detail[name], r0.detail[name]), name
+    assert r.fsim == r0.fsim and r.fsimc == r0.fsimc
+
+
+def test_fsim_1080p_prep_kernels_agree(ctx):
+    """At 1920x1080 (F = 4) the register column-sum prep and the LDS-staged
+    prep give the same planes bit for bit."""
+    img, ref = synth.rgb_pair(1920, 1080, seed=2)
+    r = iqm_amd.FSIM().compute_metric(ctx, img, ref, detail=True)
+    os.environ["IQM_FSIM_PREP2"] = "0"
+    try:
+        r0 = iqm_amd.FSIM().compute_metric(ctx, img, ref, detail=True)
     finally:
         del os.environ["IQM_FSIM_PREP2"]
     for name in ("yiq_in", "yiq_ref"):
