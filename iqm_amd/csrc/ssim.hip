@@ -19,7 +19,9 @@
 //      a fixed order.
 #include "ctx.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace iqm {
@@ -264,7 +266,7 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
                                              long pair_stride, int w, int h, int ch, const SsimWeights& wt,
                                              float c1, float c2, float* __restrict__ map,
                                              double* __restrict__ partial, int bx, int by, int gx, int gy,
-                                             int pair) {
+                                             int pair, int ty) {
     constexpr int K = 2 * R + 1;
     constexpr int LW = kRwTX + 2 * R; /* window columns (even) */
     extern __shared__ uint4 rw_smem[];
@@ -273,17 +275,17 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
     const uint8_t* A = img + (long)pair * pair_stride;
     const uint8_t* B = ref + (long)pair * pair_stride;
     const int tid = threadIdx.x, wv = tid >> 6;
-    const int x0 = bx * kRwTX, y0 = by * kRwTY;
+    const int x0 = bx * kRwTX, y0 = by * ty;
     const int c0 = max(x0 - R, 0), cend = min(x0 + kRwTX + R, w);
     const int seg = (cend - c0) * ch;
     const int rc = rw_row_chunks(R, ch);
     const int slots = 2 * K * rc;
     float* lum = (float*)(rw_smem + 2 * slots); /* [buf][u, v, u^2, v^2][LW] */
-    const int nwin = kRwTY + 2 * R;
+    const int nwin = ty + 2 * R;
     const int nch = (nwin + K - 1) / K;
 
     /* shift: luma of the tile centre (read by every lane from one address) */
-    const int cx = min(x0 + kRwTX / 2, w - 1), cy = min(y0 + kRwTY / 2, h - 1);
+    const int cx = min(x0 + kRwTX / 2, w - 1), cy = min(y0 + ty / 2, h - 1);
     const float sx = luma_rw(A + ((long)cy * w + cx) * ch, ch);
     const float sy = luma_rw(B + ((long)cy * w + cx) * ch, ch);
 
@@ -466,9 +468,9 @@ __global__ __launch_bounds__(kRwThreads) void ssim_rw_kernel(const uint8_t* __re
                                                              const uint8_t* __restrict__ ref, long pair_stride,
                                                              int w, int h, int ch, SsimWeights wt, float c1,
                                                              float c2, float* __restrict__ map,
-                                                             double* __restrict__ partial) {
+                                                             double* __restrict__ partial, int ty) {
     ssim_rw_body<R>(img, ref, pair_stride, w, h, ch, wt, c1, c2, map, partial, blockIdx.x, blockIdx.y, gridDim.x,
-                    gridDim.y, blockIdx.z);
+                    gridDim.y, blockIdx.z, ty);
 }
 
 __device__ __forceinline__ void ssim_finalize_body(const double* __restrict__ partial, int nblk, double count,
@@ -489,6 +491,11 @@ __global__ __launch_bounds__(kBlock) void ssim_finalize_kernel(const double* __r
                                                                int nblk, double count,
                                                                double* __restrict__ out) {
     ssim_finalize_body(partial, nblk, count, out, blockIdx.x);
+}
+
+static int env_int_ssim(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
 }
 
 /* output tile of the kernel used for radius R */
@@ -527,6 +534,7 @@ double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
     int TX, TY;
     ssim_tile(R, TX, TY);
+    if (R <= kRwMaxRadius) TY = std::max(8, env_int_ssim("IQM_SSIM_TY", TY));
     dim3 grid(ceil_div(w, TX), ceil_div(h, TY), n);
     const int nblk = grid.x * grid.y;
     double* partial = (double*)workspace(WS_SSIM, sizeof(double) * ((size_t)nblk * n + n));
@@ -538,7 +546,7 @@ double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
 #define IQM_SSIM_RW_CASE(r)                                                                                     \
     case r:                                                                                                     \
         launch("ssim", ssim_rw_kernel<r>, grid, dim3(kRwThreads), lds, d_img, d_ref, pair_stride, w, h, ch, wt, \
-               c1, c2, d_maps, partial);                                                                        \
+               c1, c2, d_maps, partial, TY);                                                                    \
         break;
             IQM_SSIM_RW_CASE(1) IQM_SSIM_RW_CASE(2) IQM_SSIM_RW_CASE(3)
             IQM_SSIM_RW_CASE(4) IQM_SSIM_RW_CASE(5) IQM_SSIM_RW_CASE(6)
@@ -594,7 +602,7 @@ __global__ __launch_bounds__(kRwThreads) void mix_ssim_rw_kernel(const MixJob* _
     const MixJob& J = jobs[j];
     const int local = blockIdx.x - st[j];
     ssim_rw_body<R>(J.img, J.ref, 0, J.w, J.h, J.ch, wt, c1, c2, J.ssim_map, J.ssim_partial, local % J.ssim_gx,
-                    local / J.ssim_gx, J.ssim_gx, J.ssim_gy, 0);
+                    local / J.ssim_gx, J.ssim_gx, J.ssim_gy, 0, kRwTY);
 }
 
 /* finalize, then the pair's MSSIM into its row of the batch result */
