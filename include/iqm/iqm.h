@@ -172,6 +172,12 @@ int iqm_sur_state(iqm_sur* s, int64_t* step, uint64_t* data_cursor);
  * FSIM pair; 1/f^2 fields, half of them distorted): d_x gets Y/255 and
  * d_pc the exact PC, n*size*size device floats each. */
 int iqm_sur_make_patches(iqm_sur* s, int32_t n, int32_t size, uint64_t seed, float* d_x, float* d_pc);
+/* The same with the fields made at scale*size and brought to size x size by
+ * FSIM's own box downscale (the distortions are applied at full
+ * resolution), as a large input reaches the network; scale*size must
+ * downscale to size (e.g. 256 x 4). */
+int iqm_sur_make_patches_scaled(iqm_sur* s, int32_t n, int32_t size, int32_t scale, uint64_t seed, float* d_x,
+                                float* d_pc);
 /* Mean squared error over n images of h x w (device) and its gradient. */
 int iqm_sur_loss_grad(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
                       int32_t w, double* loss);

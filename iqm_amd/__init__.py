@@ -35,7 +35,7 @@ EXPORTS = (
     "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_tune_export",
     "iqm_tune_import", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
     "iqm_sur_param_count", "iqm_sur_create", "iqm_sur_destroy", "iqm_sur_get_params",
-    "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches",
+    "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches", "iqm_sur_make_patches_scaled",
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
     "iqm_batch_dev", "iqm_synth_reference_dev",
@@ -139,6 +139,7 @@ def lib():
     L.iqm_sur_get_grad.argtypes = [vp, fp]
     L.iqm_sur_state.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(u64)]
     L.iqm_sur_make_patches.argtypes = [vp, i32, i32, u64, vp, vp]
+    L.iqm_sur_make_patches_scaled.argtypes = [vp, i32, i32, i32, u64, vp, vp]
     L.iqm_sur_loss_grad.argtypes = [vp, vp, vp, i32, i32, i32, dp]
     L.iqm_sur_train_step.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, dp]
     L.iqm_sur_infer.argtypes = [vp, vp, vp, i32, i32, i32, i32]
@@ -460,8 +461,11 @@ class Surrogate:
         _check(lib().iqm_sur_state(self.handle, ctypes.byref(st), ctypes.byref(cur)))
         return st.value, cur.value
 
-    def make_patches(self, n: int, size: int, seed: int, d_x: int, d_pc: int) -> None:
-        _check(lib().iqm_sur_make_patches(self.handle, n, size, int(seed), d_x, d_pc))
+    def make_patches(self, n: int, size: int, seed: int, d_x: int, d_pc: int, scale: int = 1) -> None:
+        if scale == 1:
+            _check(lib().iqm_sur_make_patches(self.handle, n, size, int(seed), d_x, d_pc))
+        else:
+            _check(lib().iqm_sur_make_patches_scaled(self.handle, n, size, scale, int(seed), d_x, d_pc))
 
     def loss_grad(self, d_x: int, d_t: int, n: int, h: int, w: int) -> float:
         loss = np.zeros(1, np.float64)

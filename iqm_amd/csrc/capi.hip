@@ -525,6 +525,16 @@ int iqm_sur_make_patches(iqm_sur* s, int32_t n, int32_t size, uint64_t seed, flo
     });
 }
 
+int iqm_sur_make_patches_scaled(iqm_sur* s, int32_t n, int32_t size, int32_t scale, uint64_t seed, float* d_x,
+                                float* d_pc) {
+    SUR_GUARD(s);
+    return guarded([&] {
+        IQM_CHECK(d_x && d_pc, IQM_ERR_INVALID, "null buffer");
+        IQM_CHECK(scale >= 1 && (long)size * scale <= 8192, IQM_ERR_INVALID, "scaled patch size above 8192");
+        s->s->make_patches(n, size, seed, d_x, d_pc, scale);
+    });
+}
+
 static void sur_check_shape(const float* a, const float* b, int n, int h, int w) {
     IQM_CHECK(a && b, IQM_ERR_INVALID, "null buffer");
     IQM_CHECK(n > 0 && h > 0 && w > 0, IQM_ERR_INVALID, "invalid image count or size");

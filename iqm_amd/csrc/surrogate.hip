@@ -966,16 +966,26 @@ double Surrogate::train_step(const float* d_x, const float* d_target, int nimg, 
     return loss;
 }
 
-void Surrogate::make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc) {
-    IQM_CHECK(n > 0 && n % 2 == 0 && size >= 16, IQM_ERR_INVALID, "patch count must be even and size >= 16");
-    const long per = (long)size * size;
-    const int chunk = std::min(n, 256);
+void Surrogate::make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc, int scale) {
+    IQM_CHECK(n > 0 && n % 2 == 0 && size >= 16 && scale >= 1, IQM_ERR_INVALID,
+              "patch count must be even, size >= 16 and scale >= 1");
+    /* the fields are made at scale * size and FSIM's own box downscale
+     * brings them to size x size, as it does for large inputs */
+    const int full = size * scale;
+    {
+        const int F = std::max(1, (int)std::round(full / 256.0));
+        IQM_CHECK((int)std::round((float)full / (float)F) == size, IQM_ERR_INVALID,
+                  "scale * size does not downscale to size in FSIM");
+    }
+    const long per = (long)full * full;  /* full-resolution pixels per patch */
+    const long pd = (long)size * size;   /* network-grid pixels per patch */
+    const int chunk = std::min(n, std::max(2, (256 / (scale * scale)) & ~1));
     float2* planes = nullptr;
     uint8_t* u8 = nullptr;
     float* pcs = nullptr;
     IQM_HIP(hipMalloc(&planes, sizeof(float2) * per * chunk));
     IQM_HIP(hipMalloc(&u8, per * chunk));
-    IQM_HIP(hipMalloc(&pcs, sizeof(float) * 2 * per * (chunk / 2)));
+    IQM_HIP(hipMalloc(&pcs, sizeof(float) * 2 * pd * (chunk / 2)));
     struct Free {
         void* a; void* b; void* c;
         ~Free() { (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); }
@@ -984,19 +994,19 @@ void Surrogate::make_patches(int n, int size, uint64_t seed, float* d_x, float* 
         const int k = std::min(chunk, n - k0);
         const uint64_t s = seed + (uint64_t)k0 * 1315423911ull;
         c_.launch("sur_noise", sur_noise, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0, planes, per, k, s);
-        c_.fft2d_dev(planes, size, size, k, -1, false);
-        c_.launch("sur_pink", sur_pink, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0, planes, size, k);
-        c_.fft2d_dev(planes, size, size, k, +1, true);
-        c_.launch("sur_to_u8", sur_to_u8, dim3(k), dim3(1024), 0, (const float2*)planes, u8, size, k, s);
+        c_.fft2d_dev(planes, full, full, k, -1, false);
+        c_.launch("sur_pink", sur_pink, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0, planes, full, k);
+        c_.fft2d_dev(planes, full, full, k, +1, true);
+        c_.launch("sur_to_u8", sur_to_u8, dim3(k), dim3(1024), 0, (const float2*)planes, u8, full, k, s);
         /* exact PC targets: patches 2i and 2i+1 as one FSIM pair (gray) */
         FsimBatchOut out;
         out.d_pc = pcs;
-        c_.fsim_batch(u8, u8 + per, size, size, 1, 2 * per, k / 2, out);
-        Ctx::FsimLayout L = c_.fsim_layout(per, k / 2, c_.fsim_ws_base());
-        c_.launch("sur_gather_y", sur_gather_y, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0,
-                  (const float*)L.yiq, per, k / 2, d_x + (long)k0 * per);
-        c_.launch("sur_gather_pc", sur_gather_pc, dim3(ceil_div(per * k, kBlock)), dim3(kBlock), 0,
-                  (const float*)pcs, per, k / 2, d_pc + (long)k0 * per);
+        c_.fsim_batch(u8, u8 + per, full, full, 1, 2 * per, k / 2, out);
+        Ctx::FsimLayout L = c_.fsim_layout(pd, k / 2, c_.fsim_ws_base());
+        c_.launch("sur_gather_y", sur_gather_y, dim3(ceil_div(pd * k, kBlock)), dim3(kBlock), 0,
+                  (const float*)L.yiq, pd, k / 2, d_x + (long)k0 * pd);
+        c_.launch("sur_gather_pc", sur_gather_pc, dim3(ceil_div(pd * k, kBlock)), dim3(kBlock), 0,
+                  (const float*)pcs, pd, k / 2, d_pc + (long)k0 * pd);
         c_.sync();
     }
 }

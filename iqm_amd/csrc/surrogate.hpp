@@ -71,7 +71,7 @@ class Surrogate {
     /* loss and gradient into grad (no update), for gradient checks */
     double loss_and_grad(const float* d_x, const float* d_target, int nimg, int h, int w);
     /* generation of seeded training patches and their exact PC targets */
-    void make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc);
+    void make_patches(int n, int size, uint64_t seed, float* d_x, float* d_pc, int scale = 1);
 
     /* FSIM/FSIMc over n device-resident pairs with phase congruency from the
      * network.  Pairs i with i % spot_every == 0 (spot_every > 0) are also

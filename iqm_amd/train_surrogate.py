@@ -1,7 +1,8 @@
 """Trains the phase-congruency surrogate on the device and evaluates it.
 
 Training data: `--patches` seeded size x size gray 1/f^2 patches generated on
-the device (half distorted: contrast, noise, 16-level quantisation), targets
+the device (half distorted: contrast, noise, 16-level quantisation; with
+`--scale k` made at k*size and box-downscaled by FSIM as large inputs are), targets
 the exact PC maps of the library's FSIM path; Adam on minibatches in a seeded
 order, one checkpoint per epoch (weights, Adam moments, step, cursor), and
 `--resume` continues from a checkpoint.  Evaluation: FSIM/FSIMc of seeded
@@ -35,8 +36,9 @@ def train(ctx, sur, args) -> list:
     px = S * S
     dx, dp = ctx.alloc(4 * P * px), ctx.alloc(4 * P * px)
     t0 = time.perf_counter()
-    sur.make_patches(P, S, args.seed, dx, dp)
-    print(f"[train] {P} patches of {S}x{S} in {time.perf_counter() - t0:.2f} s", flush=True)
+    sur.make_patches(P, S, args.seed, dx, dp, scale=args.scale)
+    print(f"[train] {P} patches of {S}x{S} (made at {S * args.scale}^2) in {time.perf_counter() - t0:.2f} s",
+          flush=True)
     nb = P // B
     step0, _ = sur.state()
     hist = []
@@ -93,6 +95,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--patches", type=int, default=20000)
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--scale", type=int, default=1,
+                    help="make the fields at scale*size and let FSIM downscale them (4: as 1024^2-1080p inputs)")
     ap.add_argument("--seed", type=int, default=40000)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=8)
