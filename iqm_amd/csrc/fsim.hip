@@ -900,11 +900,41 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+/* F17 (fsim_final_multiply.cpp:294-300): fixed-order sum of the block
+ * partials; fsim = sum sim / sum pcm, fsimc = sum simc / sum pcm. */
+__device__ __forceinline__ void fsim_finalize_body(const double* __restrict__ partial, int nblk,
+                                                   double* __restrict__ res, int pair) {
+    __shared__ double sh[3][kBlock];
+    double s[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double* p = partial + ((long)pair * nblk + b) * 4;
+        for (int j = 0; j < 3; ++j) s[j] += p[j];
+    }
+    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = s[j];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* r = res + (long)pair * 8;
+        r[0] = sh[0][0];
+        r[1] = sh[1][0];
+        r[2] = sh[2][0];
+        r[3] = sh[1][0] / sh[0][0];
+        r[4] = sh[2][0] / sh[0][0];
+        r[5] = r[6] = r[7] = 0.0;
+    }
+}
+
 /* F3 (fsim_gradientmap.glsl:38-56) + F15 (fsim_phase_congruency.glsl:40-63)
  * + F16 (fsim_final_multiply.glsl:26-57) + per-block FP64 partials of F17.
- * A workgroup covers a kPcTx x kPcTy tile; the Y planes of both images are
- * staged in LDS with a one-pixel halo for the Scharr stencil. */
-constexpr int kPcTx = 64, kPcTy = 4;
+ * A workgroup covers a kPcTx x kPcTy tile, each thread kPcRows pixels of one
+ * column, 4 rows apart; the Y planes of both images are staged in LDS with a
+ * one-pixel halo for the Scharr stencil. */
+constexpr int kPcTx = 64, kPcTy = 4, kPcRows = kPcTx * kPcTy / kBlock;
+static_assert(kPcRows * kBlock == kPcTx * kPcTy && kBlock / kPcTx == 4, "4 thread rows per tile");
 
 __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq, float* __restrict__ grad,
                                                   const float* __restrict__ an_e, const float* __restrict__ thr,
@@ -918,17 +948,53 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
     const long N = (long)wd * hd;
     const int x0 = bx * kPcTx, y0 = by * kPcTy;
     const float* yq = yiq + (long)pair * 6 * N;
-    for (int i = threadIdx.x; i < 2 * HY * HX; i += blockDim.x) {
+    const int tx = threadIdx.x % kPcTx, ty0 = threadIdx.x / kPcTx;
+    const int x = x0 + tx;
+    /* the halo loads are issued first, then every pixel's own operands; only
+     * the halo values are awaited before the LDS writes and the barrier, so
+     * the other loads stay in flight across it */
+    constexpr int kHalo = 2 * HY * HX, kStage = (kHalo + kBlock - 1) / kBlock;
+    float sv[kStage];
+#pragma unroll
+    for (int st = 0; st < kStage; ++st) {
+        const int i = threadIdx.x + st * kBlock;
         const int im = i / (HY * HX), rem = i - im * HY * HX;
         const int ly = rem / HX, lx = rem - ly * HX;
         const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        ys[im][ly][lx] = (gx >= 0 && gx < wd && gy >= 0 && gy < hd) ? yq[(long)im * N + (long)gy * wd + gx] : 0.f;
+        sv[st] = (i < kHalo && gx >= 0 && gx < wd && gy >= 0 && gy < hd) ? yq[(long)im * N + (long)gy * wd + gx]
+                                                                         : 0.f;
     }
-    __syncthreads();
-    const int tx = threadIdx.x % kPcTx, ty = threadIdx.x / kPcTx;
-    const int x = x0 + tx, y = y0 + ty;
+    float a_n[kPcRows][2], a_e[kPcRows][2][4], iq[kPcRows][4];
+#pragma unroll
+    for (int r = 0; r < kPcRows; ++r) {
+        const int y = y0 + ty0 + 4 * r;
+        if (x < wd && y < hd) {
+            const long idx = (long)y * wd + x;
+            for (int im = 0; im < 2; ++im) {
+                if (pc_src) {
+                    a_n[r][im] = pc_src[((long)pair * 2 + im) * N + idx];
+                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = 0.f;
+                } else {
+                    /* plane 0: sum_o An (added in the order o = 0..3 by the row pass) */
+                    const float* ae = an_e + ((long)pair * 2 + im) * kAnePlanes * N;
+                    a_n[r][im] = ae[idx];
+                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = ae[(1 + o) * N + idx];
+                }
+            }
+            for (int c = 0; c < 4; ++c) iq[r][c] = yq[(2 + c) * N + idx];
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < kStage; ++st) {
+        const int i = threadIdx.x + st * kBlock;
+        if (i < kHalo) (&ys[0][0][0])[i] = sv[st];
+    }
+    lds_barrier();
     double v0 = 0, v1 = 0, v2 = 0;
-    if (x < wd && y < hd) {
+#pragma unroll
+    for (int r = 0; r < kPcRows; ++r) {
+        const int ty = ty0 + 4 * r, y = y0 + ty;
+        if (!(x < wd && y < hd)) continue;
         const long idx = (long)y * wd + x;
         /* Scharr pair as a correlation, x offset j outer, y offset k inner;
          * the staged zeros outside the image are the shader's skipped taps */
@@ -950,14 +1016,12 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
             grad[(long)pair * 2 * N + N + idx] = gm[1];
         }
         float pc[2];
-        for (int im = 0; im < 2 && pc_src; ++im) pc[im] = pc_src[((long)pair * 2 + im) * N + idx];
+        for (int im = 0; im < 2 && pc_src; ++im) pc[im] = a_n[r][im];
         for (int im = 0; im < 2 && !pc_src; ++im) {
-            /* plane 0: sum_o An (added in the order o = 0..3 by the row pass) */
-            const float* ae = an_e + ((long)pair * 2 + im) * kAnePlanes * N;
             const float* T = thr + (long)pair * 8 + im * 4;
             float et = 0.f;
-            for (int o = 0; o < 4; ++o) et += fmaxf(ae[(1 + o) * N + idx] - T[o], 0.0f);
-            pc[im] = et / ae[idx]; /* no epsilon: NaN where sum An = 0 */
+            for (int o = 0; o < 4; ++o) et += fmaxf(a_e[r][im][o] - T[o], 0.0f);
+            pc[im] = et / a_n[r][im]; /* no epsilon: NaN where sum An = 0 */
         }
         const float T1 = 0.85f, T2 = 160.0f, T3 = 200.0f, T4 = 200.0f, lambda = 0.03f;
         const float p1 = pc[0], p2 = pc[1];
@@ -966,14 +1030,14 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
         const float g1 = gm[0], g2 = gm[1];
         const float gsim = (2.0f * g1 * g2 + T2) / (g1 * g1 + g2 * g2 + T2);
         const float sim = gsim * pcsim * pcm;
-        const float i1 = yq[2 * N + idx], q1 = yq[3 * N + idx];
-        const float i2 = yq[4 * N + idx], q2 = yq[5 * N + idx];
+        const float i1 = iq[r][0], q1 = iq[r][1];
+        const float i2 = iq[r][2], q2 = iq[r][3];
         const float isim = (2.0f * i1 * i2 + T3) / (i1 * i1 + i2 * i2 + T3);
         const float qsim = (2.0f * q1 * q2 + T4) / (q1 * q1 + q2 * q2 + T4);
         const float simc = gsim * pcsim * powf(fabsf(isim * qsim), lambda) * pcm;
-        v0 = pcm;
-        v1 = sim;
-        v2 = simc;
+        v0 += (double)pcm;
+        v1 += (double)sim;
+        v2 += (double)simc;
         const long mo = (long)pair * N + idx;
         if (simc_map) simc_map[mo] = simc;
         if (pcm_map) pcm_map[mo] = pcm;
@@ -1012,34 +1076,6 @@ __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
     float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src) {
     fsim_pcfinal_body(yiq, grad, an_e, thr, partial, wd, hd, simc_map, pc_map, pcm_map, sim_map, pc_src,
                       blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, blockIdx.z);
-}
-
-/* F17 (fsim_final_multiply.cpp:294-300): fixed-order sum of the block
- * partials; fsim = sum sim / sum pcm, fsimc = sum simc / sum pcm. */
-__device__ __forceinline__ void fsim_finalize_body(const double* __restrict__ partial, int nblk,
-                                                   double* __restrict__ res, int pair) {
-    __shared__ double sh[3][kBlock];
-    double s[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        const double* p = partial + ((long)pair * nblk + b) * 4;
-        for (int j = 0; j < 3; ++j) s[j] += p[j];
-    }
-    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = s[j];
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double* r = res + (long)pair * 8;
-        r[0] = sh[0][0];
-        r[1] = sh[1][0];
-        r[2] = sh[2][0];
-        r[3] = sh[1][0] / sh[0][0];
-        r[4] = sh[2][0] / sh[0][0];
-        r[5] = r[6] = r[7] = 0.0;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __restrict__ partial,
