@@ -70,6 +70,14 @@ __device__ __forceinline__ float2 nt_load(const float2* p) {
     return make_float2(t.x, t.y);
 }
 
+/* base + i elements with the byte offset formed in 32 bits (i * sizeof(T)
+ * < 4 GiB): a workgroup-uniform base then stays in SGPRs and the access
+ * needs no 64-bit address arithmetic per lane. */
+template <class T>
+__device__ __forceinline__ T* at32(T* base, unsigned i) {
+    return (T*)((char*)base + i * (unsigned)sizeof(T));
+}
+
 constexpr int kMaxLdsBytes = 65536;  // per-workgroup LDS budget used by every kernel
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

@@ -153,7 +153,9 @@ __host__ __device__ __forceinline__ void two_level_step1(float2* x, int j2, cons
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
         float2 v = x[k1];
-        if (k1 != 0 && j2 != 0) {
+        /* j2 = 0 multiplies by tw[0] = (1, -0), which leaves a finite v
+         * unchanged; no branch on the run-time j2 */
+        if (k1 != 0) {
             float2 w = tw[j2 * k1];
             if (S > 0) w.y = -w.y;
             v = cmul(v, w);

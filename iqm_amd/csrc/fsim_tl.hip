@@ -25,6 +25,9 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
     constexpr int ZL = Exch<N1, N2>::line;
     constexpr int LS = n + 1; /* spectrum line pitch (odd) */
     extern __shared__ float2 lds[];
+    /* read once: a read after an LDS barrier's memory clobber would be a
+     * load whose wait (vmcnt) also drains this thread's pending stores */
+    const int nthr = blockDim.x;
     const int cb = blockIdx.x % nbx_pad;
     const int rest = blockIdx.x / nbx_pad;
     const int img = rest & 1, pair = rest >> 1;
@@ -34,17 +37,18 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
     float2* S = lds;
     float2* Z = S + bx * LS;
     float2* tw = Z + bx * ZL;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < n; i += nthr) tw[i] = twg[i];
     const long N = (long)wd * n;
     const float2* src = A + ((long)pair * 2 + img) * N + x0;
     __syncthreads();
     /* forward: step 1, items (c, j2), c fastest */
-    for (int it = threadIdx.x; it < bx * N2; it += blockDim.x) {
+    for (int it = threadIdx.x; it < bx * N2; it += nthr) {
         const int c = it % bx, j2 = it / bx;
         float2 x[N1];
         if (c < nc) {
+            const unsigned o0 = (unsigned)j2 * wd + c, rs = (unsigned)wd * N2;
 #pragma unroll
-            for (int j1 = 0; j1 < N1; ++j1) x[j1] = src[(long)(N2 * j1 + j2) * wd + c];
+            for (int j1 = 0; j1 < N1; ++j1) x[j1] = *at32(src, o0 + j1 * rs);
         } else {
 #pragma unroll
             for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
         two_level_step1<N1, N2, -1>(x, j2, tw, Z + c * ZL);
     }
     lds_barrier();
-    for (int it = threadIdx.x; it < bx * N1; it += blockDim.x) {
+    for (int it = threadIdx.x; it < bx * N1; it += nthr) {
         const int c = it % bx, k1 = it / bx;
         float2 y[N2];
         two_level_step2<N1, N2, -1>(Z + c * ZL, k1, y);
@@ -60,17 +64,23 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
         for (int k2 = 0; k2 < N2; ++k2) S[c * LS + k1 + N1 * k2] = y[k2];
     }
     lds_barrier();
+    /* plane offsets below N = wd * n are 32-bit unsigned, added to
+     * workgroup-uniform base pointers */
+    const unsigned rstride = (unsigned)wd * N2; /* N2 rows */
     for (int z = 0; z < 16; ++z) {
         const float* hz = bank + (long)z * N + x0;
-        for (int it = threadIdx.x; it < bx * N2; it += blockDim.x) {
+        for (int it = threadIdx.x; it < bx * N2; it += nthr) {
             const int c = it % bx, j2 = it / bx;
             float2 x[N1];
             if (c < nc) {
+                const unsigned o0 = (unsigned)j2 * wd + c;
+                /* all N1 filter values are requested before the first is used */
+                float h[N1];
 #pragma unroll
-                for (int j1 = 0; j1 < N1; ++j1) {
-                    const int row = N2 * j1 + j2;
-                    x[j1] = cscale(S[c * LS + row], hz[(long)row * wd + c]);
-                }
+                for (int j1 = 0; j1 < N1; ++j1) h[j1] = *at32(hz, o0 + j1 * rstride);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j1 = 0; j1 < N1; ++j1) x[j1] = cscale(S[c * LS + N2 * j1 + j2], h[j1]);
             } else {
 #pragma unroll
                 for (int j1 = 0; j1 < N1; ++j1) x[j1] = make_float2(0.f, 0.f);
@@ -79,16 +89,16 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
         }
         lds_barrier();
         float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
-        for (int it = threadIdx.x; it < bx * N1; it += blockDim.x) {
+        for (int it = threadIdx.x; it < bx * N1; it += nthr) {
             const int c = it % bx, k1 = it / bx;
             float2 y[N2];
             two_level_step2<N1, N2, 1>(Z + c * ZL, k1, y);
             if (c < nc) {
                 /* streamed once to the row pass: keep it out of the L2 so the
                  * filter bank stays resident */
+                const unsigned o0 = (unsigned)k1 * wd + c, kstride = (unsigned)wd * N1;
 #pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2)
-                    nt_store(&dst[(long)(k1 + N1 * k2) * wd + c], y[k2]);
+                for (int k2 = 0; k2 < N2; ++k2) nt_store(at32(dst, o0 + k2 * kstride), y[k2]);
             }
         }
         lds_barrier(); /* the next filter overwrites Z */
@@ -182,23 +192,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void f
         }
         float* enp = ap + (long)(1 + o) * N;
         float* kp = keys + ((long)pair * 8 + io) * N;
+        const int sstride = rb * LE; /* E lines of one scale */
 #pragma unroll
         for (int q = 0; q < kSlots; ++q) {
             const int idx = threadIdx.x + q * blockDim.x;
             if (idx < nr * n) {
                 const int r = idx / n, xx = idx - r * n;
+                const float2* er = E + r * LE + xx;
                 float2 eo[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) eo[s] = E[(s * rb + r) * LE + xx];
+                for (int s = 0; s < 4; ++s) eo[s] = er[s * sstride];
                 float amp, energy, key;
                 fsim_response_epilogue(eo, invN, epsN, amp, energy, key);
-                const long g = (long)(y0 + r) * n + xx;
+                const unsigned g = (unsigned)(y0 * n + idx); /* < N */
                 /* sum over o, in the order o = 0..3 */
                 if (o == 0) sa[q] = amp;
                 else if (o < 3) sa[q] += amp;
-                else ap[g] = sa[q] + amp;
-                enp[g] = energy;
-                kp[g] = key;
+                else *at32(ap, g) = sa[q] + amp;
+                *at32(enp, g) = energy;
+                *at32(kp, g) = key;
                 atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
             }
         }

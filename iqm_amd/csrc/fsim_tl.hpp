@@ -41,18 +41,21 @@ struct TlGradArgs {
  * divisions; square roots are the hardware's (1 ulp). */
 __device__ __forceinline__ void fsim_response_epilogue(const float2 (&eo)[4], float invN, float epsN, float& amp,
                                                        float& energy, float& key) {
-    float a = 0.f, sr = 0.f, si = 0.f;
+    float a = __builtin_amdgcn_sqrtf(eo[0].x * eo[0].x + eo[0].y * eo[0].y), sr = eo[0].x, si = eo[0].y;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 1; s < 4; ++s) {
         a += __builtin_amdgcn_sqrtf(eo[s].x * eo[s].x + eo[s].y * eo[s].y);
         sr += eo[s].x;
         si += eo[s].y;
     }
     const float rx = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(sr * sr + si * si) + epsN);
     const float me = sr * rx, mo = si * rx;
-    float e = 0.f;
+    /* sum_s (eo_s . m - |eo_s x m|) with the dot products summed as
+     * (sum_s eo_s) . m */
+    float d = fabsf(eo[0].x * mo - eo[0].y * me);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) e += eo[s].x * me + eo[s].y * mo - fabsf(eo[s].x * mo - eo[s].y * me);
+    for (int s = 1; s < 4; ++s) d += fabsf(eo[s].x * mo - eo[s].y * me);
+    const float e = (sr * me + si * mo) - d;
     amp = a * invN;
     energy = e * invN;
     key = (eo[0].x * eo[0].x + eo[0].y * eo[0].y) * (invN * invN);
