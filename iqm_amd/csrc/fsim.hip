@@ -904,26 +904,29 @@ __device__ __forceinline__ double wave_sum(double v) {
  * partials; fsim = sum sim / sum pcm, fsimc = sum simc / sum pcm. */
 __device__ __forceinline__ void fsim_finalize_body(const double* __restrict__ partial, int nblk,
                                                    double* __restrict__ res, int pair) {
-    __shared__ double sh[3][kBlock];
+    /* strided per-thread sums, butterfly sums within each wave, then the
+     * waves in order: a fixed order for a given nblk and block size */
+    __shared__ double sh[3][kBlock / 64];
     double s[3] = {0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double* p = partial + ((long)pair * nblk + b) * 4;
         for (int j = 0; j < 3; ++j) s[j] += p[j];
     }
-    for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] = s[j];
+    for (int j = 0; j < 3; ++j) s[j] = wave_sum(s[j]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int j = 0; j < 3; ++j) sh[j][wid] = s[j];
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int j = 0; j < 3; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + st];
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        double t[3] = {0, 0, 0};
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w)
+            for (int j = 0; j < 3; ++j) t[j] += sh[j][w];
         double* r = res + (long)pair * 8;
-        r[0] = sh[0][0];
-        r[1] = sh[1][0];
-        r[2] = sh[2][0];
-        r[3] = sh[1][0] / sh[0][0];
-        r[4] = sh[2][0] / sh[0][0];
+        r[0] = t[0];
+        r[1] = t[1];
+        r[2] = t[2];
+        r[3] = t[1] / t[0];
+        r[4] = t[2] / t[0];
         r[5] = r[6] = r[7] = 0.0;
     }
 }
@@ -961,27 +964,29 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
         const int im = i / (HY * HX), rem = i - im * HY * HX;
         const int ly = rem / HX, lx = rem - ly * HX;
         const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        sv[st] = (i < kHalo && gx >= 0 && gx < wd && gy >= 0 && gy < hd) ? yq[(long)im * N + (long)gy * wd + gx]
-                                                                         : 0.f;
+        sv[st] = (i < kHalo && gx >= 0 && gx < wd && gy >= 0 && gy < hd)
+                     ? *at32(yq, (unsigned)(im * (int)N + gy * wd + gx))
+                     : 0.f;
     }
     float a_n[kPcRows][2], a_e[kPcRows][2][4], iq[kPcRows][4];
 #pragma unroll
     for (int r = 0; r < kPcRows; ++r) {
         const int y = y0 + ty0 + 4 * r;
         if (x < wd && y < hd) {
-            const long idx = (long)y * wd + x;
+            /* plane offsets (below 6 N) in 32 bits from per-pair bases */
+            const unsigned idx = (unsigned)(y * wd + x), n1 = (unsigned)N;
             for (int im = 0; im < 2; ++im) {
                 if (pc_src) {
-                    a_n[r][im] = pc_src[((long)pair * 2 + im) * N + idx];
+                    a_n[r][im] = *at32(pc_src + (long)pair * 2 * N, im * n1 + idx);
                     for (int o = 0; o < 4; ++o) a_e[r][im][o] = 0.f;
                 } else {
                     /* plane 0: sum_o An (added in the order o = 0..3 by the row pass) */
                     const float* ae = an_e + ((long)pair * 2 + im) * kAnePlanes * N;
-                    a_n[r][im] = ae[idx];
-                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = ae[(1 + o) * N + idx];
+                    a_n[r][im] = *at32(ae, idx);
+                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = *at32(ae, (1 + o) * n1 + idx);
                 }
             }
-            for (int c = 0; c < 4; ++c) iq[r][c] = yq[(2 + c) * N + idx];
+            for (int c = 0; c < 4; ++c) iq[r][c] = *at32(yq, (2 + c) * n1 + idx);
         }
     }
 #pragma unroll
