@@ -124,6 +124,15 @@ int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* f
 int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
                        int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                        float* d_simc_maps, double* fsim_out, double* fsimc_out);
+/* As iqm_fsim_batch_dev, but enqueued on the context's stream without waiting:
+ * d_scores (device) receives n x 2 doubles [fsim, fsimc] when the stream
+ * reaches them (iqm_sync waits).  Successive calls pipeline; the inputs and
+ * d_scores must stay valid until then.  No counterpart in the reference, whose
+ * computeMetric returns after each pair; it is for callers that keep batches
+ * in flight. */
+int iqm_fsim_batch_dev_async(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                             int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                             float* d_simc_maps, double* d_scores);
 
 /* ---- mixed batches ------------------------------------------------------ */
 /* Pairs of uneven sizes, each scored with SSIM and/or FSIM/FSIMc, issued as

@@ -31,6 +31,7 @@ IQM_ERR_NOMEM = 6
 EXPORTS = (
     "iqm_create", "iqm_destroy", "iqm_last_error", "iqm_device_name", "iqm_ssim_default_params",
     "iqm_ssim", "iqm_ssim_batch_dev", "iqm_fsim_dims", "iqm_fsim", "iqm_fsim_batch_dev",
+    "iqm_fsim_batch_dev_async",
     "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
     "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_tune_export",
     "iqm_tune_import", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
@@ -109,6 +110,7 @@ def lib():
     L.iqm_fsim.argtypes = [vp, ctypes.POINTER(Image), ctypes.POINTER(Image), dp, dp,
                            ctypes.POINTER(FsimMaps), ctypes.POINTER(FsimDetail)]
     L.iqm_fsim_batch_dev.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, vp, dp, dp]
+    L.iqm_fsim_batch_dev_async.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, vp, vp]
     L.iqm_fft2d.argtypes = [vp, vp, i32, i32, i32, i32, i32]
     L.iqm_dev_alloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
     L.iqm_dev_free.argtypes = [vp, vp]
@@ -404,6 +406,13 @@ class FSIM:
         _check(lib().iqm_fsim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
                                         d_simc_maps, _dptr(fs), _dptr(fc)))
         return fs, fc
+
+    def compute_batch_dev_async(self, ctx: Context, d_img: int, d_ref: int, w: int, h: int, ch: int,
+                                pair_stride: int, n: int, d_scores: int, d_simc_maps: int | None = None):
+        """Enqueues the batch; d_scores (device, n x 2 doubles [fsim, fsimc]) is
+        filled when the context's stream gets there (ctx.sync() waits)."""
+        _check(lib().iqm_fsim_batch_dev_async(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
+                                              d_simc_maps, d_scores))
 
 
 def _fptr(a):

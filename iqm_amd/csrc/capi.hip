@@ -44,7 +44,27 @@ Ctx::~Ctx() {
         (void)hipEventDestroy(p.second.second);
     }
     for (auto e : free_events_) (void)hipEventDestroy(e);
+    if (pinned_) (void)hipHostFree(pinned_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* Ctx::host_pinned(size_t bytes) {
+    if (pinned_bytes_ < bytes) {
+        if (pinned_) {
+            IQM_HIP(hipStreamSynchronize(stream_));
+            IQM_HIP(hipHostFree(pinned_));
+            pinned_ = nullptr;
+            pinned_bytes_ = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)4096);
+        if (hipHostMalloc(&pinned_, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            pinned_ = nullptr;
+            throw Error(IQM_ERR_NOMEM, "pinned host allocation of " + std::to_string(want) + " bytes failed");
+        }
+        pinned_bytes_ = want;
+    }
+    return pinned_;
 }
 
 void* Ctx::workspace(int slot, size_t bytes) {
@@ -401,14 +421,32 @@ int iqm_fsim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
         out.d_simc = d_simc_maps;
         double* dres = c.fsim_batch_graph(d_img, d_ref, w, h, ch, pair_stride, n, out);
         if (fsim_out || fsimc_out) {
-            std::vector<double> res((size_t)8 * n);
-            IQM_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
+            /* pinned staging: the copy is a direct transfer, not a staged one */
+            double* res = (double*)c.host_pinned(sizeof(double) * 8 * n);
+            IQM_HIP(hipMemcpyAsync(res, dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
             c.sync();
             for (int i = 0; i < n; ++i) {
                 if (fsim_out) fsim_out[i] = res[(size_t)8 * i + 3];
                 if (fsimc_out) fsimc_out[i] = res[(size_t)8 * i + 4];
             }
         }
+    });
+}
+
+int iqm_fsim_batch_dev_async(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                             int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                             float* d_simc_maps, double* d_scores) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_img && d_ref && d_scores && w > 0 && h > 0 && n > 0, IQM_ERR_INVALID,
+                  "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        iqm::Ctx& c = *ctx->c;
+        iqm::FsimBatchOut out;
+        out.d_simc = d_simc_maps;
+        const double* dres = c.fsim_batch_graph(d_img, d_ref, w, h, ch, pair_stride, n, out);
+        c.fsim_scores(dres, n, d_scores);
     });
 }
 

@@ -133,6 +133,9 @@ class Ctx {
 
     /* device workspace slot of at least `bytes` (contents not preserved) */
     void* workspace(int slot, size_t bytes);
+    /* pinned host buffer of at least `bytes` for result copies (contents not
+     * preserved; growing it waits for the stream) */
+    void* host_pinned(size_t bytes);
     void sync();
 
     const LinePlanHost& line_plan(int n);
@@ -146,6 +149,9 @@ class Ctx {
      * [sum pcm, sum sim, sum simc, fsim, fsimc, -, -, -]. */
     double* fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                        long pair_stride, int n, const FsimBatchOut& out);
+    /* [fsim, fsimc] of each pair from a result array of fsim_batch into
+     * d_scores (n x 2 doubles, device), on the stream */
+    void fsim_scores(const double* res, int n, double* d_scores);
     /* F2 of n pairs into the yiq planes; clears the median histograms */
     void launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stride, int w, int h, int ch,
                      int F, int wd, int hd, int n, float* yiq, unsigned* hist);
@@ -247,6 +253,8 @@ class Ctx {
     hipStream_t stream_ = nullptr;
     std::string name_;
     std::map<int, std::pair<void*, size_t>> ws_;
+    void* pinned_ = nullptr; /* hipHostMalloc'd result staging */
+    size_t pinned_bytes_ = 0;
     std::map<int, LinePlanHost> line_plans_;
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
     std::map<std::pair<int, int>, FsimTune> tunes_;

@@ -1088,6 +1088,16 @@ __global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __r
     fsim_finalize_body(partial, nblk, res, blockIdx.x);
 }
 
+/* [fsim, fsimc] of each pair from the n x 8 result array */
+__global__ __launch_bounds__(kBlock) void fsim_scores_kernel(const double* __restrict__ res, int n,
+                                                             double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[2 * i] = res[(long)i * 8 + 3];
+        out[2 * i + 1] = res[(long)i * 8 + 4];
+    }
+}
+
 /* ---- host orchestration ------------------------------------------------- */
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -1315,6 +1325,10 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
            nblk, res);
     return res;
+}
+
+void Ctx::fsim_scores(const double* res, int n, double* d_scores) {
+    launch("fsim_scores", fsim_scores_kernel, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, res, n, d_scores);
 }
 
 double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,

@@ -201,6 +201,41 @@ def test_fsim_batch_dev_matches_single(ctx):
         assert ss[i] == s.mssim
 
 
+def test_fsim_batch_dev_async_matches_blocking(ctx):
+    """The enqueue-only batch call leaves the blocking call's scores in its
+    device array; three calls in flight at once (different batches and
+    sizes, one with maps) each keep their own results."""
+    f = iqm_amd.FSIM()
+    cases = [(640, 360, 3), (512, 512, 2), (640, 360, 1)]
+    bufs, want = [], []
+    try:
+        for k, (w, h, n) in enumerate(cases):
+            pairs = [synth.rgb_pair(w, h, seed=300 + 10 * k + i) for i in range(n)]
+            imgs = np.stack([p[0] for p in pairs])
+            refs = np.stack([p[1] for p in pairs])
+            di, dr, ds = ctx.alloc(imgs.nbytes), ctx.alloc(refs.nbytes), ctx.alloc(n * 16)
+            bufs += [di, dr, ds]
+            ctx.upload(di, imgs)
+            ctx.upload(dr, refs)
+            want.append(f.compute_batch_dev(ctx, di, dr, w, h, 3, w * h * 3, n))
+        _, wd, hd = iqm_amd.fsim_dims(*cases[0][:2])
+        dm = ctx.alloc(cases[0][2] * wd * hd * 4)
+        bufs.append(dm)
+        for k, (w, h, n) in enumerate(cases):
+            di, dr, ds = bufs[3 * k: 3 * k + 3]
+            f.compute_batch_dev_async(ctx, di, dr, w, h, 3, w * h * 3, n, ds, dm if k == 0 else None)
+        ctx.sync()
+        for k, (w, h, n) in enumerate(cases):
+            got = ctx.download(bufs[3 * k + 2], np.empty((n, 2), np.float64))
+            assert np.array_equal(got[:, 0], want[k][0]), k
+            assert np.array_equal(got[:, 1], want[k][1]), k
+        with pytest.raises(iqm_amd.IqmError):
+            f.compute_batch_dev_async(ctx, bufs[0], bufs[1], 640, 360, 3, 640 * 360 * 3, 1, 0)
+    finally:
+        for b in bufs:
+            ctx.free(b)
+
+
 def test_errors(ctx):
     a = np.zeros((10, 12, 3), np.uint8)
     b = np.zeros((10, 13, 3), np.uint8)
