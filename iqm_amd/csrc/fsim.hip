@@ -803,6 +803,46 @@ __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys,
             }
         }
     };
+    /* the same for a round of 4 * kSelectLoads keys per lane: the lane counts
+     * its keys of bucket b1 (adding them to the level-2 histogram), one
+     * wave-wide prefix sum and one reservation place them, then the lane
+     * writes them in order */
+    auto level2_round = [&](const uint4 (&q)[kSelectLoads], const bool (&ok)[kSelectLoads]) {
+        unsigned cnt = 0;
+#pragma unroll
+        for (int v = 0; v < kSelectLoads; ++v) {
+            const unsigned e[4] = {q[v].x, q[v].y, q[v].z, q[v].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool m = ok[v] && ((e[j] >> 21) & 1023u) == b1;
+                if (m) atomicAdd(&h[(e[j] >> 10) & 2047u], 1u);
+                cnt += m ? 1u : 0u;
+            }
+        }
+        unsigned inc = cnt; /* inclusive scan over the wave */
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned t = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += t;
+        }
+        const unsigned total = __shfl(inc, 63, 64);
+        if (total == 0u) return;
+        unsigned base = 0;
+        if (lane == 63) base = atomicAdd(&ncand, total);
+        unsigned at = __shfl(base, 63, 64) + inc - cnt;
+        if (cnt == 0u) return;
+#pragma unroll
+        for (int v = 0; v < kSelectLoads; ++v) {
+            const unsigned e[4] = {q[v].x, q[v].y, q[v].z, q[v].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (ok[v] && ((e[j] >> 21) & 1023u) == b1) {
+                    if (at < (unsigned)kSelectCap) cand[at] = e[j];
+                    ++at;
+                }
+            }
+        }
+    };
     /* keys are read as uint4 when every plane starts 16-byte aligned */
     const bool vec = (N % 4) == 0;
     const long stride = blockDim.x;
@@ -819,13 +859,7 @@ __device__ __forceinline__ void fsim_select_body(const float* __restrict__ keys,
                 ok[u] = i < n4;
                 q[u] = ok[u] ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
             }
-#pragma unroll
-            for (int u = 0; u < kSelectLoads; ++u) {
-                level2(q[u].x, ok[u]);
-                level2(q[u].y, ok[u]);
-                level2(q[u].z, ok[u]);
-                level2(q[u].w, ok[u]);
-            }
+            level2_round(q, ok);
         }
     } else {
         for (long i0 = 0; i0 < N; i0 += stride) {
@@ -1100,6 +1134,10 @@ __global__ __launch_bounds__(kBlock) void fsim_scores_kernel(const double* __res
 
 /* ---- host orchestration ------------------------------------------------- */
 
+/* the kernels address a pair's planes (at most 6 float or 1 float2 plane
+ * apart from their base) with 32-bit byte offsets */
+static bool fsim_planes_fit32(long N) { return N > 0 && N * 8 <= 0xFFFFFFFFL && N * 6 * 4 <= 0xFFFFFFFFL; }
+
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 namespace {
@@ -1243,6 +1281,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const int wd = (int)std::round((float)w / (float)F);
     const int hd = (int)std::round((float)h / (float)F);
     IQM_CHECK(wd >= 2 && hd >= 2, IQM_ERR_TOO_SMALL, "image too small for FSIM");
+    IQM_CHECK(fsim_planes_fit32((long)wd * hd), IQM_ERR_UNSUPPORTED, "downscaled image too large for FSIM");
     const FsimSizePlan& plan = fsim_plan(wd, hd);
     const LinePlan& Px = line_plan(wd).plan;
     const LinePlan& Py = line_plan(hd).plan;
@@ -1337,6 +1376,7 @@ double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, 
     const int wd = (int)std::round((float)w / (float)F);
     const int hd = (int)std::round((float)h / (float)F);
     IQM_CHECK(wd >= 2 && hd >= 2, IQM_ERR_TOO_SMALL, "image too small for FSIM");
+    IQM_CHECK(fsim_planes_fit32((long)wd * hd), IQM_ERR_UNSUPPORTED, "downscaled image too large for FSIM");
     const long N = (long)wd * hd;
     const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);
     size_t off = 0;
@@ -1449,6 +1489,7 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     const int wd = (int)std::round((float)pj.w / (float)F);
     const int hd = (int)std::round((float)pj.h / (float)F);
     IQM_CHECK(F <= kPrepMaxRows, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
+    IQM_CHECK(fsim_planes_fit32((long)wd * hd), IQM_ERR_UNSUPPORTED, "downscaled image too large for FSIM");
     const FsimSizePlan& plan = fsim_plan(wd, hd);
     const LinePlan& Px = line_plan(wd).plan;
     const LinePlan& Py = line_plan(hd).plan;
