@@ -235,13 +235,13 @@ __host__ __device__ __forceinline__ void stockham_stage(const float2* src, float
         float2 x[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) x[t] = s[i + t * T];
-        if (k != 0) {
+        /* k = 0 multiplies by tw[0] = (1, -0), which leaves a finite x[t]
+         * unchanged; no branch on the per-lane k */
 #pragma unroll
-            for (int t = 1; t < R; ++t) {
-                float2 w = tw[t * k * tws];
-                if (S > 0) w.y = -w.y;
-                x[t] = cmul(x[t], w);
-            }
+        for (int t = 1; t < R; ++t) {
+            float2 w = tw[t * k * tws];
+            if (S > 0) w.y = -w.y;
+            x[t] = cmul(x[t], w);
         }
         dft_r<R, S>(x);
         float2* d = dst + line * ld + (i - k) * R + k;

@@ -109,8 +109,9 @@ __device__ __forceinline__ float2* line_fft(float2* b0, float2* b1, int ld, int 
                             const float2* tw) {
     if (!P.blue) return line_direct<S>(b0, b1, ld, nlines, P.n, P.tn1, P.tn2, P.nst, P.rad, tw);
     const int n = P.n, m = P.m;
+    const float im_ = 1.0f / (float)m, in_ = 1.0f / (float)n;
     for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-        const int l = idx / m, j = idx - l * m;
+        const int l = fast_div(idx, m, im_), j = idx - l * m;
         float2 v = make_float2(0.f, 0.f);
         if (j < n) {
             float2 c = P.chirp[j];
@@ -124,14 +125,14 @@ __device__ __forceinline__ float2* line_fft(float2* b0, float2* b1, int ld, int 
     float2* other = (r == b0) ? b1 : b0;
     const float2* g = (S < 0) ? P.gf : P.gi;
     for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-        const int l = idx / m, j = idx - l * m;
+        const int l = fast_div(idx, m, im_), j = idx - l * m;
         r[l * ld + j] = cmul(r[l * ld + j], g[j]);
     }
     lds_barrier();
     float2* q = line_direct<1>(r, other, ld, nlines, m, P.tm1, P.tm2, P.mst, P.mrad, P.mtw);
     const float inv_m = 1.0f / (float)m;
     for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-        const int l = idx / n, k = idx - l * n;
+        const int l = fast_div(idx, n, in_), k = idx - l * n;
         float2 c = P.chirp[k];
         if (S > 0) c.y = -c.y;
         q[l * ld + k] = cscale(cmul(q[l * ld + k], c), inv_m);

@@ -556,26 +556,29 @@ __device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, 
     const float2* tw = stage_twiddles(P, lds + 3 * bx * ld);
     const long N = (long)wd * hd;
     const int nc = min(bx, wd - x0);
-    const float2* src = A + ((long)pair * 2 + img) * N;
+    const float inc = 1.0f / (float)nc;
+    /* plane offsets y * wd + c (< N) in 32 bits from workgroup-uniform bases */
+    const float2* src = A + ((long)pair * 2 + img) * N + x0;
     for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
-        const int y = idx / nc, c = idx - y * nc;
-        b0[c * ld + y] = src[(long)y * wd + x0 + c];
+        const int y = fast_div(idx, nc, inc), c = idx - y * nc;
+        b0[c * ld + y] = *at32(src, (unsigned)(y * wd + c));
     }
     __syncthreads();
     float2* Fs = line_fft<-1>(b0, b1, ld, nc, P, tw);
     float2* W = (Fs == b0) ? b1 : b0;
     for (int z = 0; z < 16; ++z) {
-        const float* hz = bank + (long)z * N;
+        const float* hz = bank + (long)z * N + x0;
         for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
-            const int y = idx / nc, c = idx - y * nc;
-            W[c * ld + y] = cscale(Fs[c * ld + y], hz[(long)y * wd + x0 + c]);
+            const int y = fast_div(idx, nc, inc), c = idx - y * nc;
+            W[c * ld + y] = cscale(Fs[c * ld + y], *at32(hz, (unsigned)(y * wd + c)));
         }
         lds_barrier();
         float2* R = line_fft<1>(W, b2, ld, nc, P, tw);
-        float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N;
+        float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
         for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
-            const int y = idx / nc, c = idx - y * nc;
-            dst[(long)y * wd + x0 + c] = R[c * ld + y];
+            const int y = fast_div(idx, nc, inc), c = idx - y * nc;
+            /* streamed once to the row pass: kept out of the L2 */
+            nt_store(at32(dst, (unsigned)(y * wd + c)), R[c * ld + y]);
         }
         lds_barrier(); /* the next filter overwrites W and b2 */
     }
@@ -629,15 +632,19 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
     const int tot = nl * wd;
     const bool pf = tot <= kRowPrefetch * (int)blockDim.x;
     float2 pre[kRowPrefetch];
+    const float iwd = 1.0f / (float)wd, irb = 1.0f / (float)rb;
+    /* row offsets (y0 + r) * wd + x (< N) in 32 bits from per-plane bases */
     auto fetch = [&](int o) {
 #pragma unroll
         for (int q = 0; q < kRowPrefetch; ++q) {
             const int idx = threadIdx.x + q * blockDim.x;
             pre[q] = make_float2(0.f, 0.f);
             if (idx < tot) {
-                const int l = idx / wd, x = idx - l * wd;
-                const int s = l / rb, r = l - s * rb;
-                if (r < nr) pre[q] = Tin[((long)pair * 32 + (img * 4 + o) * 4 + s) * N + (long)(y0 + r) * wd + x];
+                const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
+                const int s = fast_div(l, rb, irb), r = l - s * rb;
+                if (r < nr)
+                    pre[q] = nt_load(at32(Tin + ((long)pair * 32 + (img * 4 + o) * 4 + s) * N,
+                                          (unsigned)((y0 + r) * wd + x)));
             }
         }
     };
@@ -649,17 +656,18 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
             for (int q = 0; q < kRowPrefetch; ++q) {
                 const int idx = threadIdx.x + q * blockDim.x;
                 if (idx < tot) {
-                    const int l = idx / wd, x = idx - l * wd;
+                    const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
                     b0[l * ld + x] = pre[q];
                 }
             }
             if (o < 3) fetch(o + 1);
         } else {
             for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-                const int l = idx / wd, x = idx - l * wd;
-                const int s = l / rb, r = l - s * rb;
+                const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
+                const int s = fast_div(l, rb, irb), r = l - s * rb;
                 float2 v = make_float2(0.f, 0.f);
-                if (r < nr) v = Tin[((long)pair * 32 + io * 4 + s) * N + (long)(y0 + r) * wd + x];
+                if (r < nr)
+                    v = nt_load(at32(Tin + ((long)pair * 32 + io * 4 + s) * N, (unsigned)((y0 + r) * wd + x)));
                 b0[l * ld + x] = v;
             }
         }
@@ -670,20 +678,22 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
         lds_barrier();
         float* enp = ap + (long)(1 + o) * N;
         float* kp = keys + ((long)pair * 8 + io) * N;
+        const int sstride = rb * ld; /* lines of one scale */
         for (int idx = threadIdx.x; idx < nr * wd; idx += blockDim.x) {
-            const int r = idx / wd, x = idx - r * wd;
+            const int r = fast_div(idx, wd, iwd), x = idx - r * wd;
+            const float2* er = R + r * ld + x;
             float2 eo[4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) eo[s] = R[(s * rb + r) * ld + x];
+            for (int s = 0; s < 4; ++s) eo[s] = er[s * sstride];
             float amp, energy, key;
             fsim_response_epilogue(eo, invN, epsN, amp, energy, key);
-            const long g = (long)(y0 + r) * wd + x;
+            const unsigned g = (unsigned)(y0 * wd + idx); /* < N */
             /* sum over o, in the order o = 0..3 (each idx stays with one thread) */
             if (o == 0) sa[idx] = amp;
             else if (o < 3) sa[idx] += amp;
-            else ap[g] = sa[idx] + amp;
-            enp[g] = energy;
-            kp[g] = key;
+            else *at32(ap, g) = sa[idx] + amp;
+            *at32(enp, g) = energy;
+            *at32(kp, g) = key;
             atomicAdd(&hl[(__float_as_uint(key) >> 21) & 1023u], 1u);
         }
         lds_barrier(); /* histogram complete; line buffers free for the next orientation */
