@@ -466,8 +466,8 @@ def main() -> int:
     ap.add_argument("--workload", default="fsimc_1080p", choices=["fsimc_1080p", "mixed", "stream", "ssim"])
     ap.add_argument("--stream-frames", type=int, default=600)
     ap.add_argument("--mixed-pairs", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--batch", type=int, default=32, help="pairs per step")
     ap.add_argument("--unique", type=int, default=4, help="distinct seeded pairs cycled in the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

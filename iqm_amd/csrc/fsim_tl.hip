@@ -135,7 +135,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void f
     float2* E = lds; /* aliases Z, see step 2 */
     float2* tw = Z + nl * ZL;
     unsigned* hl = (unsigned*)(tw + n);
-    const int img = blockIdx.y, pair = blockIdx.z;
+    /* pairs newest first: the last pairs the column pass wrote are read while
+     * they may still be in the memory-side cache, and the PC kernel, which
+     * walks pairs oldest first, then finds this pass's last outputs there */
+    const int img = blockIdx.y, pair = gridDim.z - 1 - blockIdx.z;
     const long N = (long)n * hd;
     const int y0 = blockIdx.x * rb;
     const int nr = min(rb, hd - y0);
