@@ -174,6 +174,31 @@ def test_fsim_1080p_prep_kernels_agree(ctx):
     assert r.fsim == r0.fsim and r.fsimc == r0.fsimc
 
 
+def test_fsim_select_crowded_bucket(ctx):
+    """A vertical grating with the scale-0 period (6 px) on a width holding
+    whole periods gives orientation 0 the same response at every pixel, so
+    nearly every median key of that orientation falls in one level-1 bucket:
+    more keys than the select keeps in LDS, and its last level reads the keys
+    from memory again.  The medians (where they are not rounding noise) must
+    still be the oracle's, and so must the scores and maps."""
+    w, h = 258, 256
+    rng = np.random.default_rng(7)
+    row = np.round(128 + 60 * np.sin(2 * np.pi * np.arange(w) / 6.0)).astype(np.uint8)
+    img = np.repeat(row[None, :], h, axis=0)
+    ref = np.clip(img.astype(np.int32) + rng.integers(-3, 4, img.shape), 0, 255).astype(np.uint8)
+    r, o = _fsim_case(ctx, img, ref, "fsim grating 258x256", stages=True)
+    keys = np.asarray(r.detail["keys"], np.float32).reshape(8, -1)
+    med = np.asarray(r.detail["median"], np.float64)
+    crowded = 0
+    for s in range(8):
+        b = (np.float32(med[s]).view(np.uint32) >> 21) & 1023
+        crowded = max(crowded, int(np.sum(((keys[s].view(np.uint32) >> 21) & 1023) == b)))
+    _report("grating crowded bucket", keys=float(crowded))
+    assert crowded > 16384, crowded
+    big = o.median > 1e-6 * np.max(o.median)
+    np.testing.assert_allclose(med[big], o.median[big], rtol=1e-4)
+
+
 def test_fsim_identity(ctx):
     img, ref = synth.rgb_pair(400, 300, seed=5)
     r = iqm_amd.FSIM().compute_metric(ctx, ref, ref)
