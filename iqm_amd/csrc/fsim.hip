@@ -975,6 +975,15 @@ __device__ __forceinline__ void fsim_finalize_body(const double* __restrict__ pa
     }
 }
 
+/* x^y for 0 <= x <= 1 as exp2(y log2 x) on the hardware log and exp (about 1
+ * ulp each, as a shader's pow); a denormal x is scaled into the normal range
+ * first */
+__device__ __forceinline__ float pow_unit(float x, float y) {
+    const bool tiny = x < 1.17549435e-38f;
+    const float l = __builtin_amdgcn_logf(tiny ? x * 18446744073709551616.0f : x) - (tiny ? 64.0f : 0.0f);
+    return __builtin_amdgcn_exp2f(y * l);
+}
+
 /* F3 (fsim_gradientmap.glsl:38-56) + F15 (fsim_phase_congruency.glsl:40-63)
  * + F16 (fsim_final_multiply.glsl:26-57) + per-block FP64 partials of F17.
  * A workgroup covers a kPcTx x kPcTy tile, each thread kPcRows pixels of one
@@ -1058,7 +1067,7 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
                 }
             }
             const float gv = vs / 16.0f, gh = hs / 16.0f;
-            gm[im] = sqrtf(gv * gv + gh * gh);
+            gm[im] = __builtin_amdgcn_sqrtf(gv * gv + gh * gh);
         }
         if (grad) {
             grad[(long)pair * 2 * N + idx] = gm[0];
@@ -1070,20 +1079,22 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
             const float* T = thr + (long)pair * 8 + im * 4;
             float et = 0.f;
             for (int o = 0; o < 4; ++o) et += fmaxf(a_e[r][im][o] - T[o], 0.0f);
-            pc[im] = et / a_n[r][im]; /* no epsilon: NaN where sum An = 0 */
+            pc[im] = et * __builtin_amdgcn_rcpf(a_n[r][im]); /* no epsilon: NaN where sum An = 0 */
         }
         const float T1 = 0.85f, T2 = 160.0f, T3 = 200.0f, T4 = 200.0f, lambda = 0.03f;
         const float p1 = pc[0], p2 = pc[1];
         const float pcm = (p1 < p2) ? p2 : p1;
-        const float pcsim = (2.0f * p1 * p2 + T1) / (p1 * p1 + p2 * p2 + T1);
+        /* the ratios with the hardware reciprocal (1 ulp), as a shader's divide */
+        const float pcsim = (2.0f * p1 * p2 + T1) * __builtin_amdgcn_rcpf(p1 * p1 + p2 * p2 + T1);
         const float g1 = gm[0], g2 = gm[1];
-        const float gsim = (2.0f * g1 * g2 + T2) / (g1 * g1 + g2 * g2 + T2);
+        const float gsim = (2.0f * g1 * g2 + T2) * __builtin_amdgcn_rcpf(g1 * g1 + g2 * g2 + T2);
         const float sim = gsim * pcsim * pcm;
         const float i1 = iq[r][0], q1 = iq[r][1];
         const float i2 = iq[r][2], q2 = iq[r][3];
-        const float isim = (2.0f * i1 * i2 + T3) / (i1 * i1 + i2 * i2 + T3);
-        const float qsim = (2.0f * q1 * q2 + T4) / (q1 * q1 + q2 * q2 + T4);
-        const float simc = gsim * pcsim * powf(fabsf(isim * qsim), lambda) * pcm;
+        const float isim = (2.0f * i1 * i2 + T3) * __builtin_amdgcn_rcpf(i1 * i1 + i2 * i2 + T3);
+        const float qsim = (2.0f * q1 * q2 + T4) * __builtin_amdgcn_rcpf(q1 * q1 + q2 * q2 + T4);
+        /* |isim qsim| <= 1 */
+        const float simc = gsim * pcsim * pow_unit(fabsf(isim * qsim), lambda) * pcm;
         v0 += (double)pcm;
         v1 += (double)sim;
         v2 += (double)simc;
