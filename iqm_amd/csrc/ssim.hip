@@ -327,6 +327,20 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
 #pragma unroll
     for (int d = 0; d <= 2 * R; ++d) gfull += wt.g[d];
     const float rgfull = 1.0f / gfull;
+    /* reciprocal in-bounds weight mass of the first R and the last R image
+     * rows (entry k: row k, entry R + k: row h - R + k), read by the rows
+     * that need it; interior rows use rgfull */
+    __shared__ float redge[2 * kRwMaxRadius];
+    if (tid < 2 * R) {
+        const int yk = tid < R ? tid : h - 2 * R + tid;
+        float gys = 0.f;
+#pragma unroll
+        for (int d = 0; d <= 2 * R; ++d) {
+            const int yy = yk + d - R;
+            if (yy >= 0 && yy < h) gys += wt.g[d];
+        }
+        redge[tid] = 1.0f / gys; /* only rows inside the image are read */
+    }
 
     float ring[K][2][4];
     double crop = 0.0;
@@ -407,15 +421,8 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
                     const int y = y0 + i - 2 * R;
                     if (y < h) {
                         float rgy = rgfull;
-                        if (y < R || y + R >= h) {
-                            float gys = 0.f;
-#pragma unroll
-                            for (int d = 0; d <= 2 * R; ++d) {
-                                const int yy = y + d - R;
-                                if (yy >= 0 && yy < h) gys += wt.g[d];
-                            }
-                            rgy = 1.0f / gys;
-                        }
+                        if (y < R) rgy = redge[y];
+                        else if (y + R >= h) rgy = redge[y - h + 2 * R];
                         float out[2];
 #pragma unroll
                         for (int col = 0; col < 2; ++col) {
@@ -441,8 +448,9 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
                         for (int col = 0; col < 2; ++col) {
                             if (x + col < w) {
                                 if (mrow) mrow[x + col] = out[col];
-                                if (x + col >= R && x + col <= w - R && y >= R && y <= h - R)
-                                    crop += (double)out[col];
+                                /* adding +0 outside the crop leaves the sum unchanged */
+                                const bool incrop = x + col >= R && x + col <= w - R && y >= R && y <= h - R;
+                                crop += (double)(incrop ? out[col] : 0.0f);
                             }
                         }
                     }
