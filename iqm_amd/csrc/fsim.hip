@@ -540,6 +540,19 @@ __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __rest
  * The flat grid is laid out (pair, img, padded column block) with the
  * column-block count padded to a multiple of 8, so column block cb always
  * runs on XCD cb mod 8 and each XCD's L2 holds only its slice of the bank. */
+/* element y of column line c: spectrum times filter; output y to the plane */
+struct ColFilterLoad {
+    const float2* fs; /* the column's spectrum in LDS */
+    const float* h;   /* the filter plane at the column */
+    int wd;
+    __device__ float2 operator()(int y) const { return cscale(fs[y], *at32(h, (unsigned)(y * wd))); }
+};
+struct ColStreamStore {
+    float2* d; /* the response plane at the column */
+    int wd;
+    __device__ void operator()(int y, float2 v) const { nt_store(at32(d, (unsigned)(y * wd)), v); }
+};
+
 __device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, float2* __restrict__ Tout,
                                                   const float* __restrict__ bank, int wd, int hd, int bx,
                                                   int nbx_pad, const LinePlan& P, int blk) {
@@ -568,19 +581,11 @@ __device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, 
     float2* W = (Fs == b0) ? b1 : b0;
     for (int z = 0; z < 16; ++z) {
         const float* hz = bank + (long)z * N + x0;
-        for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
-            const int y = fast_div(idx, nc, inc), c = idx - y * nc;
-            W[c * ld + y] = cscale(Fs[c * ld + y], *at32(hz, (unsigned)(y * wd + c)));
-        }
-        lds_barrier();
-        float2* R = line_fft<1>(W, b2, ld, nc, P, tw);
         float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
-        for (int idx = threadIdx.x; idx < nc * hd; idx += blockDim.x) {
-            const int y = fast_div(idx, nc, inc), c = idx - y * nc;
-            /* streamed once to the row pass: kept out of the L2 */
-            nt_store(at32(dst, (unsigned)(y * wd + c)), R[c * ld + y]);
-        }
-        lds_barrier(); /* the next filter overwrites W and b2 */
+        /* the inverse transform reads H_z * F straight from the spectrum and
+         * the bank, and streams its outputs to the row pass (out of the L2) */
+        line_fft_io<1>([&](int c) { return ColFilterLoad{Fs + c * ld, hz + c, wd}; },
+                       [&](int c) { return ColStreamStore{dst + c, wd}; }, W, b2, ld, nc, P, tw);
     }
 }
 
