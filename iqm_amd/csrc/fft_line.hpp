@@ -175,38 +175,55 @@ __device__ __forceinline__ float2* line_fft(float2* b0, float2* b1, int ld, int 
  * step-2 outputs through them (no input or output pass over LDS), Stockham
  * and Bluestein lines materialise the input in b0 first (Bluestein folds
  * the load into its first chirp product and the store into its last).
- * Work items take the line index fastest, so that lines which are columns
- * of a row-major plane are read and written coalesced.  b0 and b1 are line
+ * LineFast: work items take the line index fastest, so that lines which
+ * are columns of a row-major plane are read and written coalesced; else the
+ * element index is fastest (lines that are rows).  b0 and b1 are line
  * buffers of pitch ld (b0 unused by the two-level transform, whose exchange
  * area is b1).  Starts after the caller's barrier and ends with a barrier. */
-template <int S, class MakeLoad, class MakeStore>
+template <int S, bool LineFast, class MakeLoad, class MakeStore>
 __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStore& store, float2* b0, float2* b1,
                                             int ld, int nlines, const LinePlan& P, const float2* tw) {
     const int n = P.n;
     const float inl = 1.0f / (float)nlines;
+    /* work item it of a pass over nlines lines of len elements -> (line, element) */
+    auto split = [&](int it, int len, float ilen, int& l, int& e) {
+        if (LineFast) {
+            e = fast_div(it, nlines, inl);
+            l = it - e * nlines;
+        } else {
+            l = fast_div(it, len, ilen);
+            e = it - l * len;
+        }
+    };
     if (!P.blue && P.tn1 > 0) {
         const int n1 = P.tn1, n2 = P.tn2;
+        const float i1 = 1.0f / (float)n1, i2 = 1.0f / (float)n2;
         for (int it = threadIdx.x; it < nlines * n2; it += blockDim.x) {
-            const int j2 = fast_div(it, nlines, inl), l = it - j2 * nlines;
+            int l, j2;
+            split(it, n2, i2, l, j2);
             tl2_item1_f<S>(n1, load(l), j2, n2, tw, b1 + l * ld);
         }
         lds_barrier();
         for (int it = threadIdx.x; it < nlines * n1; it += blockDim.x) {
-            const int k1 = fast_div(it, nlines, inl), l = it - k1 * nlines;
+            int l, k1;
+            split(it, n1, i1, l, k1);
             tl2_item2_f<S>(n2, b1 + l * ld, k1, n1, store(l));
         }
         lds_barrier();
         return;
     }
+    const float in_ = 1.0f / (float)n;
     if (!P.blue) {
         for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-            const int j = fast_div(idx, nlines, inl), l = idx - j * nlines;
+            int l, j;
+            split(idx, n, in_, l, j);
             b0[l * ld + j] = load(l)(j);
         }
         lds_barrier();
         const float2* r = lds_fft_direct<S>(b0, b1, ld, nlines, n, P.nst, P.rad, tw);
         for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-            const int k = fast_div(idx, nlines, inl), l = idx - k * nlines;
+            int l, k;
+            split(idx, n, in_, l, k);
             store(l)(k, r[l * ld + k]);
         }
         lds_barrier();
@@ -215,7 +232,8 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
     const int m = P.m;
     const float im_ = 1.0f / (float)m;
     for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-        const int j = fast_div(idx, nlines, inl), l = idx - j * nlines;
+        int l, j;
+        split(idx, m, im_, l, j);
         float2 v = make_float2(0.f, 0.f);
         if (j < n) {
             float2 c = P.chirp[j];
@@ -236,7 +254,8 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
     const float2* q = line_direct<1>(r, other, ld, nlines, m, P.tm1, P.tm2, P.mst, P.mrad, P.mtw);
     const float inv_m = 1.0f / (float)m;
     for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-        const int k = fast_div(idx, nlines, inl), l = idx - k * nlines;
+        int l, k;
+        split(idx, n, in_, l, k);
         float2 c = P.chirp[k];
         if (S > 0) c.y = -c.y;
         store(l)(k, cscale(cmul(q[l * ld + k], c), inv_m));

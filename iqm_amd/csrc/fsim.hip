@@ -584,7 +584,7 @@ __device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, 
         float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
         /* the inverse transform reads H_z * F straight from the spectrum and
          * the bank, and streams its outputs to the row pass (out of the L2) */
-        line_fft_io<1>([&](int c) { return ColFilterLoad{Fs + c * ld, hz + c, wd}; },
+        line_fft_io<1, true>([&](int c) { return ColFilterLoad{Fs + c * ld, hz + c, wd}; },
                        [&](int c) { return ColStreamStore{dst + c, wd}; }, W, b2, ld, nc, P, tw);
     }
 }
