@@ -249,6 +249,9 @@ static size_t rw_lds_bytes(int R, int ch) {
     return (size_t)2 * 2 * (2 * R + 1) * rw_row_chunks(R, ch) * 16 + (size_t)2 * 4 * (kRwTX + 2 * R) * sizeof(float);
 }
 
+__device__ __forceinline__ float luma_rgb(float r, float g, float b) {
+    return (0.299f * r + 0.581f * g + 0.114f * b) * (1.0f / 255.0f);
+}
 __device__ __forceinline__ float luma_rw(const uint8_t* p, int ch) {
     float r, g, b;
     if (ch == 1) {
@@ -258,7 +261,7 @@ __device__ __forceinline__ float luma_rw(const uint8_t* p, int ch) {
         g = (float)p[1];
         b = (float)p[2];
     }
-    return (0.299f * r + 0.581f * g + 0.114f * b) * (1.0f / 255.0f);
+    return luma_rgb(r, g, b);
 }
 
 template <int R>
@@ -369,16 +372,48 @@ __device__ __forceinline__ void ssim_rw_body(const uint8_t* __restrict__ img, co
                     const int j = q == 0 ? 2 * tid : kRwTX + 2 * tid;
                     if (q == 0 || tid < R) {
                         float uv[2], vv[2];
+                        const int gx0 = x0 - R + j;
+                        if (ch >= 3 && rowin && gx0 >= 0 && gx0 + 1 < w) {
+                            /* both pixels inside: their 6 or 8 bytes come
+                             * from 3 aligned dwords, picked by alignbyte; the
+                             * third dword may reach past the row's staged
+                             * blocks into the next slot or the luma rows
+                             * after the stage buffers (inside the LDS
+                             * allocation), and its bytes past the pixels are
+                             * never used */
+                            const int bo = (gx0 - c0) * ch;
+                            float lm[2][2];
 #pragma unroll
-                        for (int e = 0; e < 2; ++e) {
-                            const int gxx = x0 - R + j + e;
-                            uv[e] = vv[e] = 0.f;
-                            if (rowin && gxx >= 0 && gxx < w) {
-                                const int bo = (gxx - c0) * ch;
-                                const float a = luma_rw(sb + off[0] + bo, ch) - sx;
-                                const float b = luma_rw(sb + off[1] + bo, ch) - sy;
+                            for (int im = 0; im < 2; ++im) {
+                                const int ob = off[im] + bo;
+                                const unsigned* wv = (const unsigned*)(sb + (ob & ~3));
+                                const unsigned w0 = wv[0], w1 = wv[1], w2 = wv[2];
+                                const unsigned d0 = __builtin_amdgcn_alignbyte(w1, w0, (unsigned)(ob & 3));
+                                const unsigned d1 = __builtin_amdgcn_alignbyte(w2, w1, (unsigned)(ob & 3));
+                                const unsigned p1 = ch == 3 ? ((d0 >> 24) | (d1 << 8)) : d1;
+                                lm[im][0] = luma_rgb((float)(d0 & 0xffu), (float)((d0 >> 8) & 0xffu),
+                                                     (float)((d0 >> 16) & 0xffu));
+                                lm[im][1] = luma_rgb((float)(p1 & 0xffu), (float)((p1 >> 8) & 0xffu),
+                                                     (float)((p1 >> 16) & 0xffu));
+                            }
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) {
+                                const float a = lm[0][e] - sx, b = lm[1][e] - sy;
                                 uv[e] = a + b;
                                 vv[e] = a - b;
+                            }
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) {
+                                const int gxx = gx0 + e;
+                                uv[e] = vv[e] = 0.f;
+                                if (rowin && gxx >= 0 && gxx < w) {
+                                    const int bo = (gxx - c0) * ch;
+                                    const float a = luma_rw(sb + off[0] + bo, ch) - sx;
+                                    const float b = luma_rw(sb + off[1] + bo, ch) - sy;
+                                    uv[e] = a + b;
+                                    vv[e] = a - b;
+                                }
                             }
                         }
                         *(float2*)(lq + j) = make_float2(uv[0], uv[1]);
