@@ -19,6 +19,7 @@ namespace iqm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 /* FP32 -> BF16 bits, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950) */
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
@@ -245,20 +246,23 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
     for (int t = 0; t < 9; ++t)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
-    const float bias0 = params[SurLayout::bl(ml) + li], bias1 = params[SurLayout::bl(ml) + 16 + li];
+    /* biases of the channels this lane's outputs hold: 16 nb + 4 g + e */
+    float bias[2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bias[nb][e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
     /* layer 1: thread = (channel l1co, pixels 5 l1pg ..); layer 6: thread =
      * (pixel tid / 8, channels 4 (tid % 8) ..); their BF16-rounded weights */
-    /* which of this lane's 12 mid-layer outputs lie inside the layer's columns
-     * and the image (bit e: tile 2h, 4 + e: tile 2h+1, 8 + e: tile 4) */
+    /* whether this lane's pixel of each of its mid-layer tiles lies inside the
+     * layer's columns and the image (bit 0: tile 2h, 1: tile 2h+1, 2: tile 4) */
     unsigned inmask = 0;
     {
         const int wl_cols = kFuSW + 2 * (6 - ml);
-        for (int e = 0; e < 4; ++e) {
-            const int ps[3] = {2 * half * 16 + 4 * g + e, (2 * half + 1) * 16 + 4 * g + e, 64 + 4 * g + e};
-            for (int k = 0; k < 3; ++k) {
-                const int gx = x0 - (6 - ml) + ps[k];
-                if (ps[k] < wl_cols && gx >= 0 && gx < w) inmask |= 1u << (4 * k + e);
-            }
+        const int ps[3] = {2 * half * 16 + li, (2 * half + 1) * 16 + li, 64 + li};
+        for (int k = 0; k < 3; ++k) {
+            const int gx = x0 - (6 - ml) + ps[k];
+            if (ps[k] < wl_cols && gx >= 0 && gx < w) inmask |= 1u << k;
         }
     }
     const int l1co = tid & 31, l1pg = tid >> 5;
@@ -266,11 +270,16 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 #pragma unroll
     for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
     const float b1r = params[SurLayout::b1 + l1co];
-    float w6r[4][9];
+    /* layer 6 weights of channel pairs (4 cg, 4 cg + 1) and (4 cg + 2, 4 cg + 3),
+     * packed as BF16 pairs for the two-term dot products */
+    bf16x2 w6p[2][9];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 2; ++k)
 #pragma unroll
-        for (int t = 0; t < 9; ++t) w6r[k][t] = bf2f(f2bf(params[SurLayout::w6 + (4 * (tid & 7) + k) * 9 + t]));
+        for (int t = 0; t < 9; ++t) {
+            const int c = 4 * (tid & 7) + 2 * k;
+            w6p[k][t] = bf16x2{(__bf16)params[SurLayout::w6 + c * 9 + t], (__bf16)params[SurLayout::w6 + (c + 1) * 9 + t]};
+        }
     __syncthreads();
     /* input rows 0 and 1 (row -1 stays zero) */
     for (int i = tid; i < 2 * (kFuSW + 12); i += blockDim.x) {
@@ -339,29 +348,32 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
                         const bf16x8 A = *(const bf16x8*)(row + ta * 16 * 32);
                         const bf16x8 Bv = *(const bf16x8*)(row + tb * 16 * 32);
                         const bf16x8 C = *(const bf16x8*)(row + 4 * 16 * 32);
-                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][0], a0, 0, 0, 0);
-                        b0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bv, B[t][0], b0, 0, 0, 0);
-                        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(C, half ? B[t][1] : B[t][0], c0, 0, 0, 0);
-                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t][1], a1, 0, 0, 0);
-                        b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bv, B[t][1], b1, 0, 0, 0);
+                        /* weights as the first operand: D = W^T X^T, so a
+                         * lane's four outputs are 4 channels of one pixel */
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][0], A, a0, 0, 0, 0);
+                        b0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][0], Bv, b0, 0, 0, 0);
+                        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(half ? B[t][1] : B[t][0], C, c0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], A, a1, 0, 0, 0);
+                        b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], Bv, b1, 0, 0, 0);
                     }
                 }
-                /* D layout: lane holds pixels 4g..4g+3 of a tile, channel li */
-                const float bc = half ? bias1 : bias0;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int pa = ta * 16 + 4 * g + e, pb = tb * 16 + 4 * g + e, pc = 64 + 4 * g + e;
-                    const bool ia = live && ((inmask >> e) & 1u), ib = live && ((inmask >> (4 + e)) & 1u),
-                               ic = live && ((inmask >> (8 + e)) & 1u);
-                    uint16_t* da = R(ml, r, pa);
-                    uint16_t* db = R(ml, r, pb);
-                    uint16_t* dc = R(ml, r, pc);
-                    da[li] = ia ? f2bf(fmaxf(a0[e] + bias0, 0.f)) : (uint16_t)0;
-                    da[16 + li] = ia ? f2bf(fmaxf(a1[e] + bias1, 0.f)) : (uint16_t)0;
-                    db[li] = ib ? f2bf(fmaxf(b0[e] + bias0, 0.f)) : (uint16_t)0;
-                    db[16 + li] = ib ? f2bf(fmaxf(b1[e] + bias1, 0.f)) : (uint16_t)0;
-                    dc[16 * half + li] = ic ? f2bf(fmaxf(c0[e] + bc, 0.f)) : (uint16_t)0;
-                }
+                /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li
+                 * of a tile: bias, ReLU, BF16, one 8-byte store */
+                auto put = [&](const f32x4& acc, int nb, int tile, unsigned bit) {
+                    uint2 v = make_uint2(0u, 0u);
+                    if (live && ((inmask >> bit) & 1u)) {
+                        v.x = (uint32_t)f2bf(fmaxf(acc[0] + bias[nb][0], 0.f)) |
+                              ((uint32_t)f2bf(fmaxf(acc[1] + bias[nb][1], 0.f)) << 16);
+                        v.y = (uint32_t)f2bf(fmaxf(acc[2] + bias[nb][2], 0.f)) |
+                              ((uint32_t)f2bf(fmaxf(acc[3] + bias[nb][3], 0.f)) << 16);
+                    }
+                    *(uint2*)(R(ml, r, tile * 16 + li) + 16 * nb + 4 * g) = v;
+                };
+                put(a0, 0, ta, 0);
+                put(a1, 1, ta, 0);
+                put(b0, 0, tb, 1);
+                put(b1, 1, tb, 1);
+                put(c0, half, 4, 2);
             }
         }
         };
@@ -375,10 +387,9 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 #pragma unroll
                 for (int t = 0; t < 9; ++t) {
                     const uint2 a = *(const uint2*)(R(5, r + t / 3 - 1, p + t % 3) + 4 * cg);
-                    v += w6r[0][t] * __uint_as_float(a.x << 16);
-                    v += w6r[1][t] * __uint_as_float(a.x & 0xffff0000u);
-                    v += w6r[2][t] * __uint_as_float(a.y << 16);
-                    v += w6r[3][t] * __uint_as_float(a.y & 0xffff0000u);
+                    /* BF16 x BF16 products are exact in FP32 */
+                    v = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a.x), w6p[0][t], v, false);
+                    v = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a.y), w6p[1][t], v, false);
                 }
             }
             v += __shfl_xor(v, 1, 64);
