@@ -677,14 +677,15 @@ void Ctx::mix_ssim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     cls[MIX_SSIM] = cls[MIX_SSIMFIN] = 0;
     cnt.blocks[MIX_SSIM][0] += nb;
     cnt.blocks[MIX_SSIMFIN][0] += 1;
+    /* the row walker's stage rows are sized for the widest input of the batch */
+    if (R <= kRwMaxRadius) cnt.lds[MIX_SSIM][0] = std::max(cnt.lds[MIX_SSIM][0], rw_lds_bytes(R, pj.ch));
 }
 
 template <int R>
-static void launch_ssim_mix(Ctx& c, int nblocks, const MixJob* jobs, const int* st, int n, const SsimWeights& wt,
-                            float c1, float c2) {
+static void launch_ssim_mix(Ctx& c, int nblocks, size_t lds, const MixJob* jobs, const int* st, int n,
+                            const SsimWeights& wt, float c1, float c2) {
     if constexpr (R <= kRwMaxRadius)
-        c.launch("ssim", mix_ssim_rw_kernel<R>, dim3(nblocks), dim3(kRwThreads), rw_lds_bytes(R, 4), jobs, st, n,
-                 wt, c1, c2); /* sized for 4 channels, the widest input */
+        c.launch("ssim", mix_ssim_rw_kernel<R>, dim3(nblocks), dim3(kRwThreads), lds, jobs, st, n, wt, c1, c2);
     else if constexpr (R <= 7)
         c.launch("ssim", mix_ssim_kernel<R, 64, 16>, dim3(nblocks), dim3(kBlock), 0, jobs, st, n, wt, c1, c2);
     else
@@ -703,7 +704,7 @@ void Ctx::mix_ssim_launch(const MixJob* d_jobs, const int* d_starts, int n, cons
     const int nb = cnt.blocks[MIX_SSIM][0];
     switch (R) {
 #define IQM_SSIM_MIX_CASE(r) \
-    case r: launch_ssim_mix<r>(*this, nb, d_jobs, st, n, wt, c1, c2); break;
+    case r: launch_ssim_mix<r>(*this, nb, cnt.lds[MIX_SSIM][0], d_jobs, st, n, wt, c1, c2); break;
         IQM_SSIM_MIX_CASE(1) IQM_SSIM_MIX_CASE(2) IQM_SSIM_MIX_CASE(3) IQM_SSIM_MIX_CASE(4) IQM_SSIM_MIX_CASE(5)
         IQM_SSIM_MIX_CASE(6) IQM_SSIM_MIX_CASE(7) IQM_SSIM_MIX_CASE(8) IQM_SSIM_MIX_CASE(9) IQM_SSIM_MIX_CASE(10)
         IQM_SSIM_MIX_CASE(11) IQM_SSIM_MIX_CASE(12) IQM_SSIM_MIX_CASE(13) IQM_SSIM_MIX_CASE(14) IQM_SSIM_MIX_CASE(15)
