@@ -342,6 +342,9 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
                 const int ta = 2 * half, tb = 2 * half + 1;
                 f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, b0 = a0, b1 = a0, c0 = a0;
                 if (live) {
+                    /* the matrix-core chain issues first when both waves of
+                     * a SIMD are ready; the other wave's FP32 layers fill in */
+                    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                     for (int t = 0; t < 9; ++t) {
                         const uint16_t* row = R(ml - 1, r + t / 3 - 1, li + t % 3) + 8 * g;
@@ -356,6 +359,7 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
                         a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], A, a1, 0, 0, 0);
                         b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], Bv, b1, 0, 0, 0);
                     }
+                    __builtin_amdgcn_s_setprio(0);
                 }
                 /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li
                  * of a tile: bias, ReLU, BF16, one 8-byte store */
