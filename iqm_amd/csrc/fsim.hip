@@ -416,8 +416,13 @@ __device__ __forceinline__ void fsim_prep2_body(const uint8_t* __restrict__ img,
         const uint4* p = (const uint4*)(((im ? a01 : a00) & ~(uintptr_t)15) + (uintptr_t)k * 16);
         uint4 v[FT];
 #pragma unroll
-        for (int r = 0; r < FT; ++r)
-            v[r] = r < nk ? *(const uint4*)((const uint8_t*)p + r * pitch) : make_uint4(0u, 0u, 0u, 0u);
+        for (int r = 0; r < FT; ++r) {
+            typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+            /* streaming loads: each image byte is read once */
+            const u32x4_t t = r < nk ? __builtin_nontemporal_load((const u32x4_t*)((const uint8_t*)p + r * pitch))
+                                     : u32x4_t{0u, 0u, 0u, 0u};
+            v[r] = make_uint4(t.x, t.y, t.z, t.w);
+        }
         unsigned e[4] = {0u, 0u, 0u, 0u}, o[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int r = 0; r < FT; ++r) {
