@@ -1045,11 +1045,13 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
                 } else {
                     /* plane 0: sum_o An (added in the order o = 0..3 by the row pass) */
                     const float* ae = an_e + ((long)pair * 2 + im) * kAnePlanes * N;
-                    a_n[r][im] = *at32(ae, idx);
-                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = *at32(ae, (1 + o) * n1 + idx);
+                    a_n[r][im] = __builtin_nontemporal_load(at32(ae, idx));
+                    for (int o = 0; o < 4; ++o) a_e[r][im][o] = __builtin_nontemporal_load(at32(ae, (1 + o) * n1 + idx));
                 }
             }
-            for (int c = 0; c < 4; ++c) iq[r][c] = *at32(yq, (2 + c) * n1 + idx);
+            /* streaming loads for the planes read once (the Y halo above is
+             * shared by neighbouring tiles through the cache) */
+            for (int c = 0; c < 4; ++c) iq[r][c] = __builtin_nontemporal_load(at32(yq, (2 + c) * n1 + idx));
         }
     }
 #pragma unroll
