@@ -12,6 +12,10 @@ pytestmark = pytest.mark.gpu
 
 CASES = [((320, 240, 3), 3), ((240, 320, 3), 3), ((401, 299, 3), 2), ((97, 131, 3), 1),
          ((513, 257, 3), 3), ((360, 360, 1), 3), ((288, 200, 4), 3)]
+# the batched SSIM sizes its staged rows for the widest input of the batch:
+# 4 channels for CASES, 3 and 1 for these
+CASES_RGB_GRAY = [c for c in CASES if c[0][2] in (1, 3)]
+CASES_GRAY = [((320, 240, 1), 3), ((97, 131, 1), 1), ((401, 299, 1), 2)]
 
 
 def _upload(ctx, a):
@@ -20,9 +24,9 @@ def _upload(ctx, a):
     return d
 
 
-def _make(ctx):
-    jobs, keep = [], []
-    for i, ((w, h, ch), methods) in enumerate(CASES):
+def _make(ctx, cases=CASES):
+    jobs = []
+    for i, ((w, h, ch), methods) in enumerate(cases):
         img, ref = synth.rgb_pair(w, h, seed=900 + i)
         if ch == 1:
             img, ref = img[:, :, 1].copy(), ref[:, :, 1].copy()
@@ -45,9 +49,10 @@ def _maps(ctx, j):
     return out
 
 
-def test_mixed_batch_matches_single_calls(ctx):
-    print(f"[batch] {len(CASES)} pairs", flush=True)
-    jobs = _make(ctx)
+@pytest.mark.parametrize("cases", [CASES, CASES_RGB_GRAY, CASES_GRAY], ids=["widest4", "widest3", "gray"])
+def test_mixed_batch_matches_single_calls(ctx, cases):
+    print(f"[batch] {len(cases)} pairs, channels {sorted({c[0][2] for c in cases})}", flush=True)
+    jobs = _make(ctx, cases)
     ssim, fsim = iqm_amd.SSIM(), iqm_amd.FSIM()
     # single-pair reference values and maps
     want, want_maps = [], []
@@ -75,7 +80,8 @@ def test_mixed_batch_matches_single_calls(ctx):
             for k in wm:
                 assert np.array_equal(gm[k], wm[k]), (rep, k)
         # other work on the stream between replays
-        ssim.compute_batch_dev(ctx, jobs[0]["d_img"], jobs[0]["d_ref"], 320, 240, 3, 0, 1)
+        j0 = jobs[0]
+        ssim.compute_batch_dev(ctx, j0["d_img"], j0["d_ref"], j0["w"], j0["h"], j0["ch"], 0, 1)
     for j in jobs:
         for k in ("d_img", "d_ref", "d_ssim_map", "d_simc_map"):
             if j[k]:
