@@ -107,6 +107,24 @@ int iqm_ssim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
                        int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                        const iqm_ssim_params* params, float* d_maps, double* mssim_out);
 
+/* Allocation-free variant of iqm_ssim_batch_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing: the FP64 workgroup partials go to a static device
+ * scratch of IQM_SSIM_NOALLOC_MAX_PARTIALS doubles (1 MiB per GPU, held until
+ * the process ends), and the caller passes every input and output.  A batch
+ * needs n * ceil(w / 256) * ceil(h / 64) partials for kernel_size <= 13
+ * (ceil(w / 64) * ceil(h / 16) for 15, ceil(w / 32) * ceil(h / 8) above);
+ * more is rejected with IQM_ERR_INVALID.  d_mssim: n device doubles; the call
+ * is enqueued on the context's stream and returns without waiting (iqm_sync
+ * waits).  All calls on one GPU share its scratch: calls on one context are
+ * ordered by its stream, calls that overlap on one GPU from different
+ * contexts or streams race. */
+#define IQM_SSIM_NOALLOC_MAX_PARTIALS 131072
+int iqm_ssim_batch_dev_noalloc(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w,
+                               int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
+                               const iqm_ssim_params* params, float* d_maps, double* d_mssim);
+
 /* ---- FSIM / FSIMc ------------------------------------------------------ */
 
 /* Downscale factor and grid (src/gpu/fsim/fsim.cpp:83,91-92,165-168). */

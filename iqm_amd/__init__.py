@@ -39,7 +39,7 @@ EXPORTS = (
     "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches", "iqm_sur_make_patches_scaled",
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
-    "iqm_batch_dev", "iqm_synth_reference_dev",
+    "iqm_batch_dev", "iqm_synth_reference_dev", "iqm_ssim_batch_dev_noalloc",
 )
 
 JOB_SSIM = 1
@@ -105,6 +105,8 @@ def lib():
     L.iqm_ssim_default_params.restype = None
     L.iqm_ssim.argtypes = [vp, ctypes.POINTER(Image), ctypes.POINTER(Image), ctypes.POINTER(SsimParams), vp, dp]
     L.iqm_ssim_batch_dev.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, ctypes.POINTER(SsimParams), vp, dp]
+    L.iqm_ssim_batch_dev_noalloc.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, ctypes.POINTER(SsimParams),
+                                             vp, vp]
     L.iqm_fsim_dims.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.iqm_fsim_dims.restype = None
     L.iqm_fsim.argtypes = [vp, ctypes.POINTER(Image), ctypes.POINTER(Image), dp, dp,
@@ -356,6 +358,14 @@ class SSIM:
         _check(lib().iqm_ssim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
                                         ctypes.byref(p), d_maps, _dptr(out)))
         return out
+
+    def compute_batch_dev_noalloc(self, ctx: Context, d_img: int, d_ref: int, w: int, h: int, ch: int,
+                                  pair_stride: int, n: int, d_mssim: int, d_maps: int | None = None) -> None:
+        """Allocation-free form: enqueues the batch with mssim to d_mssim (n device
+        doubles) and its partials in the library's static scratch; ctx.sync() waits."""
+        p = self.params()
+        _check(lib().iqm_ssim_batch_dev_noalloc(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
+                                                ctypes.byref(p), d_maps, d_mssim))
 
 
 class FSIM:

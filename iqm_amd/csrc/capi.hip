@@ -24,6 +24,7 @@ Ctx::Ctx(int device) : device_(device) {
             std::to_string(prop.sharedMemPerBlock / 1024) + " KiB LDS/block)";
     lds_per_block_ = prop.sharedMemPerBlock;
     if (const char* g = std::getenv("IQM_GRAPHS")) graphs_ = std::atoi(g) != 0;
+    ssim_noalloc_setup(device);
     IQM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
@@ -327,6 +328,22 @@ int iqm_ssim_batch_dev(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref,
         ctx->c->ssim_batch(d_img, d_ref, w, h, ch, pair_stride, n, p.kernel_size, p.k1, p.k2,
                            p.sigma, d_maps, ms.data());
         if (mssim_out) std::memcpy(mssim_out, ms.data(), sizeof(double) * n);
+    });
+}
+
+int iqm_ssim_batch_dev_noalloc(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* d_ref, int32_t w, int32_t h,
+                               int32_t ch, int64_t pair_stride, int32_t n, const iqm_ssim_params* params,
+                               float* d_maps, double* d_mssim) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(d_img && d_ref && d_mssim && w > 0 && h > 0 && n > 0, IQM_ERR_INVALID, "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        iqm_ssim_params p;
+        iqm_ssim_default_params(&p);
+        if (params) p = *params;
+        ctx->c->ssim_batch_noalloc(d_img, d_ref, w, h, ch, pair_stride, n, p.kernel_size, p.k1, p.k2, p.sigma,
+                                   d_maps, d_mssim);
     });
 }
 

@@ -185,6 +185,15 @@ class Ctx {
     double* ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                        long pair_stride, int n, int ks, float k1, float k2, float sigma,
                        float* d_maps, double* mssim_host);
+    /* The same batch with its workgroup partials in the library's static
+     * device scratch (no allocation; see ssim_noalloc_setup): mssim of each
+     * pair to d_mssim (device, n doubles), enqueued without waiting. */
+    void ssim_batch_noalloc(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                            int n, int ks, float k1, float k2, float sigma, float* d_maps, double* d_mssim);
+    /* the launches shared by both: partials to `partial`, results to `out` */
+    void ssim_enqueue(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride, int n,
+                      int ks, float k1, float k2, float sigma, float* d_maps, double* partial, double* out,
+                      bool timed);
 
     /* Pairs of any sizes and methods in one launch sequence on the stream:
      * returns the device array of n x 4 doubles [mssim, fsim, fsimc, -] (NaN
@@ -288,6 +297,15 @@ class Ctx {
     std::map<std::string, std::pair<double, int64_t>> totals_;
     std::vector<std::string> names_cache_;
 };
+
+/* Static device scratch of the allocation-free SSIM entry point: FP64
+ * workgroup partials (1 MiB per GPU, held from library load to process end),
+ * and the device ordinals it is set up for. */
+constexpr int kNoAllocPartials = 1 << 17;
+constexpr int kMaxNoAllocDevices = 64;
+/* Takes the device address of the scratch on `device` (which must be
+ * current) once; called by every context's constructor. */
+void ssim_noalloc_setup(int device);
 
 /* workspace slots */
 enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR, WS_GEN, WS_BATCH,

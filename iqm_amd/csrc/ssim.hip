@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace iqm {
@@ -552,44 +554,53 @@ static void ssim_tile(int R, int& TX, int& TY) {
     }
 }
 
-template <int R>
-static void launch_ssim(Ctx& c, dim3 grid, const uint8_t* a, const uint8_t* b, long ps, int w,
-                        int h, int ch, const SsimWeights& wt, float c1, float c2, float* map,
-                        double* partial) {
-    if constexpr (R <= 7)
-        c.launch("ssim", ssim_kernel<R, 64, 16>, grid, dim3(kBlock), 0, a, b, ps, w, h, ch, wt, c1,
-                 c2, map, partial);
-    else
-        c.launch("ssim", ssim_kernel<R, 32, 8>, grid, dim3(kBlock), 0, a, b, ps, w, h, ch, wt, c1,
-                 c2, map, partial);
-}
-
-double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
-                        long pair_stride, int n, int ks, float k1, float k2, float sigma,
-                        float* d_maps, double* mssim_host) {
-    IQM_CHECK(ks >= 3 && ks <= 2 * kMaxRadius + 1 && (ks % 2) == 1, IQM_ERR_INVALID,
-              "kernel_size must be odd and in [3, 31]");
-    const int R = (ks - 1) / 2;
-    /* ssim.cpp:390-400 needs a non-empty crop */
-    IQM_CHECK(w - R >= R && h - R >= R, IQM_ERR_TOO_SMALL, "image smaller than the SSIM window");
-    SsimWeights wt{};
-    const double sg = (double)sigma;
-    for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
+/* rows of the row walker's output tile (IQM_SSIM_TY overrides, at least 8) */
+static int ssim_tile_rows(int R) {
     int TX, TY;
     ssim_tile(R, TX, TY);
     if (R <= kRwMaxRadius) TY = std::max(8, env_int_ssim("IQM_SSIM_TY", TY));
-    dim3 grid(ceil_div(w, TX), ceil_div(h, TY), n);
+    return TY;
+}
+
+static dim3 ssim_grid(int w, int h, int ks, int n) {
+    const int R = (ks - 1) / 2;
+    int TX, TY;
+    ssim_tile(R, TX, TY);
+    TY = ssim_tile_rows(R);
+    return dim3(ceil_div(w, TX), ceil_div(h, TY), n);
+}
+
+/* Launches of one same-size SSIM batch: the map/partials kernel (one FP64
+ * partial per workgroup at partial[(pair * gy + by) * gx + bx], every one
+ * written) and the fixed-order finalize into out[0..n).  `timed` routes the
+ * launches through the context's stage timer (hipEvents); the
+ * allocation-free entry point launches without it. */
+void Ctx::ssim_enqueue(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride, int n,
+                       int ks, float k1, float k2, float sigma, float* d_maps, double* partial, double* out,
+                       bool timed) {
+    const int R = (ks - 1) / 2;
+    SsimWeights wt{};
+    const double sg = (double)sigma;
+    for (int d = -R; d <= R; ++d) wt.g[d + R] = (float)std::exp(-(double)(d * d) / (2.0 * sg * sg));
+    const dim3 grid = ssim_grid(w, h, ks, n);
     const int nblk = grid.x * grid.y;
-    double* partial = (double*)workspace(WS_SSIM, sizeof(double) * ((size_t)nblk * n + n));
-    double* out = partial + (size_t)nblk * n;
     const float c1 = k1 * k1, c2 = k2 * k2;
+    auto L = [&](const char* tag, auto kernel, dim3 g, dim3 b, size_t lds, auto... args) {
+        if (timed) {
+            launch(tag, kernel, g, b, lds, args...);
+        } else {
+            hipLaunchKernelGGL(kernel, g, b, lds, stream_, args...);
+            IQM_HIP(hipGetLastError());
+        }
+    };
     if (R <= kRwMaxRadius) {
         const size_t lds = rw_lds_bytes(R, ch);
+        const int ty = ssim_tile_rows(R);
         switch (R) {
-#define IQM_SSIM_RW_CASE(r)                                                                                     \
-    case r:                                                                                                     \
-        launch("ssim", ssim_rw_kernel<r>, grid, dim3(kRwThreads), lds, d_img, d_ref, pair_stride, w, h, ch, wt, \
-               c1, c2, d_maps, partial, TY);                                                                    \
+#define IQM_SSIM_RW_CASE(r)                                                                                         \
+    case r:                                                                                                         \
+        L("ssim", ssim_rw_kernel<r>, grid, dim3(kRwThreads), lds, d_img, d_ref, pair_stride, w, h, ch, wt, c1, c2, \
+          d_maps, partial, ty);                                                                                     \
         break;
             IQM_SSIM_RW_CASE(1) IQM_SSIM_RW_CASE(2) IQM_SSIM_RW_CASE(3)
             IQM_SSIM_RW_CASE(4) IQM_SSIM_RW_CASE(5) IQM_SSIM_RW_CASE(6)
@@ -597,22 +608,89 @@ double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
             default: break;
         }
     } else switch (R) {
-#define IQM_SSIM_CASE(r) \
-    case r: launch_ssim<r>(*this, grid, d_img, d_ref, pair_stride, w, h, ch, wt, c1, c2, d_maps, partial); break;
-        IQM_SSIM_CASE(1) IQM_SSIM_CASE(2) IQM_SSIM_CASE(3) IQM_SSIM_CASE(4) IQM_SSIM_CASE(5)
-        IQM_SSIM_CASE(6) IQM_SSIM_CASE(7) IQM_SSIM_CASE(8) IQM_SSIM_CASE(9) IQM_SSIM_CASE(10)
-        IQM_SSIM_CASE(11) IQM_SSIM_CASE(12) IQM_SSIM_CASE(13) IQM_SSIM_CASE(14) IQM_SSIM_CASE(15)
+#define IQM_SSIM_CASE(r, tx, ty)                                                                                \
+    case r:                                                                                                     \
+        L("ssim", ssim_kernel<r, tx, ty>, grid, dim3(kBlock), 0, d_img, d_ref, pair_stride, w, h, ch, wt, c1, c2, \
+          d_maps, partial);                                                                                     \
+        break;
+        IQM_SSIM_CASE(7, 64, 16) IQM_SSIM_CASE(8, 32, 8) IQM_SSIM_CASE(9, 32, 8) IQM_SSIM_CASE(10, 32, 8)
+        IQM_SSIM_CASE(11, 32, 8) IQM_SSIM_CASE(12, 32, 8) IQM_SSIM_CASE(13, 32, 8) IQM_SSIM_CASE(14, 32, 8)
+        IQM_SSIM_CASE(15, 32, 8)
 #undef IQM_SSIM_CASE
         default: break;
     }
     const double count = (double)((long)(w - 2 * R + 1) * (long)(h - 2 * R + 1));
-    launch("ssim_finalize", ssim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
-           nblk, count, out);
+    L("ssim_finalize", ssim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial, nblk, count, out);
+}
+
+static void ssim_check_args(int w, int h, int ks) {
+    IQM_CHECK(ks >= 3 && ks <= 2 * kMaxRadius + 1 && (ks % 2) == 1, IQM_ERR_INVALID,
+              "kernel_size must be odd and in [3, 31]");
+    const int R = (ks - 1) / 2;
+    /* ssim.cpp:390-400 needs a non-empty crop */
+    IQM_CHECK(w - R >= R && h - R >= R, IQM_ERR_TOO_SMALL, "image smaller than the SSIM window");
+}
+
+double* Ctx::ssim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                        long pair_stride, int n, int ks, float k1, float k2, float sigma,
+                        float* d_maps, double* mssim_host) {
+    ssim_check_args(w, h, ks);
+    const dim3 grid = ssim_grid(w, h, ks, n);
+    const size_t nblk = (size_t)grid.x * grid.y;
+    double* partial = (double*)workspace(WS_SSIM, sizeof(double) * (nblk * n + n));
+    double* out = partial + nblk * n;
+    ssim_enqueue(d_img, d_ref, w, h, ch, pair_stride, n, ks, k1, k2, sigma, d_maps, partial, out, true);
     if (mssim_host) {
         IQM_HIP(hipMemcpyAsync(mssim_host, out, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
         IQM_HIP(hipStreamSynchronize(stream_));
     }
     return out;
+}
+
+/* ---- allocation-free SSIM batch ------------------------------------------
+ * The FP64 workgroup partials of iqm_ssim_batch_dev_noalloc live in this
+ * file-scope device array instead of a context workspace.  In host code its
+ * name is only a stand-in: the device address of each GPU's copy is taken
+ * once per device with hipGetSymbolAddress (ssim_noalloc_setup, called when a
+ * context is created) and only that address is handed to kernels.  Every call
+ * on one GPU shares that GPU's copy, so calls that overlap on one GPU (two
+ * contexts, two streams) race; calls on one context are ordered by its
+ * stream.  Each workgroup writes its own partial, so nothing needs clearing
+ * between calls. */
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxNoAllocDevices] = {};
+}  // namespace
+
+void ssim_noalloc_setup(int device) {
+    IQM_CHECK(device >= 0 && device < kMaxNoAllocDevices, IQM_ERR_UNSUPPORTED,
+              "device ordinal beyond the allocation-free scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    /* the caller has made `device` current */
+    IQM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    IQM_CHECK(p != nullptr, IQM_ERR_DEVICE, "hipGetSymbolAddress returned no address for the SSIM scratch");
+    g_scratch_ptr[device] = (double*)p;
+}
+
+void Ctx::ssim_batch_noalloc(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                             int n, int ks, float k1, float k2, float sigma, float* d_maps, double* d_mssim) {
+    ssim_check_args(w, h, ks);
+    const dim3 grid = ssim_grid(w, h, ks, n);
+    const size_t need = (size_t)grid.x * grid.y * (size_t)n;
+    IQM_CHECK(need <= (size_t)kNoAllocPartials, IQM_ERR_INVALID,
+              "batch needs " + std::to_string(need) + " workgroup partials; the allocation-free SSIM holds at most " +
+                  std::to_string(kNoAllocPartials) + " (fewer pairs per call)");
+    double* partial = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        partial = g_scratch_ptr[device_];
+    }
+    IQM_CHECK(partial != nullptr, IQM_ERR_DEVICE, "allocation-free SSIM scratch not set up on this device");
+    ssim_enqueue(d_img, d_ref, w, h, ch, pair_stride, n, ks, k1, k2, sigma, d_maps, partial, d_mssim, false);
 }
 
 /* ---- mixed batches ------------------------------------------------------ */
