@@ -271,6 +271,11 @@ int iqm_sync(iqm_ctx* ctx);
  * with the same arguments (default on; IQM_GRAPHS=0 in the environment turns
  * it off at context creation). */
 int iqm_set_graphs(iqm_ctx* ctx, int32_t on);
+/* Captured graphs the context holds for same-size FSIM batches and for mixed
+ * batches (no counterpart in the reference; for monitoring and tests).  Each
+ * cache holds at most 16: a call with new buffers beyond that evicts the
+ * least recently launched graph and frees its job table. */
+int iqm_graph_cache_sizes(iqm_ctx* ctx, int32_t* same_size, int32_t* mixed);
 
 /* ---- autotuner (tile shapes of the FSIM launch sequence) ---------------- */
 

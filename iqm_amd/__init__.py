@@ -40,6 +40,7 @@ EXPORTS = (
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
     "iqm_batch_dev", "iqm_synth_reference_dev", "iqm_ssim_batch_dev_noalloc",
+    "iqm_graph_cache_sizes",
 )
 
 JOB_SSIM = 1
@@ -122,6 +123,7 @@ def lib():
     L.iqm_stream.restype = vp
     L.iqm_sync.argtypes = [vp]
     L.iqm_set_graphs.argtypes = [vp, i32]
+    L.iqm_graph_cache_sizes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.iqm_autotune_fsim.argtypes = [vp, i32, i32, i32, i32, dp]
     L.iqm_tune_export.argtypes = [vp, ctypes.c_char_p, i64]
     L.iqm_tune_export.restype = i64
@@ -266,6 +268,12 @@ class Context:
 
     def set_graphs(self, on: bool) -> None:
         _check(lib().iqm_set_graphs(self.handle, 1 if on else 0))
+
+    def graph_cache_sizes(self) -> tuple[int, int]:
+        """(same-size FSIM graphs, mixed-batch graphs) the context holds."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().iqm_graph_cache_sizes(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def autotune_fsim(self, w: int, h: int, ch: int = 3, n: int = 8) -> float:
         """Tunes the FSIM tiles for w x h inputs; returns the best us/pair."""

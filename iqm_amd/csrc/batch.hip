@@ -3,8 +3,12 @@
 // table (per pair: sizes, tiles, FFT line plans, noise constants, buffers)
 // and, per stage, the prefix sums of the pairs' workgroup counts; a
 // workgroup finds its pair by binary search and runs the same stage body as
-// a single-pair launch, so the scores are bit-identical.  Pairs whose line
-// transforms need more than 64 KiB of LDS run in a second launch of the
+// a single-pair launch, so the scores are bit-identical to the single-pair
+// calls -- except for pairs whose downscaled lines have a compile-time
+// two-level kernel (480, 270 and 256 long: 1080p, 4K and 512 x 512 inputs),
+// which the single-pair call runs and the mixed batch does not (it uses the
+// run-time transforms; the scores then differ by FP32 rounding, ~1e-8).
+// Pairs whose line transforms need more than 64 KiB of LDS run in a second launch of the
 // stage, so they do not cut the occupancy of the others.  The sequence is
 // captured once as a hipGraph with a job table of its own; a repeated batch
 // costs one graph launch.
@@ -113,6 +117,7 @@ double* Ctx::mixed_batch_graph(const PairJob* jobs, int n, const SsimArgs& sp) {
     key.append((const char*)&sp, sizeof(sp));
     auto it = mixed_graphs_.find(key);
     if (it != mixed_graphs_.end()) {
+        it->second.last_use = ++graph_tick_;
         IQM_HIP(hipGraphLaunch(it->second.exec, stream_));
         return it->second.res;
     }
@@ -143,12 +148,14 @@ double* Ctx::mixed_batch_graph(const PairJob* jobs, int n, const SsimArgs& sp) {
     GraphEntry e;
     e.res = res;
     e.table = table;
+    e.last_use = ++graph_tick_;
     const hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess) {
         (void)hipFree(table);
         IQM_HIP(ie);
     }
+    evict_lru(mixed_graphs_);
     mixed_graphs_.emplace(std::move(key), e);
     return res;
 }

@@ -90,13 +90,19 @@ void* Ctx::workspace(int slot, size_t bytes) {
     return e.first;
 }
 
+void Ctx::release_graph(GraphEntry& e) {
+    /* a launch of this graph may still be running on the stream */
+    (void)hipStreamSynchronize(stream_);
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.table) (void)hipFree(e.table);
+    e.exec = nullptr;
+    e.table = nullptr;
+}
+
 void Ctx::drop_graphs() {
-    for (auto& kv : graphs_cache_) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto& kv : graphs_cache_) release_graph(kv.second);
     graphs_cache_.clear();
-    for (auto& kv : mixed_graphs_) {
-        (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.table) (void)hipFree(kv.second.table);
-    }
+    for (auto& kv : mixed_graphs_) release_graph(kv.second);
     mixed_graphs_.clear();
 }
 
@@ -125,12 +131,15 @@ double* Ctx::fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w,
         IQM_HIP(hipStreamEndCapture(stream_, &g));
         GraphEntry e;
         e.res = res;
+        e.last_use = ++graph_tick_;
         const hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         IQM_HIP(ie);
+        evict_lru(graphs_cache_);
         it = graphs_cache_.emplace(key, e).first;
         return res; /* the capture ran nothing: the eager run above produced it */
     }
+    it->second.last_use = ++graph_tick_;
     IQM_HIP(hipGraphLaunch(it->second.exec, stream_));
     return it->second.res;
 }
@@ -795,6 +804,16 @@ int64_t iqm_tune_export(iqm_ctx* ctx, char* buf, int64_t cap) {
 int iqm_tune_import(iqm_ctx* ctx, const char* json) {
     if (!ctx || !json) return fail(IQM_ERR_INVALID, "null argument");
     return guarded([&] { ctx->c->import_tunes(json); });
+}
+
+int iqm_graph_cache_sizes(iqm_ctx* ctx, int32_t* same_size, int32_t* mixed) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        int a = 0, b = 0;
+        ctx->c->graph_cache_sizes(&a, &b);
+        if (same_size) *same_size = a;
+        if (mixed) *mixed = b;
+    });
 }
 
 int iqm_set_graphs(iqm_ctx* ctx, int32_t on) {

@@ -284,10 +284,35 @@ class Ctx {
         hipGraphExec_t exec = nullptr;
         double* res = nullptr;
         void* table = nullptr; /* device job table a mixed-batch graph reads */
+        uint64_t last_use = 0; /* graph_tick_ of its last launch */
     };
     std::map<GraphKey, GraphEntry> graphs_cache_;
     std::map<std::string, GraphEntry> mixed_graphs_;
+    uint64_t graph_tick_ = 0;
     void drop_graphs();
+    void release_graph(GraphEntry& e);
+    /* makes room for one more entry: evicts the least recently launched
+     * graph (and its job table) when the cache holds kGraphCacheCap */
+    template <class Map>
+    void evict_lru(Map& m) {
+        if (m.size() < (size_t)kGraphCacheCap) return;
+        auto victim = m.begin();
+        for (auto it = m.begin(); it != m.end(); ++it)
+            if (it->second.last_use < victim->second.last_use) victim = it;
+        release_graph(victim->second);
+        m.erase(victim);
+    }
+
+   public:
+    /* captured graphs held per cache (same-size FSIM batches, mixed batches);
+     * each holds at most kGraphCacheCap */
+    static constexpr int kGraphCacheCap = 16;
+    void graph_cache_sizes(int* same_size, int* mixed) const {
+        if (same_size) *same_size = (int)graphs_cache_.size();
+        if (mixed) *mixed = (int)mixed_graphs_.size();
+    }
+
+   private:
     bool graphs_ = true;
     bool capturing_ = false;
 

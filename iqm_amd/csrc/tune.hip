@@ -64,31 +64,60 @@ FsimTune Ctx::autotune_fsim(int w, int h, int ch, int n) {
                         t.us_per_pair = -1; /* explicit tiles (not the built-in defaults) */
                         cand.push_back(t);
                     }
-    const bool was_timing = timing_;
+    /* state restored however the loop ends: the size's previous tiles (or
+     * none), the stage timer, the two events */
+    const auto key = std::make_pair(wd, hd);
+    auto prev_it = tunes_.find(key);
+    const bool had_prev = prev_it != tunes_.end();
+    const FsimTune prev = had_prev ? prev_it->second : FsimTune{};
+    struct Restore {
+        Ctx& c;
+        bool was_timing;
+        hipEvent_t e0, e1;
+        ~Restore() {
+            c.timing_ = was_timing;
+            c.free_events_.push_back(e0);
+            c.free_events_.push_back(e1);
+        }
+    } restore{*this, timing_, take_event(), take_event()};
     timing_ = false;
-    hipEvent_t e0 = take_event(), e1 = take_event();
     FsimBatchOut out;
     FsimTune best;
     double best_ms = 1e30;
+    int failed = 0;
     for (FsimTune t : cand) {
-        tunes_[{wd, hd}] = t;
-        fsim_batch(a, b, w, h, ch, pair_bytes, n, out); /* warm-up: plans and workspaces */
-        IQM_HIP(hipEventRecord(e0, stream_));
-        const int reps = 3;
-        for (int r = 0; r < reps; ++r) fsim_batch(a, b, w, h, ch, pair_bytes, n, out);
-        IQM_HIP(hipEventRecord(e1, stream_));
-        IQM_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        IQM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
+        tunes_[key] = t;
+        try {
+            fsim_batch(a, b, w, h, ch, pair_bytes, n, out); /* warm-up: plans and workspaces */
+            IQM_HIP(hipEventRecord(restore.e0, stream_));
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r) fsim_batch(a, b, w, h, ch, pair_bytes, n, out);
+            IQM_HIP(hipEventRecord(restore.e1, stream_));
+            IQM_HIP(hipEventSynchronize(restore.e1));
+            float ms = 0;
+            IQM_HIP(hipEventElapsedTime(&ms, restore.e0, restore.e1));
+            ms /= reps;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        } catch (const Error&) {
+            /* a candidate the device refuses (launch resources, memory) is
+             * skipped; the stream is drained before the next one */
+            ++failed;
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(stream_);
+        } catch (...) {
+            if (had_prev) tunes_[key] = prev;
+            else tunes_.erase(key);
+            throw;
         }
     }
-    free_events_.push_back(e0);
-    free_events_.push_back(e1);
-    timing_ = was_timing;
+    if (best_ms >= 1e30) {
+        if (had_prev) tunes_[key] = prev;
+        else tunes_.erase(key);
+        throw Error(IQM_ERR_DEVICE, "autotune: every one of " + std::to_string(failed) + " candidates failed");
+    }
     best.us_per_pair = best_ms * 1e3 / n;
     set_fsim_tune(wd, hd, best);
     return best;
