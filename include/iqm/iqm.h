@@ -152,6 +152,22 @@ int iqm_fsim_batch_dev_async(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* 
                              int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                              float* d_simc_maps, double* d_scores);
 
+/* Host-fed FSIM: n pairs in host memory (pair_stride bytes apart, packed
+ * rows) scored in chunks of `chunk` pairs.  The input pipeline the reference
+ * runs per call (src/gpu/fsim/fsim.cpp:170-252: staging buffer, upload, wait)
+ * becomes a two-slot ring: chunk k is copied to the device on a copy stream
+ * while chunk k-1 is scored on the context's stream.  Inputs in pinned memory
+ * (iqm_host_alloc) are copied by DMA directly; pageable inputs pass through
+ * pinned staging filled by the calling thread.  Returns when all scores are
+ * in fsim_out / fsimc_out (n host doubles each, nullable); scores equal
+ * iqm_fsim_batch_dev's on the same pairs. */
+int iqm_fsim_batch_host(iqm_ctx* ctx, const uint8_t* h_img, const uint8_t* h_ref, int32_t w, int32_t h,
+                        int32_t ch, int64_t pair_stride, int32_t n, int32_t chunk, double* fsim_out,
+                        double* fsimc_out);
+/* Pinned host memory for iqm_fsim_batch_host's direct path (hipHostMalloc). */
+int iqm_host_alloc(iqm_ctx* ctx, size_t bytes, void** out);
+int iqm_host_free(iqm_ctx* ctx, void* p);
+
 /* ---- mixed batches ------------------------------------------------------ */
 /* Pairs of uneven sizes, each scored with SSIM and/or FSIM/FSIMc, issued as
  * one launch sequence and replayed as one captured hipGraph when the same
@@ -290,6 +306,21 @@ int iqm_autotune_fsim(iqm_ctx* ctx, int32_t w, int32_t h, int32_t ch, int32_t n,
 int64_t iqm_tune_export(iqm_ctx* ctx, char* buf, int64_t cap);
 /* Loads a plan file written by iqm_tune_export into the context. */
 int iqm_tune_import(iqm_ctx* ctx, const char* json);
+
+/* ---- stage marks (src/timestamps.h:18-51) ------------------------------- */
+
+/* When on, iqm_ssim and iqm_fsim record host time marks at the stage
+ * boundaries the reference's computeMetric marks (src/gpu/ssim/ssim.cpp:78,
+ * 227,232,235; src/gpu/fsim/fsim.cpp:85,89,95,138,141,155), waiting for the
+ * stream at each so that a mark times its stage (as the reference's
+ * synchronous stages do).  Off (the default) costs nothing.  Every kernel
+ * launch and every iqm_ssim / iqm_fsim call is also a roctx range named after
+ * it, for rocprofv3 --marker-trace. */
+int iqm_set_marks(iqm_ctx* ctx, int32_t on);
+/* Marks of the context's last iqm_ssim / iqm_fsim call: returns their count
+ * and copies up to max (name, nanoseconds of the C++ high_resolution_clock
+ * since its epoch) pairs; names stay valid until the next call. */
+int32_t iqm_marks(iqm_ctx* ctx, int32_t max, const char** names, int64_t* ns);
 
 /* ---- stage timing ------------------------------------------------------- */
 

@@ -18,7 +18,11 @@
 #ifndef IQM_IQM_HPP
 #define IQM_IQM_HPP
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <iomanip>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -46,10 +50,42 @@ inline iqm_image view(const InputImage& im) {
     return iqm_image{im.data.data(), im.width, im.height, 4, im.width * 4};
 }
 
-/* Host chrono marks, as src/timestamps.h keeps them. */
-struct Timestamps {
-    std::vector<std::pair<std::string, std::chrono::high_resolution_clock::time_point>> marks;
-    void mark(const std::string& name) { marks.emplace_back(name, std::chrono::high_resolution_clock::now()); }
+/* Host chrono marks (src/timestamps.h:15-51): computeMetric fills `inner`
+ * with the library's stage marks when the runtime has marks on
+ * (Runtime::setMarks), and print() writes the reference's table: per mark the
+ * time since `start` and since the previous mark, in microseconds, then the
+ * total. */
+class Timestamps {
+   public:
+    using clock = std::chrono::high_resolution_clock;
+    std::vector<std::pair<std::string, clock::time_point>> inner;
+    void mark(const std::string& name) { inner.emplace_back(name, clock::now()); }
+    /* the marks the library recorded during the last call on ctx */
+    void take(iqm_ctx* ctx) {
+        const int32_t n = iqm_marks(ctx, 0, nullptr, nullptr);
+        if (n <= 0) return;
+        std::vector<const char*> names((size_t)n);
+        std::vector<int64_t> ns((size_t)n);
+        iqm_marks(ctx, n, names.data(), ns.data());
+        for (int32_t i = 0; i < n; ++i)
+            inner.emplace_back(names[i], clock::time_point(std::chrono::duration_cast<clock::duration>(
+                                             std::chrono::nanoseconds(ns[i]))));
+    }
+    void print(clock::time_point start, clock::time_point end, std::ostream& os = std::cout) const {
+        using us = std::chrono::microseconds;
+        const auto exec = std::chrono::duration_cast<us>(end - start);
+        int longest = 5; /* "TOTAL" */
+        for (const auto& m : inner) longest = std::max(longest, (int)m.first.size());
+        const int pad = (int)std::ceil(std::log10((double)std::max<long long>(exec.count(), 1))) + 2;
+        for (size_t i = 0; i < inner.size(); ++i) {
+            const auto since = std::chrono::duration_cast<us>(inner[i].second - start);
+            const auto diff = std::chrono::duration_cast<us>(inner[i].second - (i ? inner[i - 1].second : start));
+            os << std::setw(longest) << inner[i].first << ": " << std::setw(pad) << (std::to_string(since.count()) + "us")
+               << " | " << std::setw(pad) << (std::to_string(diff.count()) + "us") << std::endl;
+        }
+        os << std::setw(longest) << "TOTAL" << ": " << std::setw(pad) << (std::to_string(exec.count()) + "us")
+           << std::endl;
+    }
 };
 
 class Runtime {
@@ -60,6 +96,8 @@ class Runtime {
     Runtime& operator=(const Runtime&) = delete;
     iqm_ctx* ctx() const { return ctx_; }
     std::string selectedDevice() const { return iqm_device_name(ctx_); }
+    /* stage marks of computeMetric (waits for the device at every stage) */
+    void setMarks(bool on) const { check(iqm_set_marks(ctx_, on ? 1 : 0)); }
 
    private:
     iqm_ctx* ctx_ = nullptr;
@@ -78,7 +116,6 @@ class SSIM {
     explicit SSIM(const Runtime&) {}
     SSIMResult computeMetric(const Runtime& rt, const InputImage& image, const InputImage& ref) {
         SSIMResult r;
-        r.timestamps.mark("start");
         r.width = (unsigned)image.width;
         r.height = (unsigned)image.height;
         r.imageData.resize((size_t)image.width * image.height);
@@ -87,7 +124,7 @@ class SSIM {
         double mssim = 0;
         check(iqm_ssim(rt.ctx(), &a, &b, &p, r.imageData.data(), &mssim));
         r.mssim = (float)mssim;
-        r.timestamps.mark("SSIM computed");
+        r.timestamps.take(rt.ctx());
         return r;
     }
     [[nodiscard]] double computeMSSIM(const float* buffer, unsigned width, unsigned height) const {
@@ -120,7 +157,6 @@ class FSIM {
     explicit FSIM(const Runtime&) {}
     FSIMResult computeMetric(const Runtime& rt, const InputImage& image, const InputImage& ref) {
         FSIMResult r;
-        r.timestamps.mark("start");
         iqm_image a = view(image), b = view(ref);
         double fs = 0, fc = 0;
         iqm_fsim_maps maps{};
@@ -135,7 +171,7 @@ class FSIM {
         check(iqm_fsim(rt.ctx(), &a, &b, &fs, &fc, wantMaps ? &maps : nullptr, nullptr));
         r.fsim = (float)fs;
         r.fsimc = (float)fc;
-        r.timestamps.mark("FSIM, FSIMc computed");
+        r.timestamps.take(rt.ctx());
         return r;
     }
 

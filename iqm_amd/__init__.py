@@ -40,7 +40,8 @@ EXPORTS = (
     "iqm_sur_loss_grad", "iqm_sur_train_step", "iqm_sur_infer", "iqm_sur_save", "iqm_sur_load",
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
     "iqm_batch_dev", "iqm_synth_reference_dev", "iqm_ssim_batch_dev_noalloc",
-    "iqm_graph_cache_sizes",
+    "iqm_graph_cache_sizes", "iqm_fsim_batch_host", "iqm_host_alloc", "iqm_host_free",
+    "iqm_set_marks", "iqm_marks",
 )
 
 JOB_SSIM = 1
@@ -114,6 +115,9 @@ def lib():
                            ctypes.POINTER(FsimMaps), ctypes.POINTER(FsimDetail)]
     L.iqm_fsim_batch_dev.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, vp, dp, dp]
     L.iqm_fsim_batch_dev_async.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, vp, vp]
+    L.iqm_fsim_batch_host.argtypes = [vp, vp, vp, i32, i32, i32, i64, i32, i32, dp, dp]
+    L.iqm_host_alloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.iqm_host_free.argtypes = [vp, vp]
     L.iqm_fft2d.argtypes = [vp, vp, i32, i32, i32, i32, i32]
     L.iqm_dev_alloc.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(vp)]
     L.iqm_dev_free.argtypes = [vp, vp]
@@ -129,6 +133,9 @@ def lib():
     L.iqm_tune_export.restype = i64
     L.iqm_tune_import.argtypes = [vp, ctypes.c_char_p]
     L.iqm_set_stage_timing.argtypes = [vp, i32]
+    L.iqm_set_marks.argtypes = [vp, i32]
+    L.iqm_marks.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(i64)]
+    L.iqm_marks.restype = i32
     L.iqm_stage_times.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(i64)]
     L.iqm_stage_times.restype = i32
     L.iqm_reset_stage_times.argtypes = [vp]
@@ -232,6 +239,15 @@ class Context:
     def free(self, ptr: int) -> None:
         _check(lib().iqm_dev_free(self.handle, ptr))
 
+    def host_alloc(self, nbytes: int) -> np.ndarray:
+        """Pinned host memory as a uint8 numpy array (freed by host_free)."""
+        p = ctypes.c_void_p()
+        _check(lib().iqm_host_alloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return np.ctypeslib.as_array((ctypes.c_uint8 * int(nbytes)).from_address(p.value))
+
+    def host_free(self, a: np.ndarray) -> None:
+        _check(lib().iqm_host_free(self.handle, a.ctypes.data))
+
     def upload(self, dst: int, a: np.ndarray) -> None:
         a = np.ascontiguousarray(a)
         _check(lib().iqm_memcpy_h2d(self.handle, dst, a.ctypes.data, a.nbytes))
@@ -291,6 +307,20 @@ class Context:
 
     def import_tunes(self, plan: str) -> None:
         _check(lib().iqm_tune_import(self.handle, plan.encode()))
+
+    def set_marks(self, on: bool) -> None:
+        """Host stage marks of iqm_ssim / iqm_fsim (src/timestamps.h)."""
+        _check(lib().iqm_set_marks(self.handle, 1 if on else 0))
+
+    def marks(self) -> list:
+        """[(name, ns)] of the last iqm_ssim / iqm_fsim call."""
+        n = lib().iqm_marks(self.handle, 0, None, None)
+        if n <= 0:
+            return []
+        names = (ctypes.c_char_p * n)()
+        ns = (ctypes.c_int64 * n)()
+        lib().iqm_marks(self.handle, n, names, ns)
+        return [(names[i].decode(), int(ns[i])) for i in range(n)]
 
     def set_stage_timing(self, on: bool) -> None:
         _check(lib().iqm_set_stage_timing(self.handle, 1 if on else 0))
@@ -423,6 +453,21 @@ class FSIM:
         fc = np.zeros(n, np.float64)
         _check(lib().iqm_fsim_batch_dev(ctx.handle, d_img, d_ref, w, h, ch, pair_stride, n,
                                         d_simc_maps, _dptr(fs), _dptr(fc)))
+        return fs, fc
+
+    def compute_batch_host(self, ctx: Context, img: np.ndarray, ref: np.ndarray, chunk: int = 32):
+        """FSIM/FSIMc of n host pairs, arrays of shape (n, h, w[, ch]) uint8 (pinned
+        ones from ctx.host_alloc are copied by DMA directly); returns (fsim, fsimc)."""
+        if img.shape != ref.shape or img.dtype != np.uint8 or ref.dtype != np.uint8:
+            raise ValueError("img and ref must be uint8 arrays of one shape")
+        if not (img.flags.c_contiguous and ref.flags.c_contiguous):
+            raise ValueError("img and ref must be C-contiguous")
+        n, h, w = img.shape[:3]
+        ch = img.shape[3] if img.ndim == 4 else 1
+        fs = np.zeros(n, np.float64)
+        fc = np.zeros(n, np.float64)
+        _check(lib().iqm_fsim_batch_host(ctx.handle, img.ctypes.data, ref.ctypes.data, w, h, ch, w * h * ch, n,
+                                         int(chunk), _dptr(fs), _dptr(fc)))
         return fs, fc
 
     def compute_batch_dev_async(self, ctx: Context, d_img: int, d_ref: int, w: int, h: int, ch: int,

@@ -7,7 +7,12 @@
 // Extra options (passed through the reference's generic "--key value" map):
 //   --device N    HIP device ordinal (default 0)
 //   --repeat N    evaluate the metric N times and print the mean time, the
-//                 counterpart of the reference's profile loop (src/profile.cpp)
+//                 counterpart of the reference's profile loop (src/profile.cpp:
+//                 218-290); each evaluation is a roctx range "iteration i"
+//                 (every stage launch inside it is a range of its own), so
+//                 rocprofv3 --marker-trace attributes the kernels per run
+// With -v the per-stage Timestamps table of the reference is printed
+// (src/timestamps.h:18-51, src/main.cpp:98-100,208-210).
 #include <chrono>
 #include <cstring>
 #include <iostream>
@@ -15,6 +20,8 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "iqm/iqm.hpp"
 #include "png_io.hpp"
@@ -123,14 +130,21 @@ void ssim(const IQM::Args& args) {
     const IQM::HIP::Runtime rt(opt_int(args, "--device", 0));
     if (args.verbose) std::cout << "Selected device: " << rt.selectedDevice() << std::endl;
     IQM::HIP::SSIM ssim(rt);
+    rt.setMarks(args.verbose);
     const int repeat = std::max(1, opt_int(args, "--repeat", 1));
-    auto start = std::chrono::high_resolution_clock::now();
+    auto start = std::chrono::high_resolution_clock::now(), begin = start;
     IQM::HIP::SSIMResult result;
-    for (int i = 0; i < repeat; ++i) result = ssim.computeMetric(rt, input, reference);
+    for (int i = 0; i < repeat; ++i) {
+        start = std::chrono::high_resolution_clock::now();
+        roctxRangePushA(("iteration " + std::to_string(i)).c_str());
+        result = ssim.computeMetric(rt, input, reference);
+        roctxRangePop();
+    }
     auto end = std::chrono::high_resolution_clock::now();
     std::cout << "MSSIM: " << result.mssim << std::endl;
-    if (args.verbose || repeat > 1)
-        print_timing("SSIM", std::chrono::duration<double, std::milli>(end - start).count() / repeat, repeat);
+    if (args.verbose) result.timestamps.print(start, end);
+    if (repeat > 1)
+        print_timing("SSIM", std::chrono::duration<double, std::milli>(end - begin).count() / repeat, repeat);
     if (args.outputPath.has_value()) {
         auto converted = convertFloatToChar(result.imageData);
         if (!iqmcli::write_png_gray(args.outputPath.value(), result.width, result.height, converted.data()))
@@ -146,14 +160,21 @@ void fsim(const IQM::Args& args) {
     IQM::HIP::FSIM fsim(rt);
     if (args.verbose) std::cout << "Selected device: " << rt.selectedDevice() << std::endl;
     fsim.wantMaps = args.outputPath.has_value();
+    rt.setMarks(args.verbose);
     const int repeat = std::max(1, opt_int(args, "--repeat", 1));
-    auto start = std::chrono::high_resolution_clock::now();
+    auto start = std::chrono::high_resolution_clock::now(), begin = start;
     IQM::HIP::FSIMResult result;
-    for (int i = 0; i < repeat; ++i) result = fsim.computeMetric(rt, input, reference);
+    for (int i = 0; i < repeat; ++i) {
+        start = std::chrono::high_resolution_clock::now();
+        roctxRangePushA(("iteration " + std::to_string(i)).c_str());
+        result = fsim.computeMetric(rt, input, reference);
+        roctxRangePop();
+    }
     auto end = std::chrono::high_resolution_clock::now();
     std::cout << "FSIM: " << result.fsim << std::endl << "FSIMc: " << result.fsimc << std::endl;
-    if (args.verbose || repeat > 1)
-        print_timing("FSIM", std::chrono::duration<double, std::milli>(end - start).count() / repeat, repeat);
+    if (args.verbose) result.timestamps.print(start, end);
+    if (repeat > 1)
+        print_timing("FSIM", std::chrono::duration<double, std::milli>(end - begin).count() / repeat, repeat);
     if (args.outputPath.has_value()) {
         auto converted = convertFloatToChar(result.simc);
         if (!iqmcli::write_png_gray(args.outputPath.value(), result.wd, result.hd, converted.data()))

@@ -46,6 +46,13 @@ Ctx::~Ctx() {
     }
     for (auto e : free_events_) (void)hipEventDestroy(e);
     if (pinned_) (void)hipHostFree(pinned_);
+    if (copy_stream_) (void)hipStreamSynchronize(copy_stream_);
+    for (int i = 0; i < 2; ++i) {
+        if (h2d_done_[i]) (void)hipEventDestroy(h2d_done_[i]);
+        if (comp_done_[i]) (void)hipEventDestroy(comp_done_[i]);
+    }
+    if (stage_) (void)hipHostFree(stage_);
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -306,18 +313,26 @@ int iqm_ssim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref,
         iqm_ssim_default_params(&p);
         if (params) p = *params;
         iqm::Ctx& c = *ctx->c;
+        roctxRangePushA("iqm_ssim");
+        c.begin_marks();
         uint8_t *da, *db;
         upload(c, iqm::WS_IN_IMG, img, &da);
         upload(c, iqm::WS_IN_REF, ref, &db);
+        c.mark("start GPU pipeline"); /* ssim.cpp:78 (after the upload here) */
         const long N = (long)img->width * img->height;
         float* dmap = map_out ? (float*)c.workspace(iqm::WS_MAPS, sizeof(float) * N) : nullptr;
-        double ms = 0.0;
-        c.ssim_batch(da, db, img->width, img->height, img->channels, 0, 1, p.kernel_size, p.k1,
-                     p.k2, p.sigma, dmap, &ms);
+        double* d_ms = c.ssim_batch(da, db, img->width, img->height, img->channels, 0, 1, p.kernel_size, p.k1,
+                                    p.k2, p.sigma, dmap, nullptr);
+        c.mark("end GPU pipeline"); /* ssim.cpp:227 */
         if (map_out) {
             IQM_HIP(hipMemcpyAsync(map_out, dmap, sizeof(float) * N, hipMemcpyDeviceToHost, c.stream()));
         }
+        c.mark("end copy from GPU"); /* ssim.cpp:232 */
+        double ms = 0.0;
+        IQM_HIP(hipMemcpyAsync(&ms, d_ms, sizeof(double), hipMemcpyDeviceToHost, c.stream()));
         c.sync();
+        c.mark("end MSSIM compute"); /* ssim.cpp:235: the cropped mean, computed on the device */
+        roctxRangePop();
         if (mssim_out) *mssim_out = ms;
     });
 }
@@ -369,11 +384,15 @@ int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* f
     return guarded([&] {
         check_pair(img, ref);
         iqm::Ctx& c = *ctx->c;
+        roctxRangePushA("iqm_fsim");
+        c.begin_marks();
+        int F, wd, hd;
+        iqm_fsim_dims(img->width, img->height, &F, &wd, &hd);
+        c.mark("downscale factor computed"); /* fsim.cpp:85 */
         uint8_t *da, *db;
         upload(c, iqm::WS_IN_IMG, img, &da);
         upload(c, iqm::WS_IN_REF, ref, &db);
-        int F, wd, hd;
-        iqm_fsim_dims(img->width, img->height, &F, &wd, &hd);
+        c.mark("images sent to gpu"); /* fsim.cpp:89 */
         const long N = (long)wd * hd;
         iqm::FsimBatchOut out;
         out.want_grad = detail != nullptr;
@@ -418,6 +437,8 @@ int iqm_fsim(iqm_ctx* ctx, const iqm_image* img, const iqm_image* ref, double* f
             IQM_HIP(hipMemcpyAsync(stats, L.stats, sizeof(stats), hipMemcpyDeviceToHost, c.stream()));
         }
         c.sync();
+        c.mark("FSIM, FSIMc computed"); /* fsim.cpp:155 */
+        roctxRangePop();
         if (fsim_out) *fsim_out = res[3];
         if (fsimc_out) *fsimc_out = res[4];
         if (detail) {
@@ -473,6 +494,39 @@ int iqm_fsim_batch_dev_async(iqm_ctx* ctx, const uint8_t* d_img, const uint8_t* 
         out.d_simc = d_simc_maps;
         const double* dres = c.fsim_batch_graph(d_img, d_ref, w, h, ch, pair_stride, n, out);
         c.fsim_scores(dres, n, d_scores);
+    });
+}
+
+int iqm_fsim_batch_host(iqm_ctx* ctx, const uint8_t* h_img, const uint8_t* h_ref, int32_t w, int32_t h,
+                        int32_t ch, int64_t pair_stride, int32_t n, int32_t chunk, double* fsim_out,
+                        double* fsimc_out) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(h_img && h_ref && w > 0 && h > 0 && n > 0 && chunk > 0, IQM_ERR_INVALID,
+                  "invalid batch arguments");
+        IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
+        IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
+        ctx->c->fsim_host_stream(h_img, h_ref, w, h, ch, pair_stride, n, chunk, fsim_out, fsimc_out);
+    });
+}
+
+int iqm_host_alloc(iqm_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(IQM_ERR_INVALID, "null argument");
+    return guarded([&] {
+        IQM_HIP(hipSetDevice(ctx->c->device()));
+        if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            throw iqm::Error(IQM_ERR_NOMEM, "pinned host allocation failed");
+        }
+    });
+}
+
+int iqm_host_free(iqm_ctx* ctx, void* p) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        ctx->c->sync();
+        IQM_HIP(hipHostFree(p));
     });
 }
 
@@ -819,6 +873,21 @@ int iqm_graph_cache_sizes(iqm_ctx* ctx, int32_t* same_size, int32_t* mixed) {
 int iqm_set_graphs(iqm_ctx* ctx, int32_t on) {
     if (!ctx) return fail(IQM_ERR_INVALID, "null context");
     return guarded([&] { ctx->c->set_graphs(on != 0); });
+}
+
+int iqm_set_marks(iqm_ctx* ctx, int32_t on) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] { ctx->c->set_marks(on != 0); });
+}
+
+int32_t iqm_marks(iqm_ctx* ctx, int32_t max, const char** names, int64_t* ns) {
+    if (!ctx) return -1;
+    const auto& m = ctx->c->marks();
+    for (int32_t i = 0; i < max && i < (int32_t)m.size(); ++i) {
+        if (names) names[i] = m[i].first.c_str();
+        if (ns) ns[i] = m[i].second;
+    }
+    return (int32_t)m.size();
 }
 
 int iqm_set_stage_timing(iqm_ctx* ctx, int32_t on) {

@@ -6,6 +6,10 @@
 #include "common.hpp"
 #include "fft.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <chrono>
+
 #include <functional>
 #include <map>
 #include <tuple>
@@ -174,6 +178,14 @@ class Ctx {
     FsimLayout fsim_layout(long N, int n, char* base) const;
     char* fsim_ws_base() { return fsim_ws_; }
 
+    /* FSIM over n host-resident pairs (pair_stride bytes apart): chunks of
+     * `chunk` pairs are copied to the device on a copy stream while the
+     * previous chunk is scored on the context's stream (two device slots;
+     * pageable inputs pass through two pinned staging slots).  Scores to the
+     * host arrays (nullable). */
+    void fsim_host_stream(const uint8_t* h_img, const uint8_t* h_ref, int w, int h, int ch, long pair_stride,
+                          int n, int chunk, double* fsim_out, double* fsimc_out);
+
     /* Seeded synthetic RGB images on the device (recipe of iqm_amd/synth.py):
      * the colour-correlated 1/f^2 reference of w x h, and the distortion of a
      * w x h RGB view (row_stride bytes) into packed rows. */
@@ -221,7 +233,9 @@ class Ctx {
             e1 = take_event();
             IQM_HIP(hipEventRecord(e0, stream_));
         }
+        roctxRangePushA(tag); /* names the launch for rocprofv3 --marker-trace */
         hipLaunchKernelGGL(k, grid, block, shmem, stream_, args...);
+        roctxRangePop();
         IQM_HIP(hipGetLastError());
         if (timing_) {
             IQM_HIP(hipEventRecord(e1, stream_));
@@ -248,6 +262,20 @@ class Ctx {
     void import_tunes(const std::string& json);
     bool graphs() const { return graphs_; }
 
+    /* Host time marks at stage boundaries (the reference's Timestamps,
+     * src/timestamps.h:18-51): when on, mark() waits for the stream and
+     * records (name, high_resolution_clock nanoseconds); begin_marks() starts
+     * the list of a new top-level call.  Off: mark() costs nothing. */
+    void set_marks(bool on) { marks_ = on; }
+    void begin_marks() { mark_list_.clear(); }
+    void mark(const char* name) {
+        if (!marks_ || capturing_) return;
+        IQM_HIP(hipStreamSynchronize(stream_));
+        const auto t = std::chrono::high_resolution_clock::now().time_since_epoch();
+        mark_list_.emplace_back(name, (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t).count());
+    }
+    const std::vector<std::pair<std::string, int64_t>>& marks() const { return mark_list_; }
+
     void set_timing(bool on);
     int stage_times(int max, const char** names, double* ms, int64_t* launches);
     void reset_times();
@@ -255,6 +283,8 @@ class Ctx {
    private:
     hipEvent_t take_event();
     void collect_timing();
+    void ensure_copy_stream();
+    void* host_staging(size_t bytes);
 
     int device_ = 0;
     size_t lds_per_block_ = 65536;
@@ -264,6 +294,12 @@ class Ctx {
     std::map<int, std::pair<void*, size_t>> ws_;
     void* pinned_ = nullptr; /* hipHostMalloc'd result staging */
     size_t pinned_bytes_ = 0;
+    /* input pipeline of fsim_host_stream */
+    hipStream_t copy_stream_ = nullptr;
+    hipEvent_t h2d_done_[2] = {nullptr, nullptr};
+    hipEvent_t comp_done_[2] = {nullptr, nullptr};
+    void* stage_ = nullptr; /* pinned staging of pageable inputs */
+    size_t stage_bytes_ = 0;
     std::map<int, LinePlanHost> line_plans_;
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
     std::map<std::pair<int, int>, FsimTune> tunes_;
@@ -316,6 +352,8 @@ class Ctx {
     bool graphs_ = true;
     bool capturing_ = false;
 
+    bool marks_ = false;
+    std::vector<std::pair<std::string, int64_t>> mark_list_;
     bool timing_ = false;
     std::vector<hipEvent_t> free_events_;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_;
@@ -334,7 +372,7 @@ void ssim_noalloc_setup(int device);
 
 /* workspace slots */
 enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR, WS_GEN, WS_BATCH,
-       WS_MIX, WS_MIXTAB };
+       WS_MIX, WS_MIXTAB, WS_STREAM, WS_STREAM_SCORES };
 
 size_t line_lds_bytes(const LinePlan& P, int lines, int bufs);
 int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap);

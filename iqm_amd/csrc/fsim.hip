@@ -1318,6 +1318,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const FsimSizePlan& plan = fsim_plan(wd, hd);
     const LinePlan& Px = line_plan(wd).plan;
     const LinePlan& Py = line_plan(hd).plan;
+    mark("FFT library initialized"); /* fsim.cpp:95: the per-size plans are ready */
     const long N = (long)wd * hd;
     const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);
     const FsimTune tune = fsim_tune(wd, hd);
@@ -1382,6 +1383,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
                    invN);
         }
     }
+    mark("pre median work recorded"); /* fsim.cpp:138 */
     /* F13 + thresholds */
     NoiseArgs na;
     for (int i = 0; i < 4; ++i) na.em_n[i] = plan.em_n[i];
@@ -1389,6 +1391,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), sizeof(unsigned) * kSelectCap,
            (const float*)keys,
            (const unsigned*)hist, thr, stats, N, na);
+    mark("noise powers computed"); /* fsim.cpp:141 */
     /* F15 + F16 + F17 */
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
            dim3(kBlock), 0, (const float*)yiq,

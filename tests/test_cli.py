@@ -95,9 +95,11 @@ def test_methods_outside_this_build(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_metrics_match_library(tmp_path, ctx):
-    import iqm_amd
+def test_cli_metrics_match_oracle(tmp_path):
+    """The CLI's stdout lines and map PNG against the FP64 oracle on the same
+    decoded pixels (the CLI prints floats with 6 significant digits)."""
     from iqm_amd import synth
+    from oracle import oracle
 
     img, ref = synth.rgb_pair(400, 300, seed=33)
     write_png(tmp_path / "a.png", img)
@@ -105,14 +107,46 @@ def test_cli_metrics_match_library(tmp_path, ctx):
     rc, out, err = run("--method", "SSIM", "--input", str(tmp_path / "a.png"), "--ref",
                        str(tmp_path / "b.png"), "--output", str(tmp_path / "m.png"))
     assert rc == 0, err
-    s = iqm_amd.SSIM().compute_metric(ctx, img, ref)
-    assert out.startswith("MSSIM: ") and abs(float(out.split()[1]) - s.mssim) < 1e-5
+    ms, smap = oracle.ssim(img, ref, prec=64)
+    assert out.startswith("MSSIM: ") and abs(float(out.split()[1]) - ms) < 1e-5
     m = read_png_gray(tmp_path / "m.png")
-    expect = (np.clip(s.image_data, 0, 1) * np.float32(255)).astype(np.uint8)
-    assert m.shape == (300, 400) and np.max(np.abs(m.astype(int) - expect.astype(int))) <= 1
-    rc, out, err = run("--method", "FSIM", "--input", str(tmp_path / "a.png"), "--ref", str(tmp_path / "b.png"))
+    expect = np.clip(smap, 0, 1) * 255.0
+    # the map is truncated to u8; the FP32 map is within 1e-4 of FP64, so the
+    # byte can differ from the FP64 truncation only where it sits on an edge
+    d = m.astype(np.float64) - np.floor(expect)
+    assert m.shape == (300, 400) and np.all(np.abs(d) <= 1)
+    assert np.mean(d != 0) < 1e-3
+    rc, out, err = run("--method", "FSIM", "--input", str(tmp_path / "a.png"), "--ref", str(tmp_path / "b.png"),
+                       "--output", str(tmp_path / "f.png"))
     assert rc == 0, err
-    f = iqm_amd.FSIM().compute_metric(ctx, img, ref)
+    o = oracle.fsim(img, synth.rgba(ref)[:, :, :3], prec=64, maps=("simc",))
     lines = out.strip().splitlines()
     assert lines[0].startswith("FSIM: ") and lines[1].startswith("FSIMc: ")
-    assert abs(float(lines[0].split()[1]) - f.fsim) < 1e-5 and abs(float(lines[1].split()[1]) - f.fsimc) < 1e-5
+    assert abs(float(lines[0].split()[1]) - o.fsim) < 1e-5 and abs(float(lines[1].split()[1]) - o.fsimc) < 1e-5
+    fm = read_png_gray(tmp_path / "f.png")
+    de = fm.astype(np.float64) - np.floor(np.clip(o.maps["simc"], 0, 1) * 255.0)
+    assert fm.shape == o.maps["simc"].shape and np.all(np.abs(de) <= 1)
+
+
+@pytest.mark.gpu
+def test_cli_verbose_timestamps_and_repeat(tmp_path):
+    """-v prints the reference's Timestamps table (src/timestamps.h:18-51) with
+    the stage names of src/gpu/fsim/fsim.cpp:85-155 and ssim.cpp:78-235;
+    --repeat N prints the mean time of N evaluations."""
+    from iqm_amd import synth
+
+    img, ref = synth.rgb_pair(320, 240, seed=34)
+    write_png(tmp_path / "a.png", img)
+    write_png(tmp_path / "b.png", ref)
+    rc, out, err = run("--method", "FSIM", "--input", str(tmp_path / "a.png"), "--ref", str(tmp_path / "b.png"), "-v")
+    assert rc == 0, err
+    names = [ln.split(":")[0].strip() for ln in out.splitlines() if "us |" in ln]
+    assert names == ["downscale factor computed", "images sent to gpu", "FFT library initialized",
+                     "pre median work recorded", "noise powers computed", "FSIM, FSIMc computed"], out
+    assert any(ln.strip().startswith("TOTAL:") for ln in out.splitlines())
+    rc, out, err = run("--method", "SSIM", "--input", str(tmp_path / "a.png"), "--ref", str(tmp_path / "b.png"), "-v",
+                       "--repeat", "3")
+    assert rc == 0, err
+    names = [ln.split(":")[0].strip() for ln in out.splitlines() if "us |" in ln]
+    assert names == ["start GPU pipeline", "end GPU pipeline", "end copy from GPU", "end MSSIM compute"], out
+    assert "mean time over 3 runs" in out
