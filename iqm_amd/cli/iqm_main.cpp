@@ -24,7 +24,7 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "iqm/iqm.hpp"
-#include "png_io.hpp"
+#include "image_io.hpp"
 
 namespace IQM {
 

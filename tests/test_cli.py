@@ -129,6 +129,35 @@ def test_cli_metrics_match_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libstb_ref.so")),
+                    reason="reference image loader not built")
+def test_cli_jpeg_input_matches_oracle(tmp_path):
+    """A JPEG pair (4:2:0 and progressive): the CLI decodes it as the reference's
+    stbi_load does (oracle/_ref), and its scores equal the FP64 oracle's on
+    those pixels."""
+    from PIL import Image
+
+    import test_image_decode as tid
+    from iqm_amd import synth
+    from oracle import oracle
+
+    img, ref = synth.rgb_pair(320, 240, seed=36)
+    Image.fromarray(img).save(tmp_path / "a.jpg", quality=80, subsampling="4:2:0")
+    Image.fromarray(ref).save(tmp_path / "b.jpg", quality=92, progressive=True)
+    a = tid.stb_load(tmp_path / "a.jpg")[:, :, :3]
+    b = tid.stb_load(tmp_path / "b.jpg")[:, :, :3]
+    rc, out, err = run("--method", "FSIM", "--input", str(tmp_path / "a.jpg"), "--ref", str(tmp_path / "b.jpg"))
+    assert rc == 0, err
+    o = oracle.fsim(a, b, prec=64)
+    lines = out.strip().splitlines()
+    assert abs(float(lines[0].split()[1]) - o.fsim) < 1e-5 and abs(float(lines[1].split()[1]) - o.fsimc) < 1e-5
+    rc, out, err = run("--method", "SSIM", "--input", str(tmp_path / "a.jpg"), "--ref", str(tmp_path / "b.jpg"))
+    assert rc == 0, err
+    ms, _ = oracle.ssim(a, b, prec=64)
+    assert abs(float(out.split()[1]) - ms) < 1e-5
+
+
+@pytest.mark.gpu
 def test_cli_verbose_timestamps_and_repeat(tmp_path):
     """-v prints the reference's Timestamps table (src/timestamps.h:18-51) with
     the stage names of src/gpu/fsim/fsim.cpp:85-155 and ssim.cpp:78-235;
