@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #ifdef _OPENMP
@@ -451,6 +452,34 @@ void noise_consts(int wd, int hd, const std::vector<T>& H, double em_n[4], doubl
     }
 }
 
+/* Per-size constants (the filter bank and the F10/F14 noise terms depend only
+ * on (wd, hd)), kept across calls: computed once per size and precision,
+ * exactly as a fresh computation would give them.  A few sizes are kept. */
+template <class T>
+struct SizeConsts {
+    int wd = 0, hd = 0;
+    std::vector<T> H;
+    double em_n[4] = {0, 0, 0, 0};
+    double est[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+template <class T>
+std::shared_ptr<const SizeConsts<T>> size_consts(int wd, int hd) {
+    static std::mutex mu;
+    static std::vector<std::shared_ptr<const SizeConsts<T>>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& c : cache)
+        if (c->wd == wd && c->hd == hd) return c;
+    auto c = std::make_shared<SizeConsts<T>>();
+    c->wd = wd;
+    c->hd = hd;
+    build_bank<T>(wd, hd, c->H);
+    noise_consts<T>(wd, hd, c->H, c->em_n, c->est);
+    if (cache.size() >= 8) cache.erase(cache.begin());
+    cache.push_back(c);
+    return c;
+}
+
 template <class T>
 int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim_out* out) {
     int F, wd, hd;
@@ -500,6 +529,7 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
     for (int im = 0; im < 2; ++im) {
         grad[im].assign(N, 0);
         const T* Y = &yiq[im][2 * N];
+#pragma omp parallel for schedule(static)
         for (int y = 0; y < hd; ++y) {
             for (int x = 0; x < wd; ++x) {
                 T vs = 0, hs = 0;
@@ -520,12 +550,11 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
         }
     }
 
-    /* F6-F9: filter bank H_z, z = 4o + s */
-    std::vector<T> H;
-    build_bank<T>(wd, hd, H);
-
-    /* F10 + F14 */
-    noise_consts<T>(wd, hd, H, out->em_n, out->est);
+    /* F6-F9: filter bank H_z, z = 4o + s; F10 + F14 (per-size constants) */
+    const auto consts = size_consts<T>(wd, hd);
+    const std::vector<T>& H = consts->H;
+    for (int i = 0; i < 4; ++i) out->em_n[i] = consts->em_n[i];
+    for (int i = 0; i < 8; ++i) out->est[i] = consts->est[i];
 
     /* F4-F5: (Y, 0) per image, forward 2-D FFT, unnormalised */
     std::vector<Cx<T>> spec[2];
@@ -539,17 +568,23 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
      * scales' responses EO = ifft2(H * F) (normalised), then
      * fsim_sum_filter_responses.glsl:33-83 */
     std::vector<T> An(8 * N), En(8 * N), keys(8 * N);
+    /* the 32 (image, orientation, scale) responses are independent: one task
+     * each (a transform inside runs on its task's thread) */
+    std::vector<std::vector<Cx<T>>> resp(32);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int task = 0; task < 32; ++task) {
+        const int im = task / 16, o = (task / 4) % 4, s = task % 4;
+        const T* hz = &H[(long)(4 * o + s) * N];
+        std::vector<Cx<T>>& eo = resp[task];
+        eo.resize(N);
+        for (long i = 0; i < N; ++i) eo[i] = {hz[i] * spec[im][i].re, hz[i] * spec[im][i].im};
+        fft2<T>(eo.data(), wd, hd, +1, true);
+    }
     for (int im = 0; im < 2; ++im) {
         for (int o = 0; o < 4; ++o) {
-            std::vector<Cx<T>> EO[4];
-            for (int s = 0; s < 4; ++s) {
-                const T* hz = &H[(long)(4 * o + s) * N];
-                EO[s].resize(N);
-                for (long i = 0; i < N; ++i)
-                    EO[s][i] = {hz[i] * spec[im][i].re, hz[i] * spec[im][i].im};
-                fft2<T>(EO[s].data(), wd, hd, +1, true);
-            }
+            const std::vector<Cx<T>>* EO = &resp[im * 16 + o * 4];
             const int io = im * 4 + o;
+#pragma omp parallel for schedule(static)
             for (long i = 0; i < N; ++i) {
                 T amp = 0, sr = 0, si = 0;
                 for (int s = 0; s < 4; ++s) {
@@ -600,6 +635,7 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
     std::vector<T> pc[2];
     for (int im = 0; im < 2; ++im) {
         pc[im].assign(N, 0);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < N; ++i) {
             T et = 0, at = 0;
             for (int o = 0; o < 4; ++o) {
@@ -616,6 +652,7 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
     const T T1 = (T)0.85, T2 = (T)160, T3 = (T)200, T4 = (T)200, lambda = (T)0.03;
     std::vector<T> pcmv(N), simv(N), simcv(N);
     double spcm = 0, ssim = 0, ssimc = 0;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < N; ++i) {
         T p1 = pc[0][i], p2 = pc[1][i];
         T pcm = std::max(p1, p2);
@@ -631,9 +668,12 @@ int fsim_impl(const uint8_t* a, const uint8_t* b, int w, int h, int ch, orc_fsim
         pcmv[i] = pcm;
         simv[i] = sim;
         simcv[i] = simc;
-        spcm += (double)pcm;
-        ssim += (double)sim;
-        ssimc += (double)simc;
+    }
+    /* sums in index order (the same order at any thread count) */
+    for (long i = 0; i < N; ++i) {
+        spcm += (double)pcmv[i];
+        ssim += (double)simv[i];
+        ssimc += (double)simcv[i];
     }
     /* F17: fsim_final_multiply.cpp:294-300 -- sums over every pixel */
     out->sums[0] = spcm;
@@ -738,6 +778,20 @@ int orc_fsim(int prec, const uint8_t* img, const uint8_t* ref, int w, int h, int
     if (!valid_image_args(img, ref, w, h, ch) || !out) return -1;
     if (prec == 64) return fsim_impl<double>(img, ref, w, h, ch, out);
     return fsim_impl<float>(img, ref, w, h, ch, out);
+}
+
+int orc_fsim_batch(int prec, int n, const uint8_t* const* imgs, const uint8_t* const* refs, int w, int h, int ch,
+                   orc_fsim_out* outs) {
+    if (n <= 0 || !imgs || !refs || !outs) return -1;
+    for (int i = 0; i < n; ++i)
+        if (!valid_image_args(imgs[i], refs[i], w, h, ch)) return -1;
+    int rc = 0;
+    /* one pair per thread; each pair's own parallel loops run on that thread */
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : rc)
+    for (int i = 0; i < n; ++i)
+        rc |= prec == 64 ? fsim_impl<double>(imgs[i], refs[i], w, h, ch, &outs[i])
+                         : fsim_impl<float>(imgs[i], refs[i], w, h, ch, &outs[i]);
+    return rc;
 }
 
 int orc_fsim_filters(int prec, int wd, int hd, double* lowpass, double* loggabor, double* angular) {

@@ -66,6 +66,10 @@ typedef struct {
 
 int orc_fsim(int prec, const uint8_t* img, const uint8_t* ref, int w, int h, int ch,
              orc_fsim_out* out);
+/* orc_fsim over n pairs of one size (arrays of n image pointers and n
+ * outputs), one pair per thread: the CPU baseline's throughput form. */
+int orc_fsim_batch(int prec, int n, const uint8_t* const* imgs, const uint8_t* const* refs, int w, int h, int ch,
+                   orc_fsim_out* outs);
 
 /* Frequency-domain filters on a wd x hd grid (fsim_lowpassfilter.glsl,
  * fsim_log_gabor.glsl, fsim_angular.glsl).  loggabor/angular hold 4 planes. */

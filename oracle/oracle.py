@@ -163,6 +163,24 @@ def fsim(img, ref, prec=64, maps=("simc",)):
         sums=np.array(out.sums[:]), maps=keep)
 
 
+def fsim_batch(pairs, prec=32):
+    """(fsim, fsimc) of same-size pairs, one pair per OpenMP thread (the CPU
+    baseline's throughput form); pairs: list of (img, ref) uint8 arrays."""
+    imgs = [_img(a) for a, _ in pairs]
+    refs = [_img(b) for _, b in pairs]
+    h, w, ch = imgs[0].shape
+    n = len(pairs)
+    L = lib()
+    L.orc_fsim_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    pi = (ctypes.c_void_p * n)(*[a.ctypes.data for a in imgs])
+    pr = (ctypes.c_void_p * n)(*[b.ctypes.data for b in refs])
+    outs = (_FsimOut * n)()
+    if L.orc_fsim_batch(prec, n, pi, pr, w, h, ch, outs) != 0:
+        raise ValueError("orc_fsim_batch failed")
+    return np.array([[o.fsim, o.fsimc] for o in outs])
+
+
 def fsim_filters(wd, hd, prec=64):
     lp = np.empty((hd, wd), np.float64)
     lg = np.empty((4, hd, wd), np.float64)
