@@ -239,12 +239,25 @@ int iqm_sur_load(iqm_sur* s, const uint8_t* buf, int64_t len);
 /* FSIM/FSIMc as iqm_fsim_batch_dev with PC from the network.  Pairs with
  * i % spot_every == 0 (spot_every > 0) are also run exactly; spot_err gets the
  * largest |PC_net - PC_exact| over 4 seeded 32x32 tiles of their images, and
- * when it exceeds spot_tol the batch is recomputed exactly (*fell_back = 1).
- * spot_err and fell_back are nullable. */
+ * when it is not within spot_tol the batch is recomputed exactly
+ * (*fell_back = 1).  The verdict is taken on the device: the exact kernels
+ * are enqueued gated by it, so a batch has no host round trip before its
+ * results, and a repeated batch (same pointers, sizes and spot settings)
+ * replays one captured graph.  A downscaled size whose spot check failed is
+ * remembered by the surrogate: later spot-checked batches of that size skip
+ * the network and run exactly (fell_back = 1, spot_err = the failing error);
+ * iqm_sur_set_failure_memory(s, 0) turns that off.  spot_err and fell_back
+ * are nullable.
+ * Cost with the committed 6-layer network at 1920x1080: about 0.6x the exact
+ * path's pairs/s without a spot check; a failing check costs network + spot
+ * pairs + the exact batch once, then exact speed. */
 int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img, const uint8_t* d_ref,
                                  int32_t w, int32_t h, int32_t ch, int64_t pair_stride, int32_t n,
                                  int32_t spot_every, float spot_tol, double* fsim_out,
                                  double* fsimc_out, float* spot_err, int32_t* fell_back);
+/* Failure memory of iqm_fsim_surrogate_batch_dev (default on); setting it
+ * clears what was remembered. */
+int iqm_sur_set_failure_memory(iqm_sur* s, int32_t on);
 
 /* ---- seeded synthetic workloads on the device -------------------------- */
 /* No counterpart in the reference: the benchmark inputs of iqm_amd/synth.py

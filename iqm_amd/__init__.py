@@ -41,7 +41,7 @@ EXPORTS = (
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
     "iqm_batch_dev", "iqm_synth_reference_dev", "iqm_ssim_batch_dev_noalloc",
     "iqm_graph_cache_sizes", "iqm_fsim_batch_host", "iqm_host_alloc", "iqm_host_free",
-    "iqm_set_marks", "iqm_marks",
+    "iqm_set_marks", "iqm_marks", "iqm_sur_set_failure_memory",
 )
 
 JOB_SSIM = 1
@@ -161,6 +161,7 @@ def lib():
     L.iqm_sur_load.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64]
     L.iqm_fsim_surrogate_batch_dev.argtypes = [vp, vp, vp, vp, i32, i32, i32, i64, i32, i32,
                                                ctypes.c_float, dp, dp, fp, ctypes.POINTER(i32)]
+    L.iqm_sur_set_failure_memory.argtypes = [vp, i32]
     L.iqm_batch_dev.argtypes = [vp, ctypes.POINTER(PairJob), i32, ctypes.POINTER(SsimParams),
                                 ctypes.POINTER(PairResult)]
     L.iqm_synth_pair_dev.argtypes = [vp, i32, i32, u64, vp, vp]
@@ -566,6 +567,11 @@ class Surrogate:
     def load(self, blob: bytes) -> None:
         buf = (ctypes.c_uint8 * len(blob)).from_buffer_copy(blob)
         _check(lib().iqm_sur_load(self.handle, buf, len(blob)))
+
+    def set_failure_memory(self, on: bool) -> None:
+        """Whether a size whose spot check failed goes straight to exact PC
+        afterwards (default on); setting it clears the memory."""
+        _check(lib().iqm_sur_set_failure_memory(self.handle, 1 if on else 0))
 
     def fsim_batch_dev(self, d_img: int, d_ref: int, w: int, h: int, ch: int, pair_stride: int,
                        n: int, spot_every: int = 0, spot_tol: float = 0.02):

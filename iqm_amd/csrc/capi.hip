@@ -107,6 +107,7 @@ void Ctx::release_graph(GraphEntry& e) {
 }
 
 void Ctx::drop_graphs() {
+    ++graph_gen_;
     for (auto& kv : graphs_cache_) release_graph(kv.second);
     graphs_cache_.clear();
     for (auto& kv : mixed_graphs_) release_graph(kv.second);
@@ -115,7 +116,7 @@ void Ctx::drop_graphs() {
 
 double* Ctx::fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                               long pair_stride, int n, const FsimBatchOut& out) {
-    if (!graphs_ || timing_) return fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
+    if (!graphs_ || timing_ || gate_ || capturing_) return fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, out);
     GraphKey key{d_img, d_ref, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, pair_stride, w, h, ch, n};
     auto it = graphs_cache_.find(key);
     if (it == graphs_cache_.end()) {
@@ -720,19 +721,14 @@ int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img,
                   "invalid batch arguments");
         IQM_CHECK(ch == 1 || ch == 3 || ch == 4, IQM_ERR_INVALID, "channels must be 1, 3 or 4");
         IQM_CHECK(n == 1 || pair_stride >= (int64_t)w * h * ch, IQM_ERR_INVALID, "pair_stride too small");
-        iqm::Ctx& c = *ctx->c;
-        int back = 0;
-        double* dres = s->s->fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, spot_tol,
-                                        spot_err, &back);
-        if (fell_back) *fell_back = back;
-        std::vector<double> res((size_t)8 * n);
-        IQM_HIP(hipMemcpyAsync(res.data(), dres, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c.stream()));
-        c.sync();
-        for (int i = 0; i < n; ++i) {
-            if (fsim_out) fsim_out[i] = res[(size_t)8 * i + 3];
-            if (fsimc_out) fsimc_out[i] = res[(size_t)8 * i + 4];
-        }
+        s->s->fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, spot_tol, fsim_out, fsimc_out,
+                         spot_err, fell_back);
     });
+}
+
+int iqm_sur_set_failure_memory(iqm_sur* s, int32_t on) {
+    SUR_GUARD(s);
+    return guarded([&] { s->s->set_remember(on != 0); });
 }
 
 int iqm_fft2d(iqm_ctx* ctx, float* data, int32_t w, int32_t h, int32_t batch, int32_t dir,

@@ -78,6 +78,12 @@ __device__ __forceinline__ T* at32(T* base, unsigned i) {
     return (T*)((char*)base + i * (unsigned)sizeof(T));
 }
 
+/* Device-side launch gate: a kernel given a gate returns at once when the
+ * flag reads 0 (a decision an earlier kernel of the stream wrote), so a
+ * captured launch sequence can skip work without a host round trip.  A null
+ * gate means always run. */
+__device__ __forceinline__ bool gate_closed(const int* gate) { return gate != nullptr && *gate == 0; }
+
 constexpr int kMaxLdsBytes = 65536;  // per-workgroup LDS budget used by every kernel
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

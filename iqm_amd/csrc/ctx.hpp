@@ -243,11 +243,22 @@ class Ctx {
         }
     }
 
+    /* Launch gate of the exact FSIM kernels (see gate_closed): while set,
+     * every kernel fsim_batch / fsim_batch_pcfn enqueues checks the flag. */
+    void set_gate(const int* g) { gate_ = g; }
+    const int* gate() const { return gate_; }
+    /* bumped whenever the captured graphs are dropped (workspaces moved) */
+    uint64_t graph_generation() const { return graph_gen_; }
+
     /* Replays the launch sequence of a repeated fsim_batch call (same
      * arguments) as a captured hipGraph.  Returns the result array. */
     double* fsim_batch_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
                              long pair_stride, int n, const FsimBatchOut& out);
     void set_graphs(bool on) { graphs_ = on; }
+    /* marks a stream capture another component runs on this context's
+     * stream (workspace growth is refused while it is on) */
+    void set_capturing(bool on) { capturing_ = on; }
+    bool timing() const { return timing_; }
 
     /* Tile table: tuned entries per (wd, hd); defaults otherwise. */
     FsimTune fsim_tune(int wd, int hd) const;
@@ -325,6 +336,8 @@ class Ctx {
     std::map<GraphKey, GraphEntry> graphs_cache_;
     std::map<std::string, GraphEntry> mixed_graphs_;
     uint64_t graph_tick_ = 0;
+    uint64_t graph_gen_ = 0;
+    const int* gate_ = nullptr;
     void drop_graphs();
     void release_graph(GraphEntry& e);
     /* makes room for one more entry: evicts the least recently launched

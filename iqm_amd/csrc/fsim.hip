@@ -481,7 +481,9 @@ template <int FT>
 __global__ __launch_bounds__(kBlock) void fsim_prep2_kernel(const uint8_t* __restrict__ img,
                                                             const uint8_t* __restrict__ ref, long pair_stride,
                                                             int w, int h, int ch, int wd, int hd, int cols,
-                                                            float* __restrict__ yiq, unsigned* __restrict__ hist) {
+                                                            float* __restrict__ yiq, unsigned* __restrict__ hist,
+                                                            const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_prep2_body<FT>(img, ref, pair_stride, w, h, ch, wd, hd, cols, yiq, hist, blockIdx.x, blockIdx.y,
                         blockIdx.z);
 }
@@ -493,7 +495,9 @@ __global__ __launch_bounds__(kBlock) void fsim_prep_kernel(const uint8_t* __rest
                                                            long pair_stride, int w, int h,
                                                            int ch, int F, int wd, int hd, int cols, int prb,
                                                            float* __restrict__ yiq,
-                                                           unsigned* __restrict__ hist) {
+                                                           unsigned* __restrict__ hist,
+                                                           const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_prep_body<FT>(img, ref, pair_stride, w, h, ch, F, wd, hd, cols, prb, yiq, hist, blockIdx.x, blockIdx.y,
                        blockIdx.z);
 }
@@ -534,7 +538,8 @@ __device__ __forceinline__ void fsim_rowfft_body(const float* __restrict__ yiq, 
 
 __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __restrict__ yiq,
                                                              float2* __restrict__ A, int wd, int hd,
-                                                             int rb, LinePlan P) {
+                                                             int rb, LinePlan P, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_rowfft_body(yiq, A, wd, hd, rb, P, blockIdx.x, blockIdx.y);
 }
 
@@ -598,7 +603,8 @@ __global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* _
                                                               float2* __restrict__ Tout,
                                                               const float* __restrict__ bank,
                                                               int wd, int hd, int bx, int nbx_pad,
-                                                              LinePlan P) {
+                                                              LinePlan P, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_colpass_body(A, Tout, bank, wd, hd, bx, nbx_pad, P, blockIdx.x);
 }
 
@@ -721,7 +727,8 @@ __global__ __launch_bounds__(kBlockFft) void fsim_rowpass_kernel(const float2* _
                                                               float* __restrict__ keys,
                                                               unsigned* __restrict__ hist,
                                                               int wd, int hd, int rb, LinePlan P,
-                                                              float invN) {
+                                                              float invN, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_rowpass_body(Tin, an_e, keys, hist, wd, hd, rb, P, invN, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -945,7 +952,8 @@ __global__ __launch_bounds__(1024) void fsim_select_kernel(const float* __restri
                                                            const unsigned* __restrict__ hist,
                                                            float* __restrict__ thr,
                                                            double* __restrict__ stats, long N,
-                                                           NoiseArgs na) {
+                                                           NoiseArgs na, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_select_body(keys, hist, thr, stats, N, na, blockIdx.x, blockIdx.y);
 }
 
@@ -1145,13 +1153,16 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
 __global__ __launch_bounds__(kBlock) void fsim_pcfinal_kernel(
     const float* __restrict__ yiq, float* __restrict__ grad, const float* __restrict__ an_e,
     const float* __restrict__ thr, double* __restrict__ partial, int wd, int hd, float* simc_map,
-    float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src) {
+    float* pc_map, float* pcm_map, float* sim_map, const float* __restrict__ pc_src, const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_pcfinal_body(yiq, grad, an_e, thr, partial, wd, hd, simc_map, pc_map, pcm_map, sim_map, pc_src,
                       blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, blockIdx.z);
 }
 
 __global__ __launch_bounds__(kBlock) void fsim_finalize_kernel(const double* __restrict__ partial,
-                                                               int nblk, double* __restrict__ res) {
+                                                               int nblk, double* __restrict__ res,
+                                                               const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     fsim_finalize_body(partial, nblk, res, blockIdx.x);
 }
 
@@ -1282,13 +1293,13 @@ void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stri
         const dim3 g2(ceil_div(wd, c2), hd, n);
         const size_t b2 = prep2_lds_bytes(F, ch, c2);
         switch (F) {
-            case 2: launch("fsim_prep", fsim_prep2_kernel<2>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            case 3: launch("fsim_prep", fsim_prep2_kernel<3>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            case 4: launch("fsim_prep", fsim_prep2_kernel<4>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            case 5: launch("fsim_prep", fsim_prep2_kernel<5>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            case 6: launch("fsim_prep", fsim_prep2_kernel<6>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            case 7: launch("fsim_prep", fsim_prep2_kernel<7>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
-            default: launch("fsim_prep", fsim_prep2_kernel<8>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist); break;
+            case 2: launch("fsim_prep", fsim_prep2_kernel<2>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            case 3: launch("fsim_prep", fsim_prep2_kernel<3>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            case 4: launch("fsim_prep", fsim_prep2_kernel<4>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            case 5: launch("fsim_prep", fsim_prep2_kernel<5>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            case 6: launch("fsim_prep", fsim_prep2_kernel<6>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            case 7: launch("fsim_prep", fsim_prep2_kernel<7>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
+            default: launch("fsim_prep", fsim_prep2_kernel<8>, g2, dim3(kBlock), b2, d_img, d_ref, pair_stride, w, h, ch, wd, hd, c2, yiq, hist, gate_); break;
         }
         return;
     }
@@ -1297,14 +1308,14 @@ void Ctx::launch_prep(const uint8_t* d_img, const uint8_t* d_ref, long pair_stri
 #define IQM_PREP_CASE(f)                                                                                      \
     case f:                                                                                                   \
         launch("fsim_prep", fsim_prep_kernel<f>, grid, dim3(kBlock), bytes, d_img, d_ref, pair_stride, w, h, ch, \
-               F, wd, hd, cols, prb, yiq, hist);                                                              \
+               F, wd, hd, cols, prb, yiq, hist, gate_);                                                       \
         break;
         IQM_PREP_CASE(1) IQM_PREP_CASE(2) IQM_PREP_CASE(3) IQM_PREP_CASE(4)
         IQM_PREP_CASE(5) IQM_PREP_CASE(6) IQM_PREP_CASE(7) IQM_PREP_CASE(8)
 #undef IQM_PREP_CASE
         default:
             launch("fsim_prep", fsim_prep_kernel<0>, grid, dim3(kBlock), bytes, d_img, d_ref, pair_stride, w, h, ch,
-                   F, wd, hd, cols, prb, yiq, hist);
+                   F, wd, hd, cols, prb, yiq, hist, gate_);
     }
 }
 
@@ -1347,11 +1358,11 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const bool tl = env_int("IQM_FSIM_GENERIC", tune.two_level ? 0 : 1) == 0;
     const size_t tl_budget = std::min<size_t>(lds_per_block_, (size_t)env_int("IQM_FSIM_TL_LDS", 160 * 1024));
     if (tl && tl_supported(wd)) {
-        TlGradArgs a{yiq, A, Px.tw, hd, n, env_int("IQM_FSIM_TL_GRB", tune.fft_rb), tl_budget};
+        TlGradArgs a{yiq, A, Px.tw, hd, n, env_int("IQM_FSIM_TL_GRB", tune.fft_rb), tl_budget, gate_};
         tl_rowfft(*this, wd, a);
     } else {
         launch("fsim_rowfft", fsim_rowfft_kernel, dim3(ceil_div(hd, t.grad_rb), n), dim3(kBlock),
-               line_lds_bytes(Px, t.grad_rb, 2), (const float*)yiq, A, wd, hd, t.grad_rb, Px);
+               line_lds_bytes(Px, t.grad_rb, 2), (const float*)yiq, A, wd, hd, t.grad_rb, Px, gate_);
     }
     /* per chunk: column half of F5, F9, F11, then the row half of F11 + F12 */
     const int nbx = ceil_div(wd, t.col_bx);
@@ -1364,23 +1375,24 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
         const int np = std::min(chunk, n - p0);
         const float2* Ach = A + (long)p0 * 2 * N;
         if (tl_col) {
-            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", tune.col_bx), xcds_, tl_budget};
+            TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", tune.col_bx), xcds_, tl_budget,
+                        gate_};
             tl_colpass(*this, hd, a);
         } else {
             launch("fsim_colpass", fsim_colpass_kernel, dim3(nbx_pad * 2 * np), dim3(kBlockFft),
                    line_lds_bytes(Py, t.col_bx, 3), Ach, T, (const float*)plan.bank, wd, hd,
-                   t.col_bx, nbx_pad, Py);
+                   t.col_bx, nbx_pad, Py, gate_);
         }
         if (tl_row) {
             TlRowArgs a{T, an_e + (long)p0 * 2 * kAnePlanes * N, keys + (long)p0 * 8 * N,
                         hist + (long)p0 * 8 * kHistBins, Px.tw, hd, np, env_int("IQM_FSIM_TL_RB", tune.row_rb),
-                        invN, tl_budget};
+                        invN, tl_budget, gate_};
             tl_rowpass(*this, wd, a);
         } else {
             launch("fsim_rowpass", fsim_rowpass_kernel, dim3(ceil_div(hd, t.row_rb), 2, np),
                    dim3(kBlockFft), row_sh, (const float2*)T, an_e + (long)p0 * 2 * kAnePlanes * N,
                    keys + (long)p0 * 8 * N, hist + (long)p0 * 8 * kHistBins, wd, hd, t.row_rb, Px,
-                   invN);
+                   invN, gate_);
         }
     }
     mark("pre median work recorded"); /* fsim.cpp:138 */
@@ -1390,15 +1402,15 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     for (int i = 0; i < 8; ++i) na.est[i] = plan.est[i];
     launch("fsim_select", fsim_select_kernel, dim3(8, n), dim3(1024), sizeof(unsigned) * kSelectCap,
            (const float*)keys,
-           (const unsigned*)hist, thr, stats, N, na);
+           (const unsigned*)hist, thr, stats, N, na, gate_);
     mark("noise powers computed"); /* fsim.cpp:141 */
     /* F15 + F16 + F17 */
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
            dim3(kBlock), 0, (const float*)yiq,
            out.want_grad ? grad : (float*)nullptr, (const float*)an_e, (const float*)thr, partial, wd,
-           hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)nullptr);
+           hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)nullptr, gate_);
     launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial,
-           nblk, res);
+           nblk, res, gate_);
     return res;
 }
 
@@ -1429,8 +1441,9 @@ double* Ctx::fsim_batch_pcfn(const uint8_t* d_img, const uint8_t* d_ref, int w, 
     pc_fn(yiq, N, n, pc);
     launch("fsim_pcfinal", fsim_pcfinal_kernel, dim3(ceil_div(wd, kPcTx), ceil_div(hd, kPcTy), n),
            dim3(kBlock), 0, (const float*)yiq, (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
-           partial, wd, hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)pc);
-    launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial, nblk, res);
+           partial, wd, hd, out.d_simc, out.d_pc, out.d_pcm, out.d_sim, (const float*)pc, gate_);
+    launch("fsim_finalize", fsim_finalize_kernel, dim3(n), dim3(kBlock), 0, (const double*)partial, nblk, res,
+           gate_);
     return res;
 }
 

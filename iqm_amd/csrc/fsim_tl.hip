@@ -20,7 +20,9 @@ __global__ __launch_bounds__(512) void fsim_colpass_tl_kernel(const float2* __re
                                                                float2* __restrict__ Tout,
                                                                const float* __restrict__ bank,
                                                                const float2* __restrict__ twg,
-                                                               int wd, int bx, int nbx_pad) {
+                                                               int wd, int bx, int nbx_pad,
+                                                               const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     constexpr int n = N1 * N2;
     constexpr int ZL = Exch<N1, N2>::line;
     constexpr int LS = n + 1; /* spectrum line pitch (odd) */
@@ -125,7 +127,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) void f
                                                                float* __restrict__ keys,
                                                                unsigned* __restrict__ hist,
                                                                const float2* __restrict__ twg,
-                                                               int hd, int rb, float invN) {
+                                                               int hd, int rb, float invN,
+                                                               const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     constexpr int n = N1 * N2; /* = wd */
     constexpr int ZL = Exch<N1, N2>::line > n + 1 ? Exch<N1, N2>::line : n + 1;
     constexpr int LE = ZL; /* E shares Z's line pitch */
@@ -233,7 +237,9 @@ template <int N1, int N2>
 __global__ __launch_bounds__(512) void fsim_rowfft_tl_kernel(const float* __restrict__ yiq,
                                                              float2* __restrict__ A,
                                                                   const float2* __restrict__ twg,
-                                                                  int hd, int rb) {
+                                                                  int hd, int rb,
+                                                                  const int* __restrict__ gate) {
+    if (gate_closed(gate)) return;
     constexpr int n = N1 * N2; /* = wd */
     constexpr int ZL = Exch<N1, N2>::line > n + 1 ? Exch<N1, N2>::line : n + 1;
     extern __shared__ float2 lds[];
@@ -318,7 +324,7 @@ void col_launch(Ctx& c, const TlColArgs& a) {
     const int nbx_pad = ceil_div(nbx, a.xcds) * a.xcds;
     const int thr = threads_for(bx * (N1 > N2 ? N1 : N2));
     c.launch("fsim_colpass", fsim_colpass_tl_kernel<N1, N2>, dim3(nbx_pad * 2 * a.npairs), dim3(thr),
-             col_lds<N1, N2>(bx), a.A, a.T, a.bank, a.tw, a.wd, bx, nbx_pad);
+             col_lds<N1, N2>(bx), a.A, a.T, a.bank, a.tw, a.wd, bx, nbx_pad, a.gate);
     (void)n;
 }
 
@@ -333,7 +339,7 @@ void row_launch(Ctx& c, const TlRowArgs& a) {
     IQM_CHECK(4 * rb * N2 <= thr && rb * N1 * N2 <= slots * thr, IQM_ERR_UNSUPPORTED,
               "row pass tile does not fit its thread mapping");
     c.launch("fsim_rowpass", fsim_rowpass_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), 2, a.npairs),
-             dim3(thr), row_lds<N1, N2>(rb), a.T, a.an_e, a.keys, a.hist, a.tw, a.hd, rb, a.invN);
+             dim3(thr), row_lds<N1, N2>(rb), a.T, a.an_e, a.keys, a.hist, a.tw, a.hd, rb, a.invN, a.gate);
 }
 
 template <int N1, int N2>
@@ -345,7 +351,7 @@ void grad_launch(Ctx& c, const TlGradArgs& a) {
     while (rb > 1 && lds(rb) > a.lds_budget) --rb;
     const int thr = threads_for(rb * (N1 > N2 ? N1 : N2));
     c.launch("fsim_rowfft", fsim_rowfft_tl_kernel<N1, N2>, dim3(ceil_div(a.hd, rb), a.npairs),
-             dim3(thr), lds(rb), a.yiq, a.A, a.tw, a.hd, rb);
+             dim3(thr), lds(rb), a.yiq, a.A, a.tw, a.hd, rb, a.gate);
 }
 
 }  // namespace

@@ -12,6 +12,7 @@ struct TlColArgs {
     const float2* tw;     /* exp(-2 pi i j / hd) */
     int wd, npairs, bx, xcds;
     size_t lds_budget;
+    const int* gate;      /* nullable device flag: 0 skips the launch's work */
 };
 
 struct TlRowArgs {
@@ -23,6 +24,7 @@ struct TlRowArgs {
     int hd, npairs, rb;
     float invN;
     size_t lds_budget;
+    const int* gate;
 };
 
 struct TlGradArgs {
@@ -31,6 +33,7 @@ struct TlGradArgs {
     const float2* tw;     /* exp(-2 pi i j / wd) */
     int hd, npairs, rb;
     size_t lds_budget;
+    const int* gate;
 };
 
 /* F12 (fsim_sum_filter_responses.glsl:33-83) for one pixel and orientation

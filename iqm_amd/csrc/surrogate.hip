@@ -839,10 +839,11 @@ Surrogate::Surrogate(Ctx& c, uint64_t seed) : c_(c) {
 
 Surrogate::~Surrogate() {
     (void)hipStreamSynchronize(c_.stream());
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second.exec);
     for (void* q : {(void*)m.master, (void*)m.grad, (void*)m.adam_m, (void*)m.adam_v, (void*)m.wbf,
                     (void*)m.wbf_t, (void*)d_loss_, (void*)acts_.x, (void*)acts_.pred,
                     (void*)acts_.dpred, (void*)acts_.dz[0], (void*)acts_.dz[1], (void*)acts_.partial,
-                    (void*)spot_pc_, (void*)spot_err_})
+                    (void*)spot_pc_, (void*)spot_err_, (void*)d_out_, (void*)d_flags_})
         if (q) (void)hipFree(q);
     for (auto* a : acts_.a)
         if (a) (void)hipFree(a);
@@ -1038,53 +1039,198 @@ void Surrogate::make_patches(int n, int size, uint64_t seed, float* d_x, float* 
     }
 }
 
-double* Surrogate::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
-                              long pair_stride, int n, int spot_every, float tol, float* spot_err,
-                              int* fell_back) {
-    float* pc_net = nullptr;
-    int wd = 0, hd = 0;
-    double* res = c_.fsim_batch_pcfn(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{},
-                                     [&](const float* yiq, long N, int np, float* pc) {
-                                         (void)N;
-                                         const int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
-                                         wd = (int)std::round((float)w / (float)F);
-                                         hd = (int)std::round((float)h / (float)F);
-                                         infer_strided(yiq, 6 * N, N, 1.0f / 255.0f, pc, 2 * np, hd, wd);
-                                         pc_net = pc;
-                                     });
+/* Spot-check verdict: the largest tile error; the batch falls back to exact
+ * PC when it is not within tol (NaN included). */
+__global__ __launch_bounds__(kBlock) void sur_decide_kernel(const float* __restrict__ err, int count, float tol,
+                                                            SpotFlags* __restrict__ flags) {
+    __shared__ float red[kBlock / 64];
+    float m = 0.f;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) m = fmaxf(m, err[i]);
+    for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.f;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = fmaxf(r, red[w]);
+        flags->err = r;
+        flags->use_exact = (r <= tol) ? 0 : 1;
+    }
+}
+
+/* result of the batch: the exact array when the verdict fell back, else the
+ * network's */
+__global__ __launch_bounds__(kBlock) void sur_select_kernel(const double* __restrict__ net,
+                                                            const double* __restrict__ exact,
+                                                            const SpotFlags* __restrict__ flags, int count,
+                                                            double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = flags->use_exact ? exact[i] : net[i];
+}
+
+void Surrogate::ensure_batch_buffers(int n, int m, long N) {
+    bool moved = false;
+    if ((size_t)n > out_cap_) {
+        c_.sync();
+        if (d_out_) (void)hipFree(d_out_);
+        d_out_ = nullptr;
+        out_cap_ = 0;
+        IQM_HIP(hipMalloc(&d_out_, sizeof(double) * 8 * n));
+        out_cap_ = (size_t)n;
+        moved = true;
+    }
+    if ((size_t)m * 2 * N > spot_cap_) {
+        c_.sync();
+        if (spot_pc_) (void)hipFree(spot_pc_);
+        if (spot_err_) (void)hipFree(spot_err_);
+        spot_pc_ = nullptr;
+        spot_err_ = nullptr;
+        spot_cap_ = 0;
+        IQM_HIP(hipMalloc(&spot_pc_, sizeof(float) * 2 * N * m));
+        IQM_HIP(hipMalloc(&spot_err_, sizeof(float) * 2 * kSpotTiles * m));
+        spot_cap_ = (size_t)m * 2 * N;
+        moved = true;
+    }
+    if (!d_flags_) {
+        IQM_HIP(hipMalloc(&d_flags_, sizeof(SpotFlags)));
+        moved = true;
+    }
+    if (moved) drop_graphs();
+}
+
+double* Surrogate::enqueue(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                           int n, int spot_every, float tol) {
+    const int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+    const int wd = (int)std::round((float)w / (float)F), hd = (int)std::round((float)h / (float)F);
+    const long N = (long)wd * hd;
+    const int m = spot_every > 0 ? (n + spot_every - 1) / spot_every : 0;
+    ensure_batch_buffers(n, m, N);
+    const double* net = c_.fsim_batch_pcfn(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{},
+                                           [&](const float* yiq, long Np, int np, float* pc) {
+                                               infer_strided(yiq, 6 * Np, Np, 1.0f / 255.0f, pc, 2 * np, hd, wd);
+                                               pc_net_ = pc;
+                                           });
+    if (spot_every <= 0) return const_cast<double*>(net);
+    /* exact PC of every spot_every-th pair, compared on seeded tiles */
+    FsimBatchOut ex;
+    ex.d_pc = spot_pc_;
+    c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride * spot_every, m, ex);
+    c_.launch("sur_spot", sur_spot_kernel, dim3(kSpotTiles, 2, m), dim3(kBlock), 0, (const float*)pc_net_,
+              (const float*)spot_pc_, wd, hd, spot_every, (uint64_t)0x5EED5EEDull, spot_err_);
+    c_.launch("sur_decide", sur_decide_kernel, dim3(1), dim3(kBlock), 0, (const float*)spot_err_,
+              2 * kSpotTiles * m, tol, d_flags_);
+    /* the whole batch exactly, each kernel gated by the verdict: it does its
+     * work only when the batch falls back */
+    c_.set_gate(&d_flags_->use_exact);
+    const double* exact = nullptr;
+    try {
+        exact = c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{});
+    } catch (...) {
+        c_.set_gate(nullptr);
+        throw;
+    }
+    c_.set_gate(nullptr);
+    c_.launch("sur_select", sur_select_kernel, dim3(ceil_div(8 * n, kBlock)), dim3(kBlock), 0, net, exact,
+              (const SpotFlags*)d_flags_, 8 * n, d_out_);
+    return d_out_;
+}
+
+void Surrogate::drop_graphs() {
+    if (!graphs_.empty()) (void)hipStreamSynchronize(c_.stream());
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs_.clear();
+}
+
+double* Surrogate::enqueue_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch,
+                                 long pair_stride, int n, int spot_every, float tol) {
+    if (!c_.graphs() || c_.timing()) return enqueue(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, tol);
+    if (graph_gen_ != c_.graph_generation()) {
+        drop_graphs();
+        graph_gen_ = c_.graph_generation();
+    }
+    uint32_t tolbits;
+    std::memcpy(&tolbits, &tol, 4);
+    const GraphKey key{d_img, d_ref, pair_stride, w, h, ch, n, spot_every, tolbits};
+    auto it = graphs_.find(key);
+    if (it != graphs_.end()) {
+        it->second.last_use = ++graph_tick_;
+        IQM_HIP(hipGraphLaunch(it->second.exec, c_.stream()));
+        return it->second.res;
+    }
+    /* an eager run sizes every workspace and builds the per-size plans */
+    double* res = enqueue(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, tol);
+    if (graph_gen_ != c_.graph_generation()) {
+        drop_graphs();
+        graph_gen_ = c_.graph_generation();
+    }
+    hipGraph_t g = nullptr;
+    IQM_HIP(hipStreamBeginCapture(c_.stream(), hipStreamCaptureModeThreadLocal));
+    c_.set_capturing(true);
+    double* res2 = nullptr;
+    try {
+        res2 = enqueue(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, tol);
+    } catch (...) {
+        c_.set_capturing(false);
+        (void)hipStreamEndCapture(c_.stream(), &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    c_.set_capturing(false);
+    IQM_HIP(hipStreamEndCapture(c_.stream(), &g));
+    IQM_CHECK(res2 == res, IQM_ERR_DEVICE, "surrogate batch buffers moved during capture");
+    GraphEntry e;
+    e.res = res;
+    e.last_use = ++graph_tick_;
+    const hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    IQM_HIP(ie);
+    if (graphs_.size() >= 8) {
+        auto victim = graphs_.begin();
+        for (auto i = graphs_.begin(); i != graphs_.end(); ++i)
+            if (i->second.last_use < victim->second.last_use) victim = i;
+        (void)hipStreamSynchronize(c_.stream());
+        (void)hipGraphExecDestroy(victim->second.exec);
+        graphs_.erase(victim);
+    }
+    graphs_.emplace(key, e);
+    return res; /* the capture ran nothing: the eager run above produced it */
+}
+
+void Surrogate::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                           int n, int spot_every, float tol, double* fsim_out, double* fsimc_out, float* spot_err,
+                           int* fell_back) {
+    const int F = std::max(1, (int)std::round(std::min(w, h) / 256.0));
+    const auto size = std::make_pair((int)std::round((float)w / (float)F), (int)std::round((float)h / (float)F));
+    const bool known_bad = spot_every > 0 && remember_ && failed_.count(size);
+    const double* res;
+    if (known_bad) {
+        /* the network failed this size's spot check before: exact PC */
+        res = c_.fsim_batch_graph(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{});
+    } else {
+        res = enqueue_graph(d_img, d_ref, w, h, ch, pair_stride, n, spot_every, tol);
+    }
+    char* host = (char*)c_.host_pinned(sizeof(double) * 8 * n + sizeof(SpotFlags));
+    double* hres = (double*)host;
+    SpotFlags* hflags = (SpotFlags*)(host + sizeof(double) * 8 * n);
+    IQM_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, c_.stream()));
+    if (spot_every > 0 && !known_bad)
+        IQM_HIP(hipMemcpyAsync(hflags, d_flags_, sizeof(SpotFlags), hipMemcpyDeviceToHost, c_.stream()));
+    c_.sync();
+    for (int i = 0; i < n; ++i) {
+        if (fsim_out) fsim_out[i] = hres[(size_t)8 * i + 3];
+        if (fsimc_out) fsimc_out[i] = hres[(size_t)8 * i + 4];
+    }
     float err = 0.f;
     int back = 0;
-    if (spot_every > 0) {
-        const int m = (n + spot_every - 1) / spot_every;
-        const long N = (long)wd * hd;
-        if ((size_t)m * 2 * N > spot_cap_) {
-            c_.sync();
-            if (spot_pc_) (void)hipFree(spot_pc_);
-            if (spot_err_) (void)hipFree(spot_err_);
-            spot_pc_ = nullptr;
-            spot_err_ = nullptr;
-            spot_cap_ = 0;
-            IQM_HIP(hipMalloc(&spot_pc_, sizeof(float) * 2 * N * m));
-            IQM_HIP(hipMalloc(&spot_err_, sizeof(float) * 2 * kSpotTiles * m));
-            spot_cap_ = (size_t)m * 2 * N;
-        }
-        FsimBatchOut ex;
-        ex.d_pc = spot_pc_;
-        c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride * spot_every, m, ex);
-        c_.launch("sur_spot", sur_spot_kernel, dim3(kSpotTiles, 2, m), dim3(kBlock), 0, (const float*)pc_net,
-                  (const float*)spot_pc_, wd, hd, spot_every, (uint64_t)0x5EED5EEDull, spot_err_);
-        std::vector<float> e((size_t)2 * kSpotTiles * m);
-        IQM_HIP(hipMemcpyAsync(e.data(), spot_err_, sizeof(float) * e.size(), hipMemcpyDeviceToHost, c_.stream()));
-        c_.sync();
-        for (float v : e) err = std::max(err, v);
-        if (!(err <= tol)) {
-            back = 1;
-            res = c_.fsim_batch(d_img, d_ref, w, h, ch, pair_stride, n, FsimBatchOut{});
-        }
+    if (known_bad) {
+        err = failed_[size];
+        back = 1;
+    } else if (spot_every > 0) {
+        err = hflags->err;
+        back = hflags->use_exact;
+        if (back && remember_) failed_[size] = err;
     }
     if (spot_err) *spot_err = err;
     if (fell_back) *fell_back = back;
-    return res;
 }
 
 /* versioned little-endian blob: magic, version, step, seed, cursor, then the

@@ -7,7 +7,10 @@
 #include "ctx.hpp"
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 namespace iqm {
@@ -26,6 +29,12 @@ struct SurLayout {
     static constexpr int total = b6 + 1;               /* 37,601 */
     static constexpr int wl(int l) { return wmid + (l - 2) * mid_stride; }           /* l = 2..5 */
     static constexpr int bl(int l) { return wmid + (l - 2) * mid_stride + 32 * 32 * 9; }
+};
+
+/* Device verdict of a surrogate batch's spot check. */
+struct SpotFlags {
+    int use_exact; /* 1: the batch fell back to exact PC */
+    float err;     /* largest spot-check tile error */
 };
 
 struct SurModel {
@@ -76,11 +85,28 @@ class Surrogate {
     /* FSIM/FSIMc over n device-resident pairs with phase congruency from the
      * network.  Pairs i with i % spot_every == 0 (spot_every > 0) are also
      * evaluated exactly; the error is the largest |PC_net - PC_exact| over 4
-     * seeded 32 x 32 tiles of each of their two images.  When it exceeds tol
-     * the whole batch is recomputed exactly.  Returns the device result array
-     * (n x 8 doubles, as Ctx::fsim_batch). */
-    double* fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
-                       int n, int spot_every, float tol, float* spot_err, int* fell_back);
+     * seeded 32 x 32 tiles of each of their two images.  When it is not
+     * within tol the whole batch is recomputed exactly.  The verdict is taken
+     * on the device and the exact kernels are gated by it, so the sequence
+     * has no host round trip (a repeated batch replays one captured graph).
+     * With the failure memory on (the default), a downscaled size whose
+     * check failed once goes straight to exact PC afterwards (fell_back = 1,
+     * spot_err = the failing error).  Scores and verdict to the host
+     * (all nullable). */
+    void fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride, int n,
+                    int spot_every, float tol, double* fsim_out, double* fsimc_out, float* spot_err,
+                    int* fell_back);
+    /* the launch sequence of one such batch on the stream (no host sync);
+     * returns the device result array (n x 8 doubles, as Ctx::fsim_batch) */
+    double* enqueue(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride, int n,
+                    int spot_every, float tol);
+    double* enqueue_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
+                          int n, int spot_every, float tol);
+    /* failure memory on/off (clears what was remembered) */
+    void set_remember(bool on) {
+        remember_ = on;
+        failed_.clear();
+    }
 
     void set_params(const float* host);   /* FP32 master, SurLayout order */
     void get_params(float* host) const;
@@ -101,6 +127,31 @@ class Surrogate {
     float* spot_pc_ = nullptr;   /* exact PC maps of the spot-checked pairs */
     float* spot_err_ = nullptr;  /* per (pair, image, tile) max-abs errors */
     size_t spot_cap_ = 0;        /* pairs spot_pc_/spot_err_ hold */
+    const float* pc_net_ = nullptr; /* network PC maps of the last enqueue */
+    double* d_out_ = nullptr;    /* selected result array, n x 8 */
+    size_t out_cap_ = 0;
+    SpotFlags* d_flags_ = nullptr;
+    bool remember_ = true;
+    std::map<std::pair<int, int>, float> failed_; /* (wd, hd) -> failing spot error */
+    void ensure_batch_buffers(int n, int m, long N);
+    void drop_graphs();
+    struct GraphKey {
+        const void *a, *b;
+        long stride;
+        int w, h, ch, n, every;
+        uint32_t tol;
+        bool operator<(const GraphKey& k) const {
+            return std::tie(a, b, stride, w, h, ch, n, every, tol) <
+                   std::tie(k.a, k.b, k.stride, k.w, k.h, k.ch, k.n, k.every, k.tol);
+        }
+    };
+    struct GraphEntry {
+        hipGraphExec_t exec = nullptr;
+        double* res = nullptr;
+        uint64_t last_use = 0;
+    };
+    std::map<GraphKey, GraphEntry> graphs_;
+    uint64_t graph_tick_ = 0, graph_gen_ = 0;
 };
 
 }  // namespace iqm

@@ -252,11 +252,26 @@ def test_fsim_with_surrogate_pc(ctx, sur):
             worst = max(worst, _spot_err(pc_net, pc_ex, wd, hd, i // 2))
     print(f"[sur fsim] spot_err gpu={err:.6e} host={worst:.6e}", flush=True)
     assert abs(err - worst) <= 1e-6
-    # a tolerance below any error forces the exact fallback for the whole batch
+    # a tolerance below any error forces the exact fallback for the whole
+    # batch; the verdict is taken on the device and gates the exact kernels
     fs2, fc2, err2, back2 = sur.fsim_batch_dev(di, dr, w, h, 3, stride, n, spot_every=2, spot_tol=-1.0)
     ex_fs, ex_fc = fsim.compute_batch_dev(ctx, di, dr, w, h, 3, stride, n)
     assert back2 and err2 == err
     assert np.array_equal(fs2, ex_fs) and np.array_equal(fc2, ex_fc)
+    # failure memory: this size failed, so a later checked batch runs exactly
+    # even with a tolerance it would pass, and reports the remembered error
+    fs3, fc3, err3, back3 = sur.fsim_batch_dev(di, dr, w, h, 3, stride, n, spot_every=2, spot_tol=1e9)
+    assert back3 and err3 == err and np.array_equal(fs3, ex_fs) and np.array_equal(fc3, ex_fc)
+    # memory off: the network runs again; repeated calls replay captured
+    # graphs, passing (network scores) and failing (exact scores) alike
+    sur.set_failure_memory(False)
+    for _ in range(2):
+        fs4, fc4, err4, back4 = sur.fsim_batch_dev(di, dr, w, h, 3, stride, n, spot_every=2, spot_tol=1e9)
+        assert not back4 and err4 == err and np.array_equal(fs4, fs) and np.array_equal(fc4, fc)
+        fs5, fc5, err5, back5 = sur.fsim_batch_dev(di, dr, w, h, 3, stride, n, spot_every=2, spot_tol=-1.0)
+        assert back5 and err5 == err and np.array_equal(fs5, ex_fs) and np.array_equal(fc5, ex_fc)
+        sur.set_failure_memory(False)
+    sur.set_failure_memory(True)
     ctx.free(di)
     ctx.free(dr)
 
