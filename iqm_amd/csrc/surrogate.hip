@@ -12,7 +12,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace iqm {
@@ -200,6 +202,238 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
 
 /* All six layers of the forward pass in one kernel, for inference.  A
  * workgroup streams a strip of kFuSW output columns of one image down its
+ * rows.  At step s the matrix-core layer l (2..5) reads one new row q of
+ * layer l-1 (the row that layer wrote at step s-1) and applies the three tap
+ * rows of the 3x3 kernel to it at once: row q is tap row 0 of output row
+ * q+1 (whose sums start here), tap row 1 of output row q and tap row 2 of
+ * output row q-1, which is then complete and is written to the layer's ring.
+ * So every row is read from LDS once, the three open output rows live in
+ * registers, and a layer's ring holds two rows (the one being written and
+ * the one the next layer reads).  An output's taps are still summed in the
+ * order t = 0..8 (three per step, dx ascending), so the bits are those of
+ * sur_conv_mid.  Layer 6 keeps its three open rows the same way; layer 1
+ * (FP32, from a 4-row input ring) computes row s from input rows s-1..s+1 as
+ * sur_conv_first does.  Output row s - 2(l-1) completes at step s; steps run
+ * from -2 to h + 9 (layer 6 completes row h-1).  Rows outside the image read
+ * as zeros: slots of negative rows are never written before they are read,
+ * and rows >= h are written as zeros.  The halo shrinks by one column per
+ * layer (layer l covers kFuSW + 2(6-l) columns), positions outside the image
+ * are zeros (the zero padding of the layer-by-layer network).  Waves 2k and
+ * 2k+1 own layer k+2, one 16-channel half each (its B fragments in
+ * registers, all five pixel tiles, three accumulators per tile); the FP32
+ * layers 1 and 6 are shared by all threads. */
+constexpr int kFuSW = 64;                          /* output columns per strip */
+constexpr int kFuPitch = 84;                       /* ring row pitch (pixels) */
+constexpr int kFuXPitch = 80;                      /* input ring row pitch (floats) */
+constexpr int kFuThreads = 512;
+constexpr size_t kFuRingBytes = (size_t)5 * 2 * kFuPitch * 32 * 2;
+constexpr size_t kFuXBytes = (size_t)4 * kFuXPitch * 4; /* input rows, then the rings */
+constexpr size_t kFuLds = kFuXBytes + kFuRingBytes;
+static_assert(kFuPitch % 4 == 0 && kFuXBytes % 256 == 0, "ring rows on 256-byte boundaries");
+
+__global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
+                                                               long istride, float scale,
+                                                               const float* __restrict__ params,
+                                                               const uint16_t* __restrict__ wbf,
+                                                               float* __restrict__ out, int h, int w) {
+    extern __shared__ __align__(16) uint8_t fu_smem[];
+    __shared__ float b6s;
+    float* xr = (float*)fu_smem;                                  /* [4][kFuXPitch] */
+    uint16_t* ring = (uint16_t*)(fu_smem + kFuXBytes);            /* [layer 1..5][2][kFuPitch][32] */
+    /* channel c of pixel p: a pixel's 64 bytes hold its four 16-byte chunks
+     * of 8 channels in the order chunk ^ ((p >> 1) & 3), so the 8 lanes of
+     * one 128-byte LDS access reading one chunk of 8 consecutive pixels (a
+     * matrix-core fragment) fall into 8 different 16-byte bank groups */
+    auto RS = [&](int layer, int row, int p, int c) {
+        return ring + (((layer - 1) * 2 + (row & 1)) * kFuPitch + p) * 32 + ((((c >> 3) ^ (p >> 1)) & 3) << 3) +
+               (c & 7);
+    };
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kFuSW;
+    const int j = blockIdx.y;
+    const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
+    float* oj = out + (long)j * h * w;
+
+    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
+    if (tid == 0) b6s = params[SurLayout::b6];
+    /* this wave's layer, channel half and B fragments (k = ci, n = co) */
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int ml = 2 + (wave >> 1), nb = wave & 1;
+    const uint16_t* wl = wbf + (ml - 2) * 9216;
+    bf16x8 B[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) B[t] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
+    /* biases of the channels this lane's outputs hold: 16 nb + 4 g + e */
+    float bias[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bias[e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
+    /* bit k: this lane's pixel of tile k lies inside the layer's columns and the image */
+    unsigned inmask = 0;
+    {
+        const int lcols = kFuSW + 2 * (6 - ml);
+        for (int k = 0; k < 5; ++k) {
+            const int p = 16 * k + li, gx = x0 - (6 - ml) + p;
+            if (p < lcols && gx >= 0 && gx < w) inmask |= 1u << k;
+        }
+    }
+    /* layer 1: thread = (channel l1co, pixels 5 l1pg ..), its BF16-rounded weights */
+    const int l1co = tid & 31, l1pg = tid >> 5;
+    float w1r[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
+    const float b1r = params[SurLayout::b1 + l1co];
+    /* layer 6: thread = (pixel tid / 8, channels 4 (tid % 8) ..); weights of
+     * channel pairs packed as BF16 pairs for the two-term dot products */
+    bf16x2 w6p[2][9];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int c = 4 * (tid & 7) + 2 * k;
+            w6p[k][t] = bf16x2{(__bf16)params[SurLayout::w6 + c * 9 + t], (__bf16)params[SurLayout::w6 + (c + 1) * 9 + t]};
+        }
+    /* input rows enter two steps ahead: row s + 2 is stored at step s (from
+     * a register loaded the step before) */
+    const int xq = tid, xgx = x0 - 6 + tid;
+    const bool xload = tid < kFuSW + 12 && xgx >= 0 && xgx < w;
+    float xnext = (xload && 0 < h) ? xi[xgx] : 0.f;
+    __syncthreads();
+
+    /* open sums: [k] belongs to the output row started at step u = s + 2
+     * with u % 3 == k */
+    float a6[3] = {0.f, 0.f, 0.f};
+    f32x4 acc[3][5];
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) acc[k][c] = z4;
+
+    auto step = [&](auto kk, int s) {
+        constexpr int kn = decltype(kk)::value;  /* slot of the row started now */
+        constexpr int km = (kn + 2) % 3;         /* slot of the row started last step */
+        constexpr int kc = (kn + 1) % 3;         /* slot of the row completed now */
+        /* input row s + 2 to LDS; issue row s + 3 */
+        if (tid < kFuSW + 12) {
+            xr[((s + 2) & 3) * kFuXPitch + xq] = xnext * scale;
+            xnext = (xload && s + 3 >= 0 && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
+        }
+        auto layer1 = [&]() {
+            /* row s from input rows s-1..s+1: kFuSW + 10 columns x 32 channels */
+            const int r = s;
+            const int p0 = l1pg * 5;
+            if (r < 0 || r > h || p0 >= kFuSW + 10) return;
+            float xw[3][7];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int qq = 0; qq < 7; ++qq) xw[dy][qq] = xr[((r + dy - 1) & 3) * kFuXPitch + p0 + qq];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const int p = p0 + k;
+                const int gx = x0 - 5 + p;
+                float v = 0.f;
+                if (r < h && gx >= 0 && gx < w) {
+                    v = b1r;
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
+                    v = fmaxf(v, 0.f);
+                }
+                if (p < kFuSW + 10) *RS(1, r, p, l1co) = f2bf(v);
+            }
+        };
+        auto layers_mid = [&]() {
+            /* layer ml reads row q = s - 2 ml + 3 of layer ml - 1 and
+             * completes row q - 1; five tiles, three open rows each */
+            static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
+            const int q = s - 2 * ml + 3;
+            /* q = h + 1 only writes row h as zeros: the next layer reads it
+             * as padding and its slot still holds row h - 2 */
+            if (q < -1 || q > h + 1) return;
+            __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+            for (int dx = 0; dx < 3 && q <= h; ++dx) {
+                const uint16_t* row = RS(ml - 1, q, li + dx, 8 * g);
+#pragma unroll
+                for (int tile = 0; tile < 5; ++tile) {
+                    /* tiles are 16 pixels apart, which keeps the chunk order */
+                    const bf16x8 F = *(const bf16x8*)(row + tile * 16 * 32);
+                    /* weights as the first operand: D = W^T X^T, so a lane's
+                     * four outputs are 4 channels of one pixel; taps dx (new
+                     * row), 3 + dx (open row), 6 + dx (completing row) */
+                    acc[kn][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[dx], F, dx == 0 ? z4 : acc[kn][tile], 0, 0, 0);
+                    acc[km][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[3 + dx], F, acc[km][tile], 0, 0, 0);
+                    acc[kc][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[6 + dx], F, acc[kc][tile], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_setprio(0);
+            const int o = q - 1;
+            if (o < 0) return;
+            const bool live = o < h;
+            /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li of a
+             * tile: bias, ReLU, BF16, one 8-byte store */
+#pragma unroll
+            for (int tile = 0; tile < 5; ++tile) {
+                const f32x4 a = acc[kc][tile];
+                uint2 v = make_uint2(0u, 0u);
+                if (live && ((inmask >> tile) & 1u)) {
+                    v.x = (uint32_t)f2bf(fmaxf(a[0] + bias[0], 0.f)) | ((uint32_t)f2bf(fmaxf(a[1] + bias[1], 0.f)) << 16);
+                    v.y = (uint32_t)f2bf(fmaxf(a[2] + bias[2], 0.f)) | ((uint32_t)f2bf(fmaxf(a[3] + bias[3], 0.f)) << 16);
+                }
+                *(uint2*)RS(ml, o, tile * 16 + li, 16 * nb + 4 * g) = v;
+            }
+        };
+        auto layer6 = [&]() {
+            /* reads row q = s - 9 of layer 5, completes output row s - 10 */
+            const int q = s - 9;
+            if (q < -1 || q > h) return;
+            const int p = tid >> 3, cg = tid & 7;
+            float vn = 0.f, vm = a6[km], vc = a6[kc];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const uint2 a = *(const uint2*)RS(5, q, p + dx, 4 * cg);
+                const bf16x2 lo = __builtin_bit_cast(bf16x2, a.x), hi = __builtin_bit_cast(bf16x2, a.y);
+                /* BF16 x BF16 products are exact in FP32 */
+                vn = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][dx], vn, false);
+                vn = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][dx], vn, false);
+                vm = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][3 + dx], vm, false);
+                vm = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][3 + dx], vm, false);
+                vc = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][6 + dx], vc, false);
+                vc = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][6 + dx], vc, false);
+            }
+            a6[kn] = vn;
+            a6[km] = vm;
+            vc += __shfl_xor(vc, 1, 64);
+            vc += __shfl_xor(vc, 2, 64);
+            vc += __shfl_xor(vc, 4, 64);
+            const int o = q - 1, gx = x0 + p;
+            if (cg == 0 && o >= 0 && o < h && gx < w) oj[(long)o * w + gx] = vc + b6s;
+        };
+        /* the layers of a step touch disjoint LDS rows, so their order is
+         * free: waves 0-3 run the FP32 layers first, waves 4-7 the
+         * matrix-core layer first, and each SIMD (waves k and k+4) overlaps
+         * the two */
+        if (wave < 4) {
+            layer1();
+            layer6();
+            layers_mid();
+        } else {
+            layers_mid();
+            layer1();
+            layer6();
+        }
+        lds_barrier();
+    };
+    for (int u = 0; u < h + 12; u += 3) {
+        step(std::integral_constant<int, 0>{}, u - 2);
+        step(std::integral_constant<int, 1>{}, u - 1);
+        step(std::integral_constant<int, 2>{}, u);
+    }
+}
+
+/* All six layers of the forward pass in one kernel, for inference.  A
+ * workgroup streams a strip of kFuSW output columns of one image down its
  * rows; every layer's activations stay in LDS in a ring of 4 rows.  Layer l
  * (1..6) works on row s - 2(l-1) at step s, so all layers advance together
  * with one barrier per step: layer l reads rows r-1..r+1 of layer l-1, which
@@ -211,15 +445,10 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
  * 2k+1 own layer k+2 (B fragments of both 16-channel halves in registers for
  * the whole kernel, pixel tiles split between the two waves); the FP32 layers
  * 1 and 6 are shared by all threads. */
-constexpr int kFuSW = 64;                          /* output columns per strip */
-constexpr int kFuPitch = 84;                       /* ring row pitch (pixels) */
-constexpr int kFuXPitch = 80;                      /* input ring row pitch (floats) */
-constexpr int kFuThreads = 512;
-constexpr size_t kFuRingBytes = (size_t)5 * 4 * kFuPitch * 32 * 2;
-constexpr size_t kFuLds = (size_t)4 * kFuXPitch * 4 + kFuRingBytes;
-static_assert(kFuPitch % 4 == 0 && (4 * kFuXPitch * 4) % 256 == 0, "ring rows on 256-byte boundaries");
+constexpr size_t kFuRingBytesV1 = (size_t)5 * 4 * kFuPitch * 32 * 2;
+constexpr size_t kFuLdsV1 = (size_t)4 * kFuXPitch * 4 + kFuRingBytesV1;
 
-__global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
+__global__ __launch_bounds__(kFuThreads) void sur_fused_v1_kernel(const float* __restrict__ x, long pstride,
                                                                long istride, float scale,
                                                                const float* __restrict__ params,
                                                                const uint16_t* __restrict__ wbf,
@@ -915,8 +1144,12 @@ void Surrogate::forward(int nimg, int h, int w) {
 static void launch_fused(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale, float* out,
                          int nimg, int h, int w) {
     IQM_CHECK(nimg <= 65535, IQM_ERR_INVALID, "too many images for one fused launch");
-    c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps, is,
-             scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
+    if (std::getenv("IQM_SUR_FUSED_V1"))
+        c.launch("sur_fused", sur_fused_v1_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLdsV1, x,
+                 ps, is, scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
+    else
+        c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps,
+                 is, scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
 }
 
 void Surrogate::infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
