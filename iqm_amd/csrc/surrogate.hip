@@ -26,6 +26,18 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 /* FP32 -> BF16 bits, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950) */
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+/* two values to BF16 in one conversion, lo in the low half */
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+/* ReLU on BF16 bits: a BF16 value is negative exactly when its bits are as a
+ * signed 16-bit integer, so max(bits, 0) is the ReLU (-0 becomes +0) */
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), (i16x2){0, 0}));
+}
+__device__ __forceinline__ uint16_t relu_bf16(uint16_t v) { return (int16_t)v < 0 ? (uint16_t)0 : v; }
 
 /* ---- forward ----------------------------------------------------------- */
 
@@ -116,7 +128,9 @@ __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restric
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
         const int r = wave * 4 + rr;
-        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+        /* the forward sums start from the bias (the accumulator's initial
+         * value); ReLU is taken on the BF16 result */
+        f32x4 acc0 = {b0, b0, b0, b0}, acc1 = {b1, b1, b1, b1};
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int dy = t / 3, dx = t % 3;
@@ -128,13 +142,13 @@ __global__ __launch_bounds__(kBlock) void sur_conv_mid(const uint16_t* __restric
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int px = 4 * g + e;
-            float v0 = acc0[e], v1 = acc1[e];
+            uint16_t v0 = f2bf(acc0[e]), v1 = f2bf(acc1[e]);
             if (FWD) {
-                v0 = fmaxf(v0 + b0, 0.f);
-                v1 = fmaxf(v1 + b1, 0.f);
+                v0 = relu_bf16(v0);
+                v1 = relu_bf16(v1);
             }
-            tout[(r * 16 + px) * 32 + li] = f2bf(v0);
-            tout[(r * 16 + px) * 32 + 16 + li] = f2bf(v1);
+            tout[(r * 16 + px) * 32 + li] = v0;
+            tout[(r * 16 + px) * 32 + 16 + li] = v1;
         }
     }
     __syncthreads();
@@ -265,9 +279,9 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 #pragma unroll
     for (int t = 0; t < 9; ++t) B[t] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
     /* biases of the channels this lane's outputs hold: 16 nb + 4 g + e */
-    float bias[4];
+    f32x4 b4;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) bias[e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
+    for (int e = 0; e < 4; ++e) b4[e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
     /* bit k: this lane's pixel of tile k lies inside the layer's columns and the image */
     unsigned inmask = 0;
     {
@@ -362,7 +376,7 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
                     /* weights as the first operand: D = W^T X^T, so a lane's
                      * four outputs are 4 channels of one pixel; taps dx (new
                      * row), 3 + dx (open row), 6 + dx (completing row) */
-                    acc[kn][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[dx], F, dx == 0 ? z4 : acc[kn][tile], 0, 0, 0);
+                    acc[kn][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[dx], F, dx == 0 ? b4 : acc[kn][tile], 0, 0, 0);
                     acc[km][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[3 + dx], F, acc[km][tile], 0, 0, 0);
                     acc[kc][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[6 + dx], F, acc[kc][tile], 0, 0, 0);
                 }
@@ -372,14 +386,15 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
             if (o < 0) return;
             const bool live = o < h;
             /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li of a
-             * tile: bias, ReLU, BF16, one 8-byte store */
+             * tile (the sums started from the bias): BF16, ReLU, one 8-byte
+             * store */
 #pragma unroll
             for (int tile = 0; tile < 5; ++tile) {
                 const f32x4 a = acc[kc][tile];
                 uint2 v = make_uint2(0u, 0u);
                 if (live && ((inmask >> tile) & 1u)) {
-                    v.x = (uint32_t)f2bf(fmaxf(a[0] + bias[0], 0.f)) | ((uint32_t)f2bf(fmaxf(a[1] + bias[1], 0.f)) << 16);
-                    v.y = (uint32_t)f2bf(fmaxf(a[2] + bias[2], 0.f)) | ((uint32_t)f2bf(fmaxf(a[3] + bias[3], 0.f)) << 16);
+                    v.x = relu_bf16x2(pack_bf16(a[0], a[1]));
+                    v.y = relu_bf16x2(pack_bf16(a[2], a[3]));
                 }
                 *(uint2*)RS(ml, o, tile * 16 + li, 16 * nb + 4 * g) = v;
             }
