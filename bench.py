@@ -158,8 +158,10 @@ def cpu_baseline(pairs, budget_s: float) -> dict:
 
 def surrogate_leg(args, ctx, fsim, d_img, d_ref, B, pair_bytes, fs_exact, fc_exact, n_unique) -> dict:
     """The same batch with phase congruency from the trained network: pairs/s
-    with no spot check, pairs/s with the exact spot check on pair 0 of every
-    batch (fallback above 0.02), and |dFSIM| against the exact scores."""
+    with no spot check, with the exact spot check on pair 0 of every batch
+    (fallback above 0.02) decided anew each batch, and with the default
+    failure memory (a size that failed runs exactly afterwards); |dFSIM|
+    against the exact scores."""
     import iqm_amd
 
     sur = iqm_amd.Surrogate(ctx, seed=1)
@@ -167,7 +169,11 @@ def surrogate_leg(args, ctx, fsim, d_img, d_ref, B, pair_bytes, fs_exact, fc_exa
         sur.load(f.read())
     timer = HipTimer(ctx.stream)
     out = {}
-    for name, every in (("no_spot_check", 0), ("spot_check_every_batch", B)):
+    legs = (("no_spot_check", 0, True),
+            ("spot_check_every_batch", B, False),   # each batch takes its own check (and fallback)
+            ("spot_check_failure_memory", B, True))  # the default: a failed size runs exactly afterwards
+    for name, every, memory in legs:
+        sur.set_failure_memory(memory)
         for _ in range(max(1, args.warmup)):
             sur.fsim_batch_dev(d_img, d_ref, W, H, CH, pair_bytes, B, spot_every=every)
         ctx.sync()
