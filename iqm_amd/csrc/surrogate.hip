@@ -297,6 +297,12 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 #pragma unroll
     for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
     const float b1r = params[SurLayout::b1 + l1co];
+    unsigned l1mask = 0; /* bit k: pixel 5 l1pg + k is a column of the image */
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int gx = x0 - 5 + l1pg * 5 + k;
+        if (gx >= 0 && gx < w) l1mask |= 1u << k;
+    }
     /* layer 6: thread = (pixel tid / 8, channels 4 (tid % 8) ..); weights of
      * channel pairs packed as BF16 pairs for the two-term dot products */
     bf16x2 w6p[2][9];
@@ -343,17 +349,15 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int qq = 0; qq < 7; ++qq) xw[dy][qq] = xr[((r + dy - 1) & 3) * kFuXPitch + p0 + qq];
+            const bool rin = r < h;
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const int p = p0 + k;
-                const int gx = x0 - 5 + p;
-                float v = 0.f;
-                if (r < h && gx >= 0 && gx < w) {
-                    v = b1r;
+                /* computed everywhere, kept where the column is real */
+                float v = b1r;
 #pragma unroll
-                    for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
-                    v = fmaxf(v, 0.f);
-                }
+                for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
+                v = (rin && ((l1mask >> k) & 1u)) ? fmaxf(v, 0.f) : 0.f;
                 if (p < kFuSW + 10) *RS(1, r, p, l1co) = f2bf(v);
             }
         };
@@ -391,11 +395,9 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 #pragma unroll
             for (int tile = 0; tile < 5; ++tile) {
                 const f32x4 a = acc[kc][tile];
-                uint2 v = make_uint2(0u, 0u);
-                if (live && ((inmask >> tile) & 1u)) {
-                    v.x = relu_bf16x2(pack_bf16(a[0], a[1]));
-                    v.y = relu_bf16x2(pack_bf16(a[2], a[3]));
-                }
+                const bool keep = live && ((inmask >> tile) & 1u);
+                const uint2 v = make_uint2(keep ? relu_bf16x2(pack_bf16(a[0], a[1])) : 0u,
+                                           keep ? relu_bf16x2(pack_bf16(a[2], a[3])) : 0u);
                 *(uint2*)RS(ml, o, tile * 16 + li, 16 * nb + 4 * g) = v;
             }
         };
