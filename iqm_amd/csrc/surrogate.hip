@@ -41,48 +41,71 @@ __device__ __forceinline__ uint16_t relu_bf16(uint16_t v) { return (int16_t)v < 
 
 /* ---- forward ----------------------------------------------------------- */
 
-/* Layer 1: 1 -> 32 channels, FP32 FMA, ReLU.  Image j of the input is read
- * at x + (j/2)*pstride + (j%2)*istride, times `scale` (so FSIM's Y planes can
- * be read in place). */
+/* An input value as the BF16 pair (hi, lo) = (bf16(v), bf16(v - hi)), hi in
+ * the low half: 16 significant bits (v - hi is exact), and its products with
+ * BF16 weights are exact in FP32. */
+__device__ __forceinline__ uint32_t x_pair(float v) {
+    const float hi = bf2f(f2bf(v));
+    return pack_bf16(v, v - hi);
+}
+
+/* Layer-1 A fragment (weights; M = output channel co, K = 8 g .. 8 g + 7):
+ * k = 8 dy + 2 dx + part for tap (dy, dx) and part (hi, lo) of the input
+ * pair, dy = g; k % 8 >= 6 and g = 3 are zeros. */
+__device__ __forceinline__ bf16x8 sur_first_a(const float* __restrict__ params, int co, int g) {
+    bf16x8 a = {};
+    if (g < 3)
+#pragma unroll
+        for (int e = 0; e < 6; ++e) a[e] = (__bf16)params[SurLayout::w1 + co * 9 + 3 * g + e / 2];
+    return a;
+}
+
+/* Layer 1: 1 -> 32 channels, ReLU, on the matrix cores.  Image j of the
+ * input is read at x + (j/2)*pstride + (j%2)*istride, times `scale` (so
+ * FSIM's Y planes can be read in place).  Each input value enters as its
+ * BF16 pair x_pair and the weights in BF16, so every product is exact and
+ * the sums, from the bias, are FP32 inside one instruction: K = tap row dy
+ * (lane group) x tap column dx x (hi, lo), 18 of the 32.  A wavefront
+ * computes 16 pixels of a row x 32 channels (one instruction per 16
+ * channels); the fused kernel builds the same operands, so its layer 1 has
+ * the same bits. */
 __global__ __launch_bounds__(kBlock) void sur_conv_first(const float* __restrict__ x, long pstride,
                                                          long istride, float scale,
                                                          const float* __restrict__ params,
                                                          uint16_t* __restrict__ out, int nimg, int h,
                                                          int w) {
-    __shared__ float wsh[32 * 9 + 32];
-    for (int i = threadIdx.x; i < 32 * 9 + 32; i += blockDim.x) {
-        const float v = params[SurLayout::w1 + i];
-        wsh[i] = i < 32 * 9 ? bf2f(f2bf(v)) : v; /* BF16 weights, FP32 bias */
-    }
-    __syncthreads();
-    const long HW = (long)h * w;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nimg * HW) return;
-    const int j = (int)(p / HW);
-    const long q = p - (long)j * HW;
-    const int y = (int)(q / w), xx = (int)(q - (long)y * w);
+    const int lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int tw = (w + 15) / 16;
+    const long tile = (long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (tile >= (long)nimg * h * tw) return;
+    const int j = (int)(tile / ((long)h * tw));
+    const int rem = (int)(tile - (long)j * h * tw);
+    const int y = rem / tw, px = (rem - y * tw) * 16 + li;
     const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
-    float v[9];
+    /* B fragment: lane (pixel li, group g) holds tap row g, columns px-1..px+1 */
+    uint32_t v[3] = {0u, 0u, 0u};
+    const int yy = y + g - 1;
+    if (g < 3 && yy >= 0 && yy < h)
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
-        v[t] = (yy >= 0 && yy < h && xc >= 0 && xc < w) ? xi[(long)yy * w + xc] * scale : 0.f;
-    }
-    uint32_t packed[16];
-#pragma unroll
-    for (int co = 0; co < 32; co += 2) {
-        float s0 = wsh[32 * 9 + co], s1 = wsh[32 * 9 + co + 1];
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            s0 += wsh[co * 9 + t] * v[t];
-            s1 += wsh[(co + 1) * 9 + t] * v[t];
+        for (int dx = 0; dx < 3; ++dx) {
+            const int xc = px + dx - 1;
+            if (xc >= 0 && xc < w) v[dx] = x_pair(xi[(long)yy * w + xc] * scale);
         }
-        packed[co / 2] = (uint32_t)f2bf(fmaxf(s0, 0.f)) | ((uint32_t)f2bf(fmaxf(s1, 0.f)) << 16);
-    }
-    uint4* o = (uint4*)(out + p * 32);
+    const bf16x8 F = __builtin_bit_cast(bf16x8, make_uint4(v[0], v[1], v[2], 0u));
+    uint16_t* o = out + (((long)j * h + y) * w + px) * 32;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        o[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+    for (int hf = 0; hf < 2; ++hf) {
+        const bf16x8 A = sur_first_a(params, 16 * hf + li, g);
+        f32x4 c;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = params[SurLayout::b1 + 16 * hf + 4 * g + e];
+        /* D layout: lane holds channels 16 hf + 4g..4g+3 of pixel li */
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, F, c, 0, 0, 0);
+        if (px < w)
+            *(uint2*)(o + 16 * hf + 4 * g) =
+                make_uint2(relu_bf16x2(pack_bf16(d[0], d[1])), relu_bf16x2(pack_bf16(d[2], d[3])));
+    }
 }
 
 /* Layers 2-5 (and their input gradient): 3x3, 32 -> 32, NHWC BF16, MFMA.
@@ -216,229 +239,320 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
 
 /* All six layers of the forward pass in one kernel, for inference.  A
  * workgroup streams a strip of kFuSW output columns of one image down its
- * rows.  At step s the matrix-core layer l (2..5) reads one new row q of
- * layer l-1 (the row that layer wrote at step s-1) and applies the three tap
- * rows of the 3x3 kernel to it at once: row q is tap row 0 of output row
- * q+1 (whose sums start here), tap row 1 of output row q and tap row 2 of
- * output row q-1, which is then complete and is written to the layer's ring.
- * So every row is read from LDS once, the three open output rows live in
- * registers, and a layer's ring holds two rows (the one being written and
- * the one the next layer reads).  An output's taps are still summed in the
- * order t = 0..8 (three per step, dx ascending), so the bits are those of
- * sur_conv_mid.  Layer 6 keeps its three open rows the same way; layer 1
- * (FP32, from a 4-row input ring) computes row s from input rows s-1..s+1 as
- * sur_conv_first does.  Output row s - 2(l-1) completes at step s; steps run
- * from -2 to h + 9 (layer 6 completes row h-1).  Rows outside the image read
- * as zeros: slots of negative rows are never written before they are read,
- * and rows >= h are written as zeros.  The halo shrinks by one column per
- * layer (layer l covers kFuSW + 2(6-l) columns), positions outside the image
- * are zeros (the zero padding of the layer-by-layer network).  Waves 2k and
- * 2k+1 own layer k+2, one 16-channel half each (its B fragments in
- * registers, all five pixel tiles, three accumulators per tile); the FP32
- * layers 1 and 6 are shared by all threads. */
-constexpr int kFuSW = 64;                          /* output columns per strip */
-constexpr int kFuPitch = 84;                       /* ring row pitch (pixels) */
-constexpr int kFuXPitch = 80;                      /* input ring row pitch (floats) */
+ * rows; every step s ends with one barrier.  The 3x3 layers 2-5 run on the
+ * matrix cores: at step s layer l reads one row q = s - 2l + 3 of the layer
+ * below from LDS and applies the three tap rows of its kernel to it at once:
+ * row q is tap row 0 of output row q + 1 (whose sums start here, from the
+ * bias), tap row 1 of output row q and tap row 2 of output row q - 1, which
+ * is then complete and is written to the layer's ring.  So every row is read
+ * from LDS once, three open output rows live in registers, and a layer's ring
+ * holds two rows (the one being written and the one the next layer reads).
+ * An output's taps are summed in the order t = 0..8 (three per step, dx
+ * ascending), so the bits are those of sur_conv_mid.  A layer row is five
+ * 16-pixel tiles.  Every wave works on the matrix cores: waves k and k + 4
+ * share a SIMD and split the row of layer k + 2 (fu_has), and wave k + 4
+ * also does a quarter of the rest, on the matrix cores too: the input row
+ * s + 2 (into a 4-row ring, each value as its BF16 pair x_pair), layer-1
+ * units of row s with the operands of sur_conv_first (so the same bits),
+ * and output tile k of layer 6, whose instructions give the three tap rows'
+ * sums of a pixel (three open output rows, like the 3x3 layers; its sums in
+ * another order than sur_conv_last's).  The FP32 conversions and stores are
+ * spread between the matrix-core instructions: a SIMD runs about two of them
+ * per matrix-core instruction at no cost, while a wave of FP32 work beside a
+ * matrix-core wave slows both.  Output row s - 2(l-1) of layer l completes at
+ * step s; steps run from -2 to h + 9 (layer 6 completes row h-1).  Rows
+ * outside the image read as zeros: rows -2, -1 and h of every layer are
+ * written as zeros.  The halo shrinks by one column per layer (layer l
+ * covers kFuSW + 2(6-l) columns; the five tiles of a row cover more, and
+ * those extra columns only feed extra columns), and columns outside the
+ * image are zeros, the zero padding of the layer-by-layer network. */
+constexpr int kFuSW = 64;      /* output columns per strip */
+constexpr int kFuPitch = 84;   /* ring row pitch (pixels) */
+constexpr int kFuXPitch = 96;  /* input ring row pitch (BF16 pairs); rows 0-3 the ring, row 4 zeros */
+constexpr int kFuXCols = 82;   /* input columns x0 - 6 .. x0 + 75 */
 constexpr int kFuThreads = 512;
 constexpr size_t kFuRingBytes = (size_t)5 * 2 * kFuPitch * 32 * 2;
-constexpr size_t kFuXBytes = (size_t)4 * kFuXPitch * 4; /* input rows, then the rings */
-constexpr size_t kFuLds = kFuXBytes + kFuRingBytes;
-static_assert(kFuPitch % 4 == 0 && kFuXBytes % 256 == 0, "ring rows on 256-byte boundaries");
+constexpr size_t kFuXBytes = 2048; /* input rows, then the rings */
+/* where a dropped layer-1 unit stores: 64 lanes x 8 bytes past either
+ * layer-1 row slot's base */
+constexpr size_t kFuSinkBytes = (size_t)kFuPitch * 32 * 2 + 64 * 8;
+constexpr size_t kFuLds = kFuXBytes + kFuRingBytes + kFuSinkBytes;
+static_assert(5 * kFuXPitch * 4 <= (int)kFuXBytes && kFuXBytes % 256 == 0, "input ring, rings on 256-byte boundaries");
+static_assert(kFuPitch % 4 == 0 && kFuXCols <= kFuXPitch, "ring pitches");
+static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
 
-__global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
-                                                               long istride, float scale,
-                                                               const float* __restrict__ params,
-                                                               const uint16_t* __restrict__ wbf,
-                                                               float* __restrict__ out, int h, int w) {
-    extern __shared__ __align__(16) uint8_t fu_smem[];
-    __shared__ float b6s;
-    float* xr = (float*)fu_smem;                                  /* [4][kFuXPitch] */
-    uint16_t* ring = (uint16_t*)(fu_smem + kFuXBytes);            /* [layer 1..5][2][kFuPitch][32] */
-    /* channel c of pixel p: a pixel's 64 bytes hold its four 16-byte chunks
-     * of 8 channels in the order chunk ^ ((p >> 1) & 3), so the 8 lanes of
-     * one 128-byte LDS access reading one chunk of 8 consecutive pixels (a
-     * matrix-core fragment) fall into 8 different 16-byte bank groups */
-    auto RS = [&](int layer, int row, int p, int c) {
-        return ring + (((layer - 1) * 2 + (row & 1)) * kFuPitch + p) * 32 + ((((c >> 3) ^ (p >> 1)) & 3) << 3) +
-               (c & 7);
-    };
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * kFuSW;
-    const int j = blockIdx.y;
-    const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
-    float* oj = out + (long)j * h * w;
+/* Element offset of channel c of pixel p within a ring row.  A pixel's 64
+ * bytes hold its four 16-byte chunks of 8 channels in the order
+ * chunk ^ ((p >> 1) & 3), so the 8 lanes of one 128-byte LDS access that read
+ * one chunk of 8 consecutive pixels (a matrix-core fragment) fall into 8
+ * different 16-byte bank groups.  Moving 16 pixels keeps the order. */
+__device__ __forceinline__ int fu_pix(int p, int c) {
+    return p * 32 + ((((c >> 3) ^ (p >> 1)) & 3) << 3) + (c & 7);
+}
+/* element offset of ring row `slot` (0, 1) of layer 1..5 */
+__device__ __forceinline__ int fu_row(int layer, int slot) { return ((layer - 1) * 2 + slot) * kFuPitch * 32; }
 
-    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
-    if (tid == 0) b6s = params[SurLayout::b6];
-    /* this wave's layer, channel half and B fragments (k = ci, n = co) */
-    const int lane = tid & 63, wave = tid >> 6;
+/* Tiles of a layer row per wave: wave k (EXTRA false) has tiles 0 and 1
+ * (both channel halves) and half 0 of tile 2, wave k + 4 (EXTRA) half 1 of
+ * tile 2 and tiles 3 and 4; local tile lt is tile lt + fu_t0(EXTRA). */
+__host__ __device__ constexpr int fu_t0(bool extra) { return extra ? 2 : 0; }
+__host__ __device__ constexpr bool fu_has(bool extra, int lt, int hf) {
+    return extra ? (lt != 0 || hf == 1) : (lt != 2 || hf == 0);
+}
+
+/* One wave of the fused kernel: its three tiles of layer k + 2; EXTRA
+ * (waves 4-7) adds the input row, a share of layer 1 and a quarter of
+ * layer 6. */
+template <bool EXTRA>
+__device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __restrict__ ring,
+                                        const float* __restrict__ xi, float* __restrict__ oj, float scale,
+                                        const float* __restrict__ params, const uint16_t* __restrict__ wbf, int k,
+                                        int lane, int x0, int h, int w) {
+    constexpr int T0 = fu_t0(EXTRA);
+    const int ml = 2 + k;
     const int li = lane & 15, g = lane >> 4;
-    const int ml = 2 + (wave >> 1), nb = wave & 1;
+    /* B fragments (k = ci, n = co) of both channel halves */
     const uint16_t* wl = wbf + (ml - 2) * 9216;
-    bf16x8 B[9];
+    bf16x8 B[2][9];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) B[t] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
-    /* biases of the channels this lane's outputs hold: 16 nb + 4 g + e */
-    f32x4 b4;
+    for (int hf = 0; hf < 2; ++hf)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) b4[e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
-    /* bit k: this lane's pixel of tile k lies inside the layer's columns and the image */
-    unsigned inmask = 0;
-    {
-        const int lcols = kFuSW + 2 * (6 - ml);
-        for (int k = 0; k < 5; ++k) {
-            const int p = 16 * k + li, gx = x0 - (6 - ml) + p;
-            if (p < lcols && gx >= 0 && gx < w) inmask |= 1u << k;
-        }
+        for (int t = 0; t < 9; ++t) B[hf][t] = *(const bf16x8*)&wl[(t * 32 + hf * 16 + li) * 32 + 8 * g];
+    /* biases of the channels this lane's outputs hold: 16 hf + 4 g + e */
+    f32x4 b4[2];
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b4[hf][e] = params[SurLayout::bl(ml) + 16 * hf + 4 * g + e];
+    /* lane offsets: fragment of pixel li + dx (chunk g), outputs of pixel li */
+    int rd[3], wr[2];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) rd[dx] = fu_pix(li + dx, 8 * g);
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) wr[hf] = fu_pix(li, 16 * hf + 4 * g);
+    const uint16_t* in = ring + fu_row(ml - 1, 0);
+    uint16_t* outr = ring + fu_row(ml, 0);
+    /* column of ring position p: x0 - (6 - ml) + p */
+    const int gx0 = x0 - (6 - ml);
+    const bool edge = gx0 + 16 * T0 < 0 || gx0 + 16 * (T0 + 3) > w;
+    unsigned keep = 0; /* bit lt: this lane's pixel of local tile lt is a column of the image */
+#pragma unroll
+    for (int lt = 0; lt < 3; ++lt) {
+        const int gx = gx0 + 16 * (T0 + lt) + li;
+        if (gx >= 0 && gx < w) keep |= 1u << lt;
     }
-    /* layer 1: thread = (channel l1co, pixels 5 l1pg ..), its BF16-rounded weights */
-    const int l1co = tid & 31, l1pg = tid >> 5;
-    float w1r[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
-    const float b1r = params[SurLayout::b1 + l1co];
-    unsigned l1mask = 0; /* bit k: pixel 5 l1pg + k is a column of the image */
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int gx = x0 - 5 + l1pg * 5 + k;
-        if (gx >= 0 && gx < w) l1mask |= 1u << k;
-    }
-    /* layer 6: thread = (pixel tid / 8, channels 4 (tid % 8) ..); weights of
-     * channel pairs packed as BF16 pairs for the two-term dot products */
-    bf16x2 w6p[2][9];
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int c = 4 * (tid & 7) + 2 * k;
-            w6p[k][t] = bf16x2{(__bf16)params[SurLayout::w6 + c * 9 + t], (__bf16)params[SurLayout::w6 + (c + 1) * 9 + t]};
-        }
-    /* input rows enter two steps ahead: row s + 2 is stored at step s (from
-     * a register loaded the step before) */
-    const int xq = tid, xgx = x0 - 6 + tid;
-    const bool xload = tid < kFuSW + 12 && xgx >= 0 && xgx < w;
-    float xnext = (xload && 0 < h) ? xi[xgx] : 0.f;
-    __syncthreads();
 
-    /* open sums: [k] belongs to the output row started at step u = s + 2
-     * with u % 3 == k */
+    /* ---- FP32 share (EXTRA): thread vt = 64 k + lane */
+    const int vt = k * 64 + lane;
+    /* input: thread vt < kFuXCols owns column x0 - 6 + vt; row s + 2 is
+     * stored at step s from a register loaded the step before */
+    const int xgx = x0 - 6 + vt;
+    const bool xown = EXTRA && vt < kFuXCols;
+    const bool xload = xown && xgx >= 0 && xgx < w;
+    float xnext = (xload && 0 < h) ? xi[xgx] : 0.f;
+    /* layer 1: channel half k & 1 of tiles (k >> 1) + 2i, i < n1 (a third
+     * unit of waves with n1 = 2 is computed on tile 4 and stored to the
+     * sink) */
+    const int hf1 = k & 1, t1 = k >> 1, n1 = t1 == 0 ? 3 : 2;
+    int w1o[3]; /* store offset of unit i within a layer-1 ring row */
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        w1o[i] = i < n1 ? wr[hf1] + (t1 + 2 * i) * 16 * 32
+                        : (int)(kFuRingBytes / 2) + lane * 4; /* the sink */
+    bf16x8 A1 = {};
+    f32x4 c1 = {0.f, 0.f, 0.f, 0.f};
+    unsigned keep1 = 0; /* bit i: this lane's pixel of tile t1 + 2i is a column of the image */
+    if (EXTRA) {
+        A1 = sur_first_a(params, 16 * hf1 + li, g);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c1[e] = params[SurLayout::b1 + 16 * hf1 + 4 * g + e];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int gx = x0 - 5 + 16 * (t1 + 2 * i) + li;
+            if (gx >= 0 && gx < w) keep1 |= 1u << i;
+        }
+    }
+    uint16_t* ring1 = ring + fu_row(1, 0);
+    /* layer 6 (EXTRA): output tile k on the matrix cores, one instruction
+     * per tap column dx: A6[dx] has M = tap row dy (rows 0-2 of 16) and
+     * K = the 32 channels, so D[dy][p] = sum over dx and channels of the
+     * dy-th tap row's products for pixel p, and lanes of group 0 hold the
+     * three tap rows' sums of their pixel */
+    bf16x8 A6[3];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+        A6[dx] = bf16x8{};
+        if (EXTRA && li < 3)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) A6[dx][e] = (__bf16)params[SurLayout::w6 + (8 * g + e) * 9 + 3 * li + dx];
+    }
+    const float b6 = params[SurLayout::b6];
+    const uint16_t* ring5 = ring + fu_row(5, 0);
+    const int gx6 = x0 + 16 * k + li;
     float a6[3] = {0.f, 0.f, 0.f};
-    f32x4 acc[3][5];
+
+    f32x4 acc[3][2][3];
     const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int c = 0; c < 5; ++c) acc[k][c] = z4;
+        for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+            for (int lt = 0; lt < 3; ++lt) acc[a][hf][lt] = z4;
 
     auto step = [&](auto kk, int s) {
-        constexpr int kn = decltype(kk)::value;  /* slot of the row started now */
-        constexpr int km = (kn + 2) % 3;         /* slot of the row started last step */
-        constexpr int kc = (kn + 1) % 3;         /* slot of the row completed now */
-        /* input row s + 2 to LDS; issue row s + 3 */
-        if (tid < kFuSW + 12) {
-            xr[((s + 2) & 3) * kFuXPitch + xq] = xnext * scale;
-            xnext = (xload && s + 3 >= 0 && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
-        }
-        auto layer1 = [&]() {
-            /* row s from input rows s-1..s+1: kFuSW + 10 columns x 32 channels */
-            const int r = s;
-            const int p0 = l1pg * 5;
-            if (r < 0 || r > h || p0 >= kFuSW + 10) return;
-            float xw[3][7];
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                for (int qq = 0; qq < 7; ++qq) xw[dy][qq] = xr[((r + dy - 1) & 3) * kFuXPitch + p0 + qq];
-            const bool rin = r < h;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                const int p = p0 + k;
-                /* computed everywhere, kept where the column is real */
-                float v = b1r;
-#pragma unroll
-                for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
-                v = (rin && ((l1mask >> k) & 1u)) ? fmaxf(v, 0.f) : 0.f;
-                if (p < kFuSW + 10) *RS(1, r, p, l1co) = f2bf(v);
-            }
+        constexpr int kn = decltype(kk)::value; /* slot of the row started now */
+        constexpr int km = (kn + 2) % 3;        /* slot of the row started last step */
+        constexpr int kc = (kn + 1) % 3;        /* slot of the row completed now */
+        /* ---- FP32 share, reads first: input row s + 2 to LDS (row s + 3
+         * issued), the layer-5 row q6 = s - 9 of layer 6 and the layer-1
+         * fragments of row s: K = tap row g (lane group) x column dx x
+         * (hi, lo) from the input ring; group 3 reads the ring's zero row */
+        const int q6 = s - 9;
+        const uint16_t* r5 = ring5 + (q6 & 1) * kFuPitch * 32 + k * 16 * 32;
+        const uint32_t* xrow = xr + (g < 3 ? ((s + g - 1) & 3) : 4) * kFuXPitch + li;
+        uint16_t* r1 = ring1 + (s & 1) * kFuPitch * 32;
+        const bool live1 = s >= 0 && s < h;
+        bf16x8 F6[3];
+        f32x4 d6 = z4;
+        bf16x8 F1[3];
+        f32x4 d1[3];
+        auto l6_read = [&](int dx) { F6[dx] = *(const bf16x8*)(r5 + rd[dx]); };
+        auto l6_mfma = [&](int dx) { d6 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A6[dx], F6[dx], d6, 0, 0, 0); };
+        auto l1_read = [&](int i) {
+            const uint32_t* xp = xrow + 16 * (i < n1 ? t1 + 2 * i : 4);
+            F1[i] = __builtin_bit_cast(bf16x8, make_uint4(xp[0], xp[1], xp[2], 0u));
         };
-        auto layers_mid = [&]() {
-            /* layer ml reads row q = s - 2 ml + 3 of layer ml - 1 and
-             * completes row q - 1; five tiles, three open rows each */
-            static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
-            const int q = s - 2 * ml + 3;
-            /* q = h + 1 only writes row h as zeros: the next layer reads it
-             * as padding and its slot still holds row h - 2 */
-            if (q < -1 || q > h + 1) return;
-            __builtin_amdgcn_s_setprio(1);
+        if (EXTRA) {
+            if (xown) {
+                xr[((s + 2) & 3) * kFuXPitch + vt] = x_pair(xnext * scale);
+                xnext = (xload && s + 3 >= 0 && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
+            }
+        }
+        /* layer-1 unit i: sums from the bias; row s: rows -2, -1 and h..
+         * as zeros, columns outside the image as zeros */
+        auto l1_mfma = [&](int i) { d1[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, F1[i], c1, 0, 0, 0); };
+        auto l1_store = [&](int i) {
+            const bool kp = live1 && ((keep1 >> i) & 1u);
+            *(uint2*)(r1 + w1o[i]) = make_uint2(kp ? relu_bf16x2(pack_bf16(d1[i][0], d1[i][1])) : 0u,
+                                                kp ? relu_bf16x2(pack_bf16(d1[i][2], d1[i][3])) : 0u);
+        };
+
+        const int q = s - 2 * ml + 3;
+        if (q >= -1 && q <= h) {
+            /* row o = q - 1 completes */
+            const int o = q - 1;
+            const uint16_t* row = in + (q & 1) * kFuPitch * 32;
+            uint16_t* orow = outr + (o & 1) * kFuPitch * 32;
+            /* D layout: lane holds channels 16 hf + 4g..4g+3 of pixel li of
+             * a tile: BF16, ReLU on the bits, one 8-byte store */
+            auto store = [&](int lt) {
 #pragma unroll
-            for (int dx = 0; dx < 3 && q <= h; ++dx) {
-                const uint16_t* row = RS(ml - 1, q, li + dx, 8 * g);
+                for (int hf = 0; hf < 2; ++hf)
+                    if (fu_has(EXTRA, lt, hf)) {
+                        const f32x4 a = acc[kc][hf][lt];
+                        *(uint2*)(orow + wr[hf] + (T0 + lt) * 16 * 32) =
+                            make_uint2(relu_bf16x2(pack_bf16(a[0], a[1])), relu_bf16x2(pack_bf16(a[2], a[3])));
+                    }
+            };
+            /* fragment of tap column i / 3, local tile i % 3; the next one
+             * is read while the matrix cores work on the current one */
+            auto frag = [&](int i) { return *(const bf16x8*)(row + rd[i / 3] + (T0 + i % 3) * 16 * 32); };
+            bf16x8 Fn = frag(0);
 #pragma unroll
-                for (int tile = 0; tile < 5; ++tile) {
-                    /* tiles are 16 pixels apart, which keeps the chunk order */
-                    const bf16x8 F = *(const bf16x8*)(row + tile * 16 * 32);
-                    /* weights as the first operand: D = W^T X^T, so a lane's
-                     * four outputs are 4 channels of one pixel; taps dx (new
-                     * row), 3 + dx (open row), 6 + dx (completing row) */
-                    acc[kn][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[dx], F, dx == 0 ? b4 : acc[kn][tile], 0, 0, 0);
-                    acc[km][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[3 + dx], F, acc[km][tile], 0, 0, 0);
-                    acc[kc][tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[6 + dx], F, acc[kc][tile], 0, 0, 0);
+            for (int i = 0; i < 9; ++i) {
+                const int dx = i / 3, lt = i % 3;
+                const bf16x8 F = Fn;
+                if (i + 1 < 9) Fn = frag(i + 1);
+                /* weights as the first operand: D = W^T X^T, so a lane's
+                 * four outputs are 4 channels of one pixel; taps dx (new
+                 * row), 3 + dx (open row), 6 + dx (completing row) */
+#pragma unroll
+                for (int hf = 0; hf < 2; ++hf)
+                    if (fu_has(EXTRA, lt, hf)) {
+                        acc[kn][hf][lt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            B[hf][dx], F, dx == 0 ? b4[hf] : acc[kn][hf][lt], 0, 0, 0);
+                        acc[km][hf][lt] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[hf][3 + dx], F, acc[km][hf][lt], 0, 0, 0);
+                        acc[kc][hf][lt] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[hf][6 + dx], F, acc[kc][hf][lt], 0, 0, 0);
+                    }
+                /* between the matrix-core instructions: the conversions
+                 * and stores of tile lt - 1 of row o (complete), and in
+                 * the EXTRA waves layers 1 and 6 (reads, instructions and
+                 * stores three fragments apart) */
+                if (dx == 2 && lt > 0) store(lt - 1);
+                if (EXTRA) {
+                    if (i <= 2) {
+                        l1_read(i);
+                        l6_read(i);
+                    } else if (i <= 5) {
+                        l1_mfma(i - 3);
+                        l6_mfma(i - 3);
+                    } else {
+                        l1_store(i - 6);
+                    }
+                }
+                __builtin_amdgcn_sched_group_barrier(0x100, 4, 0); /* the reads */
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* one MFMA */
+                    __builtin_amdgcn_sched_group_barrier(0x002, 2, 0); /* two VALU */
+                }
+                __builtin_amdgcn_sched_group_barrier(0x200, 3, 0); /* the stores */
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            store(2);
+            /* rows -2, -1 as zeros (the next layer reads row -1 as padding
+             * and reads neither slot at this step); columns outside the
+             * image as zeros */
+            if (o < 0) {
+#pragma unroll
+                for (int lt = 0; lt < 3; ++lt)
+#pragma unroll
+                    for (int hf = 0; hf < 2; ++hf)
+                        if (fu_has(EXTRA, lt, hf))
+                            *(uint2*)(orow + wr[hf] + (T0 + lt) * 16 * 32) = make_uint2(0u, 0u);
+            } else if (edge) {
+#pragma unroll
+                for (int lt = 0; lt < 3; ++lt)
+                    if (!((keep >> lt) & 1u))
+#pragma unroll
+                        for (int hf = 0; hf < 2; ++hf)
+                            if (fu_has(EXTRA, lt, hf))
+                                *(uint2*)(orow + wr[hf] + (T0 + lt) * 16 * 32) = make_uint2(0u, 0u);
+            }
+        } else {
+            if (q == h + 1) {
+                /* row h as zeros: the next layer reads it as padding (its
+                 * slot still holds row h - 2) */
+                uint16_t* orow = outr + (h & 1) * kFuPitch * 32;
+#pragma unroll
+                for (int lt = 0; lt < 3; ++lt)
+#pragma unroll
+                    for (int hf = 0; hf < 2; ++hf)
+                        if (fu_has(EXTRA, lt, hf))
+                            *(uint2*)(orow + wr[hf] + (T0 + lt) * 16 * 32) = make_uint2(0u, 0u);
+            }
+            if (EXTRA) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    l1_read(i);
+                    l6_read(i);
+                    l1_mfma(i);
+                    l6_mfma(i);
+                    l1_store(i);
                 }
             }
-            __builtin_amdgcn_s_setprio(0);
-            const int o = q - 1;
-            if (o < 0) return;
-            const bool live = o < h;
-            /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li of a
-             * tile (the sums started from the bias): BF16, ReLU, one 8-byte
-             * store */
-#pragma unroll
-            for (int tile = 0; tile < 5; ++tile) {
-                const f32x4 a = acc[kc][tile];
-                const bool keep = live && ((inmask >> tile) & 1u);
-                const uint2 v = make_uint2(keep ? relu_bf16x2(pack_bf16(a[0], a[1])) : 0u,
-                                           keep ? relu_bf16x2(pack_bf16(a[2], a[3])) : 0u);
-                *(uint2*)RS(ml, o, tile * 16 + li, 16 * nb + 4 * g) = v;
-            }
-        };
-        auto layer6 = [&]() {
-            /* reads row q = s - 9 of layer 5, completes output row s - 10 */
-            const int q = s - 9;
-            if (q < -1 || q > h) return;
-            const int p = tid >> 3, cg = tid & 7;
-            float vn = 0.f, vm = a6[km], vc = a6[kc];
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const uint2 a = *(const uint2*)RS(5, q, p + dx, 4 * cg);
-                const bf16x2 lo = __builtin_bit_cast(bf16x2, a.x), hi = __builtin_bit_cast(bf16x2, a.y);
-                /* BF16 x BF16 products are exact in FP32 */
-                vn = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][dx], vn, false);
-                vn = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][dx], vn, false);
-                vm = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][3 + dx], vm, false);
-                vm = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][3 + dx], vm, false);
-                vc = __builtin_amdgcn_fdot2_f32_bf16(lo, w6p[0][6 + dx], vc, false);
-                vc = __builtin_amdgcn_fdot2_f32_bf16(hi, w6p[1][6 + dx], vc, false);
-            }
-            a6[kn] = vn;
-            a6[km] = vm;
-            vc += __shfl_xor(vc, 1, 64);
-            vc += __shfl_xor(vc, 2, 64);
-            vc += __shfl_xor(vc, 4, 64);
-            const int o = q - 1, gx = x0 + p;
-            if (cg == 0 && o >= 0 && o < h && gx < w) oj[(long)o * w + gx] = vc + b6s;
-        };
-        /* the layers of a step touch disjoint LDS rows, so their order is
-         * free: waves 0-3 run the FP32 layers first, waves 4-7 the
-         * matrix-core layer first, and each SIMD (waves k and k+4) overlaps
-         * the two */
-        if (wave < 4) {
-            layer1();
-            layer6();
-            layers_mid();
-        } else {
-            layers_mid();
-            layer1();
-            layer6();
+        }
+        if (EXTRA) {
+            /* layer 6: row q6 is tap row 0 of output row q6 + 1 (started),
+             * 1 of row q6 and 2 of row q6 - 1 (complete) */
+            a6[kn] = d6[0];
+            a6[km] += d6[1];
+            const float v = a6[kc] + d6[2];
+            const int o6 = q6 - 1;
+            if (g == 0 && o6 >= 0 && o6 < h && gx6 < w) oj[(long)o6 * w + gx6] = v + b6;
         }
         lds_barrier();
     };
@@ -449,232 +563,26 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
     }
 }
 
-/* All six layers of the forward pass in one kernel, for inference.  A
- * workgroup streams a strip of kFuSW output columns of one image down its
- * rows; every layer's activations stay in LDS in a ring of 4 rows.  Layer l
- * (1..6) works on row s - 2(l-1) at step s, so all layers advance together
- * with one barrier per step: layer l reads rows r-1..r+1 of layer l-1, which
- * were written in earlier steps, while layer l-1 writes row r+2 into the
- * fourth slot.  The halo shrinks by one column per layer (layer l covers
- * kFuSW + 2(6-l) columns), and positions outside the image are written as
- * zeros, which is the zero padding of the layer-by-layer network, so layers
- * 1-5 produce the same bits as sur_conv_first / sur_conv_mid.  Waves 2k and
- * 2k+1 own layer k+2 (B fragments of both 16-channel halves in registers for
- * the whole kernel, pixel tiles split between the two waves); the FP32 layers
- * 1 and 6 are shared by all threads. */
-constexpr size_t kFuRingBytesV1 = (size_t)5 * 4 * kFuPitch * 32 * 2;
-constexpr size_t kFuLdsV1 = (size_t)4 * kFuXPitch * 4 + kFuRingBytesV1;
-
-__global__ __launch_bounds__(kFuThreads) void sur_fused_v1_kernel(const float* __restrict__ x, long pstride,
+__global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
                                                                long istride, float scale,
                                                                const float* __restrict__ params,
                                                                const uint16_t* __restrict__ wbf,
                                                                float* __restrict__ out, int h, int w) {
     extern __shared__ __align__(16) uint8_t fu_smem[];
-    __shared__ float b6s;
-    float* xr = (float*)fu_smem;                                  /* [4][kFuXPitch] */
-    uint16_t* ring = (uint16_t*)(fu_smem + 4 * kFuXPitch * 4);    /* [layer 1..5][4][kFuPitch][32] */
-    auto R = [&](int layer, int row, int p) { return ring + (((layer - 1) * 4 + (row & 3)) * kFuPitch + p) * 32; };
-    /* channel c of pixel p: a pixel's 64 bytes hold its four 16-byte chunks
-     * of 8 channels in the order chunk ^ ((p >> 1) & 3), so the 8 lanes of
-     * one 128-byte LDS access reading one chunk of 8 consecutive pixels (a
-     * matrix-core A fragment) fall into 8 different 16-byte bank groups
-     * (measured bank-conflict cycles: 39% of the kernel's cycles with the
-     * plain layout, 3.5% with this one); ring rows start on 256-byte
-     * boundaries */
-    auto RS = [&](int layer, int row, int p, int c) {
-        return R(layer, row, p) + ((((c >> 3) ^ (p >> 1)) & 3) << 3) + (c & 7);
-    };
+    uint32_t* xr = (uint32_t*)fu_smem;                 /* [5][kFuXPitch] */
+    uint16_t* ring = (uint16_t*)(fu_smem + kFuXBytes); /* [layer 1..5][2][kFuPitch][32] */
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kFuSW;
     const int j = blockIdx.y;
+    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    /* both kinds of wave run the same steps, one barrier each */
     const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
     float* oj = out + (long)j * h * w;
-
-    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
-    if (tid == 0) b6s = params[SurLayout::b6];
-    /* this wave's layer and its B fragments (k = ci, n = co) and biases */
-    const int lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, g = lane >> 4;
-    const int ml = 2 + (wave >> 1), half = wave & 1;
-    const uint16_t* wl = wbf + (ml - 2) * 9216;
-    bf16x8 B[9][2];
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) B[t][nb] = *(const bf16x8*)&wl[(t * 32 + nb * 16 + li) * 32 + 8 * g];
-    /* biases of the channels this lane's outputs hold: 16 nb + 4 g + e */
-    float bias[2][4];
-#pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) bias[nb][e] = params[SurLayout::bl(ml) + 16 * nb + 4 * g + e];
-    /* layer 1: thread = (channel l1co, pixels 5 l1pg ..); layer 6: thread =
-     * (pixel tid / 8, channels 4 (tid % 8) ..); their BF16-rounded weights */
-    /* whether this lane's pixel of each of its mid-layer tiles lies inside the
-     * layer's columns and the image (bit 0: tile 2h, 1: tile 2h+1, 2: tile 4) */
-    unsigned inmask = 0;
-    {
-        const int wl_cols = kFuSW + 2 * (6 - ml);
-        const int ps[3] = {2 * half * 16 + li, (2 * half + 1) * 16 + li, 64 + li};
-        for (int k = 0; k < 3; ++k) {
-            const int gx = x0 - (6 - ml) + ps[k];
-            if (ps[k] < wl_cols && gx >= 0 && gx < w) inmask |= 1u << k;
-        }
-    }
-    const int l1co = tid & 31, l1pg = tid >> 5;
-    float w1r[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) w1r[t] = bf2f(f2bf(params[SurLayout::w1 + l1co * 9 + t]));
-    const float b1r = params[SurLayout::b1 + l1co];
-    /* layer 6 weights of channel pairs (4 cg, 4 cg + 1) and (4 cg + 2, 4 cg + 3),
-     * packed as BF16 pairs for the two-term dot products */
-    bf16x2 w6p[2][9];
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int c = 4 * (tid & 7) + 2 * k;
-            w6p[k][t] = bf16x2{(__bf16)params[SurLayout::w6 + c * 9 + t], (__bf16)params[SurLayout::w6 + (c + 1) * 9 + t]};
-        }
-    __syncthreads();
-    /* input rows 0 and 1 (row -1 stays zero) */
-    for (int i = tid; i < 2 * (kFuSW + 12); i += blockDim.x) {
-        const int r = i / (kFuSW + 12), q = i - r * (kFuSW + 12);
-        const int gx = x0 - 6 + q;
-        xr[r * kFuXPitch + q] = (r < h && gx >= 0 && gx < w) ? xi[(long)r * w + gx] * scale : 0.f;
-    }
-    /* input rows are loaded one step before they are stored to LDS, so the
-     * load's latency overlaps a step of work */
-    const int xq = tid, xgx = x0 - 6 + tid;
-    const bool xload = tid < kFuSW + 12 && xgx >= 0 && xgx < w;
-    float xnext = (xload && 2 < h) ? xi[2L * w + xgx] : 0.f;
-    __syncthreads();
-
-    for (int s = 0; s < h + 10; ++s) {
-        /* input row s + 2 (loaded last step); issue row s + 3 */
-        if (tid < kFuSW + 12) {
-            xr[((s + 2) & 3) * kFuXPitch + xq] = xnext * scale;
-            xnext = (xload && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
-        }
-        auto layer1 = [&]() {
-        /* layer 1, row s: kFuSW + 10 columns x 32 channels; thread = (channel,
-         * 5 adjacent pixels) with the channel's weights in registers */
-        {
-            const int r = s;
-            const int p0 = l1pg * 5;
-            if (p0 < kFuSW + 10) {
-                float xw[3][7];
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) xw[dy][q] = xr[((r + dy - 1) & 3) * kFuXPitch + p0 + q];
-#pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    const int p = p0 + k;
-                    const int gx = x0 - 5 + p;
-                    float v = 0.f;
-                    if (r < h && gx >= 0 && gx < w) {
-                        v = b1r;
-#pragma unroll
-                        for (int t = 0; t < 9; ++t) v += w1r[t] * xw[t / 3][k + t % 3];
-                        v = fmaxf(v, 0.f);
-                    }
-                    if (p < kFuSW + 10) *RS(1, r, p, l1co) = f2bf(v);
-                }
-            }
-        }
-        };
-        auto layers_mid = [&]() {
-        /* layers 2-5 on the matrix cores: a layer row is 5 pixel tiles x 2
-         * channel halves; wave `half` of the layer takes tiles 2h and 2h+1
-         * (both halves, one A load per two instructions) and half h of tile
-         * 4, as five independent accumulator chains */
-        {
-            static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
-            const int r = s - 2 * (ml - 1);
-            const int wl_cols = kFuSW + 2 * (6 - ml);
-            if (r >= 0) {
-                const bool live = r < h;
-                const int ta = 2 * half, tb = 2 * half + 1;
-                f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0, b0 = a0, b1 = a0, c0 = a0;
-                if (live) {
-                    /* the matrix-core chain issues first when both waves of
-                     * a SIMD are ready; the other wave's FP32 layers fill in */
-                    __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-                    for (int t = 0; t < 9; ++t) {
-                        /* tiles are 16 pixels apart, which keeps the chunk order */
-                        const uint16_t* row = RS(ml - 1, r + t / 3 - 1, li + t % 3, 8 * g);
-                        const bf16x8 A = *(const bf16x8*)(row + ta * 16 * 32);
-                        const bf16x8 Bv = *(const bf16x8*)(row + tb * 16 * 32);
-                        const bf16x8 C = *(const bf16x8*)(row + 4 * 16 * 32);
-                        /* weights as the first operand: D = W^T X^T, so a
-                         * lane's four outputs are 4 channels of one pixel */
-                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][0], A, a0, 0, 0, 0);
-                        b0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][0], Bv, b0, 0, 0, 0);
-                        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(half ? B[t][1] : B[t][0], C, c0, 0, 0, 0);
-                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], A, a1, 0, 0, 0);
-                        b1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[t][1], Bv, b1, 0, 0, 0);
-                    }
-                    __builtin_amdgcn_s_setprio(0);
-                }
-                /* D layout: lane holds channels 16 nb + 4g..4g+3 of pixel li
-                 * of a tile: bias, ReLU, BF16, one 8-byte store */
-                auto put = [&](const f32x4& acc, int nb, int tile, unsigned bit) {
-                    uint2 v = make_uint2(0u, 0u);
-                    if (live && ((inmask >> bit) & 1u)) {
-                        v.x = (uint32_t)f2bf(fmaxf(acc[0] + bias[nb][0], 0.f)) |
-                              ((uint32_t)f2bf(fmaxf(acc[1] + bias[nb][1], 0.f)) << 16);
-                        v.y = (uint32_t)f2bf(fmaxf(acc[2] + bias[nb][2], 0.f)) |
-                              ((uint32_t)f2bf(fmaxf(acc[3] + bias[nb][3], 0.f)) << 16);
-                    }
-                    *(uint2*)RS(ml, r, tile * 16 + li, 16 * nb + 4 * g) = v;
-                };
-                put(a0, 0, ta, 0);
-                put(a1, 1, ta, 0);
-                put(b0, 0, tb, 1);
-                put(b1, 1, tb, 1);
-                put(c0, half, 4, 2);
-            }
-        }
-        };
-        auto layer6 = [&]() {
-        /* layer 6, row s - 10: thread = (pixel, group of 4 channels) */
-        {
-            const int r = s - 10;
-            const int p = tid >> 3, cg = tid & 7;
-            float v = 0.f;
-            if (r >= 0 && r < h) {
-#pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    const uint2 a = *(const uint2*)RS(5, r + t / 3 - 1, p + t % 3, 4 * cg);
-                    /* BF16 x BF16 products are exact in FP32 */
-                    v = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a.x), w6p[0][t], v, false);
-                    v = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a.y), w6p[1][t], v, false);
-                }
-            }
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            const int gx = x0 + p;
-            if (cg == 0 && r >= 0 && r < h && gx < w) oj[(long)r * w + gx] = v + b6s;
-        }
-        };
-        /* the layers of a step touch disjoint ring rows, so their order is
-         * free: waves 0-3 run the FP32 layers first, waves 4-7 the matrix-core
-         * layer first, and each SIMD (waves k and k+4) overlaps the two */
-        if (wave < 4) {
-            layer1();
-            layer6();
-            layers_mid();
-        } else {
-            layers_mid();
-            layer1();
-            layer6();
-        }
-        lds_barrier();
-    }
+    if (tid < 256)
+        fu_wave<false>(xr, ring, xi, oj, scale, params, wbf, tid >> 6, tid & 63, x0, h, w);
+    else
+        fu_wave<true>(xr, ring, xi, oj, scale, params, wbf, (tid >> 6) - 4, tid & 63, x0, h, w);
 }
 
 /* ---- backward ---------------------------------------------------------- */
@@ -1143,7 +1051,8 @@ void Surrogate::ensure_acts(int nimg, int h, int w) {
 static void launch_forward(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale,
                            uint16_t* const a[5], float* out, int nimg, int h, int w) {
     const long P = (long)nimg * h * w;
-    c.launch("sur_conv1", sur_conv_first, dim3(ceil_div(P, kBlock)), dim3(kBlock), 0, x, ps, is, scale,
+    const long tiles1 = (long)nimg * h * ceil_div(w, 16);
+    c.launch("sur_conv1", sur_conv_first, dim3(ceil_div(tiles1, kBlock / 64)), dim3(kBlock), 0, x, ps, is, scale,
              (const float*)m.master, a[0], nimg, h, w);
     const dim3 g(ceil_div(w, 16), ceil_div(h, 16), nimg);
     for (int l = 2; l <= 5; ++l)
@@ -1161,11 +1070,7 @@ void Surrogate::forward(int nimg, int h, int w) {
 static void launch_fused(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale, float* out,
                          int nimg, int h, int w) {
     IQM_CHECK(nimg <= 65535, IQM_ERR_INVALID, "too many images for one fused launch");
-    if (std::getenv("IQM_SUR_FUSED_V1"))
-        c.launch("sur_fused", sur_fused_v1_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLdsV1, x,
-                 ps, is, scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
-    else
-        c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps,
+    c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps,
                  is, scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
 }
 
