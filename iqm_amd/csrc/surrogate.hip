@@ -266,20 +266,18 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
  * covers kFuSW + 2(6-l) columns; the five tiles of a row cover more, and
  * those extra columns only feed extra columns), and columns outside the
  * image are zeros, the zero padding of the layer-by-layer network. */
-constexpr int kFuSW = 64;      /* output columns per strip */
-constexpr int kFuPitch = 84;   /* ring row pitch (pixels) */
-constexpr int kFuXPitch = 96;  /* input ring row pitch (BF16 pairs); rows 0-3 the ring, row 4 zeros */
-constexpr int kFuXCols = 82;   /* input columns x0 - 6 .. x0 + 75 */
+constexpr int kFuSW = 72;      /* output columns per strip */
+constexpr int kFuPitch = 96;   /* ring row pitch (pixels): layer 1 computes six tiles */
+constexpr int kFuXPitch = 112; /* input ring row pitch (BF16 pairs); rows 0-3 the ring, row 4 zeros */
+constexpr int kFuXCols = 98;   /* input columns x0 - 6 .. x0 + 91 */
 constexpr int kFuThreads = 512;
 constexpr size_t kFuRingBytes = (size_t)5 * 2 * kFuPitch * 32 * 2;
-constexpr size_t kFuXBytes = 2048; /* input rows, then the rings */
-/* where a dropped layer-1 unit stores: 64 lanes x 8 bytes past either
- * layer-1 row slot's base */
-constexpr size_t kFuSinkBytes = (size_t)kFuPitch * 32 * 2 + 64 * 8;
-constexpr size_t kFuLds = kFuXBytes + kFuRingBytes + kFuSinkBytes;
+constexpr size_t kFuXBytes = 2304; /* input rows, then the rings */
+constexpr size_t kFuLds = kFuXBytes + kFuRingBytes;
 static_assert(5 * kFuXPitch * 4 <= (int)kFuXBytes && kFuXBytes % 256 == 0, "input ring, rings on 256-byte boundaries");
 static_assert(kFuPitch % 4 == 0 && kFuXCols <= kFuXPitch, "ring pitches");
-static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per layer row");
+static_assert((kFuSW + 8 + 15) / 16 == 5 && (kFuSW + 2 + 15) / 16 == 5, "five tiles per row of layers 2-5");
+static_assert((kFuSW + 10 + 15) / 16 == 6 && kFuPitch >= 6 * 16 && kFuXCols >= 6 * 16 + 2, "six tiles per layer-1 row");
 
 /* Element offset of channel c of pixel p within a ring row.  A pixel's 64
  * bytes hold its four 16-byte chunks of 8 channels in the order
@@ -345,20 +343,17 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
     /* ---- FP32 share (EXTRA): thread vt = 64 k + lane */
     const int vt = k * 64 + lane;
     /* input: thread vt < kFuXCols owns column x0 - 6 + vt; row s + 2 is
-     * stored at step s from a register loaded the step before */
+     * loaded when step s starts and stored when it ends (rows and columns
+     * outside the image as zeros), by buffer loads whose row offset is a
+     * scalar and whose column offset is a fixed lane register */
     const int xgx = x0 - 6 + vt;
     const bool xown = EXTRA && vt < kFuXCols;
-    const bool xload = xown && xgx >= 0 && xgx < w;
-    float xnext = (xload && 0 < h) ? xi[xgx] : 0.f;
-    /* layer 1: channel half k & 1 of tiles (k >> 1) + 2i, i < n1 (a third
-     * unit of waves with n1 = 2 is computed on tile 4 and stored to the
-     * sink) */
-    const int hf1 = k & 1, t1 = k >> 1, n1 = t1 == 0 ? 3 : 2;
-    int w1o[3]; /* store offset of unit i within a layer-1 ring row */
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        w1o[i] = i < n1 ? wr[hf1] + (t1 + 2 * i) * 16 * 32
-                        : (int)(kFuRingBytes / 2) + lane * 4; /* the sink */
+    const bool xcol = xown && xgx >= 0 && xgx < w;
+    const __amdgpu_buffer_rsrc_t xrs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)xi, (short)0, (int)(h * w * 4), 0x00020000);
+    const int xoff = min(max(xgx, 0), w - 1) * 4;
+    /* layer 1: channel half k & 1 of tiles (k >> 1) + 2i, i < 3 */
+    const int hf1 = k & 1, t1 = k >> 1;
     bf16x8 A1 = {};
     f32x4 c1 = {0.f, 0.f, 0.f, 0.f};
     unsigned keep1 = 0; /* bit i: this lane's pixel of tile t1 + 2i is a column of the image */
@@ -373,7 +368,7 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
         }
     }
     uint16_t* ring1 = ring + fu_row(1, 0);
-    /* layer 6 (EXTRA): output tile k on the matrix cores, one instruction
+    /* layer 6 (not EXTRA): output tile k on the matrix cores, one instruction
      * per tap column dx: A6[dx] has M = tap row dy (rows 0-2 of 16) and
      * K = the 32 channels, so D[dy][p] = sum over dx and channels of the
      * dy-th tap row's products for pixel p, and lanes of group 0 hold the
@@ -382,7 +377,7 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) {
         A6[dx] = bf16x8{};
-        if (EXTRA && li < 3)
+        if (!EXTRA && li < 3)
 #pragma unroll
             for (int e = 0; e < 8; ++e) A6[dx][e] = (__bf16)params[SurLayout::w6 + (8 * g + e) * 9 + 3 * li + dx];
     }
@@ -390,6 +385,8 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
     const uint16_t* ring5 = ring + fu_row(5, 0);
     const int gx6 = x0 + 16 * k + li;
     float a6[3] = {0.f, 0.f, 0.f};
+    /* wave 0 also has output tile 4 (columns 64 .. kFuSW - 1 of the strip) */
+    float a6b[3] = {0.f, 0.f, 0.f};
 
     f32x4 acc[3][2][3];
     const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
@@ -420,21 +417,19 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
         auto l6_read = [&](int dx) { F6[dx] = *(const bf16x8*)(r5 + rd[dx]); };
         auto l6_mfma = [&](int dx) { d6 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A6[dx], F6[dx], d6, 0, 0, 0); };
         auto l1_read = [&](int i) {
-            const uint32_t* xp = xrow + 16 * (i < n1 ? t1 + 2 * i : 4);
+            const uint32_t* xp = xrow + 16 * (t1 + 2 * i);
             F1[i] = __builtin_bit_cast(bf16x8, make_uint4(xp[0], xp[1], xp[2], 0u));
         };
-        if (EXTRA) {
-            if (xown) {
-                xr[((s + 2) & 3) * kFuXPitch + vt] = x_pair(xnext * scale);
-                xnext = (xload && s + 3 >= 0 && s + 3 < h) ? xi[(long)(s + 3) * w + xgx] : 0.f;
-            }
-        }
+        float xv = 0.f;
+        if (EXTRA && k < 2) /* the waves holding input columns */
+            xv = __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(xrs, xoff, min(max(s + 2, 0), h - 1) * w * 4, 0));
         /* layer-1 unit i: sums from the bias; row s: rows -2, -1 and h..
          * as zeros, columns outside the image as zeros */
         auto l1_mfma = [&](int i) { d1[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, F1[i], c1, 0, 0, 0); };
         auto l1_store = [&](int i) {
             const bool kp = live1 && ((keep1 >> i) & 1u);
-            *(uint2*)(r1 + w1o[i]) = make_uint2(kp ? relu_bf16x2(pack_bf16(d1[i][0], d1[i][1])) : 0u,
+            *(uint2*)(r1 + wr[hf1] + (t1 + 2 * i) * 16 * 32) = make_uint2(kp ? relu_bf16x2(pack_bf16(d1[i][0], d1[i][1])) : 0u,
                                                 kp ? relu_bf16x2(pack_bf16(d1[i][2], d1[i][3])) : 0u);
         };
 
@@ -478,20 +473,22 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
                             __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[hf][6 + dx], F, acc[kc][hf][lt], 0, 0, 0);
                     }
                 /* between the matrix-core instructions: the conversions
-                 * and stores of tile lt - 1 of row o (complete), and in
-                 * the EXTRA waves layers 1 and 6 (reads, instructions and
-                 * stores three fragments apart) */
+                 * and stores of tile lt - 1 of row o (complete), and
+                 * layer 1 (EXTRA waves) or layer 6 (the others): reads,
+                 * instructions and stores three fragments apart */
                 if (dx == 2 && lt > 0) store(lt - 1);
                 if (EXTRA) {
-                    if (i <= 2) {
+                    if (i <= 2)
                         l1_read(i);
-                        l6_read(i);
-                    } else if (i <= 5) {
+                    else if (i <= 5)
                         l1_mfma(i - 3);
-                        l6_mfma(i - 3);
-                    } else {
+                    else
                         l1_store(i - 6);
-                    }
+                } else {
+                    if (i <= 2)
+                        l6_read(i);
+                    else if (i <= 5)
+                        l6_mfma(i - 3);
                 }
                 __builtin_amdgcn_sched_group_barrier(0x100, 4, 0); /* the reads */
 #pragma unroll
@@ -534,18 +531,19 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
                         if (fu_has(EXTRA, lt, hf))
                             *(uint2*)(orow + wr[hf] + (T0 + lt) * 16 * 32) = make_uint2(0u, 0u);
             }
-            if (EXTRA) {
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
+            for (int i = 0; i < 3; ++i) {
+                if (EXTRA) {
                     l1_read(i);
-                    l6_read(i);
                     l1_mfma(i);
-                    l6_mfma(i);
                     l1_store(i);
+                } else {
+                    l6_read(i);
+                    l6_mfma(i);
                 }
             }
         }
-        if (EXTRA) {
+        if (!EXTRA) {
             /* layer 6: row q6 is tap row 0 of output row q6 + 1 (started),
              * 1 of row q6 and 2 of row q6 - 1 (complete) */
             a6[kn] = d6[0];
@@ -553,9 +551,41 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
             const float v = a6[kc] + d6[2];
             const int o6 = q6 - 1;
             if (g == 0 && o6 >= 0 && o6 < h && gx6 < w) oj[(long)o6 * w + gx6] = v + b6;
+            if (k == 0) {
+                const uint16_t* r5b = ring5 + (q6 & 1) * kFuPitch * 32 + 4 * 16 * 32;
+                f32x4 e = z4;
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+                    e = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A6[dx], *(const bf16x8*)(r5b + rd[dx]), e, 0, 0, 0);
+                a6b[kn] = e[0];
+                a6b[km] += e[1];
+                const float vb = a6b[kc] + e[2];
+                if (g == 0 && li < kFuSW - 64 && o6 >= 0 && o6 < h && gx6 + 64 < w)
+                    oj[(long)o6 * w + gx6 + 64] = vb + b6;
+            }
+        } else if (k < 2) {
+            /* input row s + 2, read by layer 1 from the next step on */
+            const int r = s + 2;
+            const float v = (xcol && r >= 0 && r < h) ? xv * scale : 0.f;
+            if (xown) xr[(r & 3) * kFuXPitch + vt] = x_pair(v);
         }
         lds_barrier();
     };
+    /* every set-up load has arrived before the loop (an empty asm that reads
+     * the values makes the compiler wait for them here), so the loop waits
+     * only for its own */
+    auto ready = [](const auto& v) { asm volatile("" ::"v"(v)); };
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        ready(b4[hf]);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) ready(B[hf][t]);
+    }
+    ready(A1);
+    ready(c1);
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) ready(A6[dx]);
+    ready(b6);
     for (int u = 0; u < h + 12; u += 3) {
         step(std::integral_constant<int, 0>{}, u - 2);
         step(std::integral_constant<int, 1>{}, u - 1);
