@@ -23,6 +23,7 @@ Ctx::Ctx(int device) : device_(device) {
             std::to_string(prop.multiProcessorCount) + " CUs, " +
             std::to_string(prop.sharedMemPerBlock / 1024) + " KiB LDS/block)";
     lds_per_block_ = prop.sharedMemPerBlock;
+    cus_ = std::max(1, prop.multiProcessorCount);
     if (const char* g = std::getenv("IQM_GRAPHS")) graphs_ = std::atoi(g) != 0;
     ssim_noalloc_setup(device);
     IQM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));

@@ -129,6 +129,7 @@ class Ctx {
     Ctx& operator=(const Ctx&) = delete;
 
     int device() const { return device_; }
+    int cus() const { return cus_; } /* compute units of the device */
     /* LDS bytes one workgroup may allocate on this device */
     size_t lds_per_block() const { return lds_per_block_; }
     int xcds() const { return xcds_; }
@@ -298,6 +299,7 @@ class Ctx {
     void* host_staging(size_t bytes);
 
     int device_ = 0;
+    int cus_ = 1;
     size_t lds_per_block_ = 65536;
     int xcds_ = 8;
     hipStream_t stream_ = nullptr;

@@ -238,8 +238,10 @@ __global__ __launch_bounds__(kBlock) void sur_conv_last(const uint16_t* __restri
 }
 
 /* All six layers of the forward pass in one kernel, for inference.  A
- * workgroup streams a strip of kFuSW output columns of one image down its
- * rows; every step s ends with one barrier.  The 3x3 layers 2-5 run on the
+ * workgroup streams strips of kFuSW output columns of one image down their
+ * rows (a persistent grid: each workgroup takes an even share of all strip
+ * rows, as segments with six halo rows on either side); every step s ends
+ * with one barrier.  The 3x3 layers 2-5 run on the
  * matrix cores: at step s layer l reads one row q = s - 2l + 3 of the layer
  * below from LDS and applies the three tap rows of its kernel to it at once:
  * row q is tap row 0 of output row q + 1 (whose sums start here, from the
@@ -303,9 +305,10 @@ __host__ __device__ constexpr bool fu_has(bool extra, int lt, int hf) {
  * layer 6. */
 template <bool EXTRA>
 __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __restrict__ ring,
-                                        const float* __restrict__ xi, float* __restrict__ oj, float scale,
-                                        const float* __restrict__ params, const uint16_t* __restrict__ wbf, int k,
-                                        int lane, int x0, int h, int w) {
+                                        const float* __restrict__ x, long pstride, long istride,
+                                        float* __restrict__ out, float scale, const float* __restrict__ params,
+                                        const uint16_t* __restrict__ wbf, int k, int lane, int tid, int nimg,
+                                        int himg, int w) {
     constexpr int T0 = fu_t0(EXTRA);
     const int ml = 2 + k;
     const int li = lane & 15, g = lane >> 4;
@@ -328,8 +331,57 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
     for (int dx = 0; dx < 3; ++dx) rd[dx] = fu_pix(li + dx, 8 * g);
 #pragma unroll
     for (int hf = 0; hf < 2; ++hf) wr[hf] = fu_pix(li, 16 * hf + 4 * g);
+    /* layer 1: channel half k & 1 of tiles (k >> 1) + 2i, i < 3 */
+    const int hf1 = k & 1, t1 = k >> 1;
+    bf16x8 A1 = {};
+    f32x4 c1 = {0.f, 0.f, 0.f, 0.f};
+    if (EXTRA) {
+        A1 = sur_first_a(params, 16 * hf1 + li, g);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c1[e] = params[SurLayout::b1 + 16 * hf1 + 4 * g + e];
+    }
+    /* layer 6 (not EXTRA): output tile k on the matrix cores, one instruction
+     * per tap column dx: A6[dx] has M = tap row dy (rows 0-2 of 16) and
+     * K = the 32 channels, so D[dy][p] = sum over dx and channels of the
+     * dy-th tap row's products for pixel p, and lanes of group 0 hold the
+     * three tap rows' sums of their pixel */
+    bf16x8 A6[3];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+        A6[dx] = bf16x8{};
+        if (!EXTRA && li < 3)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) A6[dx][e] = (__bf16)params[SurLayout::w6 + (8 * g + e) * 9 + 3 * li + dx];
+    }
+    const float b6 = params[SurLayout::b6];
+    /* every set-up load has arrived before the segments (an empty asm that
+     * reads the values makes the compiler wait for them here), so the loop
+     * waits only for its own */
+    auto ready = [](const auto& v) { asm volatile("" ::"v"(v)); };
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+        ready(b4[hf]);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) ready(B[hf][t]);
+    }
+    ready(A1);
+    ready(c1);
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) ready(A6[dx]);
+    ready(b6);
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
     const uint16_t* in = ring + fu_row(ml - 1, 0);
     uint16_t* outr = ring + fu_row(ml, 0);
+    uint16_t* ring1 = ring + fu_row(1, 0);
+    const uint16_t* ring5 = ring + fu_row(5, 0);
+
+    /* One segment: output rows olo .. ohi - 1 of the h rows of a strip at
+     * column x0 that start at xi (input) and oj (output).  Rows outside
+     * 0 .. h - 1 are zeros; the rows computed for the segment reach six rows
+     * beyond the stored ones on either side (where the strip has them), the
+     * depth of the network, so zeros that stand in for real rows never reach
+     * a stored output. */
+    auto segment = [&](const float* __restrict__ xi, float* __restrict__ oj, int x0, int h, int olo, int ohi) {
     /* column of ring position p: x0 - (6 - ml) + p */
     const int gx0 = x0 - (6 - ml);
     const bool edge = gx0 + 16 * T0 < 0 || gx0 + 16 * (T0 + 3) > w;
@@ -352,44 +404,18 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
     const __amdgpu_buffer_rsrc_t xrs =
         __builtin_amdgcn_make_buffer_rsrc((void*)xi, (short)0, (int)(h * w * 4), 0x00020000);
     const int xoff = min(max(xgx, 0), w - 1) * 4;
-    /* layer 1: channel half k & 1 of tiles (k >> 1) + 2i, i < 3 */
-    const int hf1 = k & 1, t1 = k >> 1;
-    bf16x8 A1 = {};
-    f32x4 c1 = {0.f, 0.f, 0.f, 0.f};
-    unsigned keep1 = 0; /* bit i: this lane's pixel of tile t1 + 2i is a column of the image */
-    if (EXTRA) {
-        A1 = sur_first_a(params, 16 * hf1 + li, g);
+    unsigned keep1 = 0; /* bit i: this lane's pixel of layer-1 tile t1 + 2i is a column of the image */
 #pragma unroll
-        for (int e = 0; e < 4; ++e) c1[e] = params[SurLayout::b1 + 16 * hf1 + 4 * g + e];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int gx = x0 - 5 + 16 * (t1 + 2 * i) + li;
-            if (gx >= 0 && gx < w) keep1 |= 1u << i;
-        }
+    for (int i = 0; i < 3; ++i) {
+        const int gx = x0 - 5 + 16 * (t1 + 2 * i) + li;
+        if (gx >= 0 && gx < w) keep1 |= 1u << i;
     }
-    uint16_t* ring1 = ring + fu_row(1, 0);
-    /* layer 6 (not EXTRA): output tile k on the matrix cores, one instruction
-     * per tap column dx: A6[dx] has M = tap row dy (rows 0-2 of 16) and
-     * K = the 32 channels, so D[dy][p] = sum over dx and channels of the
-     * dy-th tap row's products for pixel p, and lanes of group 0 hold the
-     * three tap rows' sums of their pixel */
-    bf16x8 A6[3];
-#pragma unroll
-    for (int dx = 0; dx < 3; ++dx) {
-        A6[dx] = bf16x8{};
-        if (!EXTRA && li < 3)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) A6[dx][e] = (__bf16)params[SurLayout::w6 + (8 * g + e) * 9 + 3 * li + dx];
-    }
-    const float b6 = params[SurLayout::b6];
-    const uint16_t* ring5 = ring + fu_row(5, 0);
     const int gx6 = x0 + 16 * k + li;
     float a6[3] = {0.f, 0.f, 0.f};
     /* wave 0 also has output tile 4 (columns 64 .. kFuSW - 1 of the strip) */
     float a6b[3] = {0.f, 0.f, 0.f};
 
     f32x4 acc[3][2][3];
-    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -550,7 +576,7 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
             a6[km] += d6[1];
             const float v = a6[kc] + d6[2];
             const int o6 = q6 - 1;
-            if (g == 0 && o6 >= 0 && o6 < h && gx6 < w) oj[(long)o6 * w + gx6] = v + b6;
+            if (g == 0 && o6 >= olo && o6 < ohi && gx6 < w) oj[(long)o6 * w + gx6] = v + b6;
             if (k == 0) {
                 const uint16_t* r5b = ring5 + (q6 & 1) * kFuPitch * 32 + 4 * 16 * 32;
                 f32x4 e = z4;
@@ -560,7 +586,7 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
                 a6b[kn] = e[0];
                 a6b[km] += e[1];
                 const float vb = a6b[kc] + e[2];
-                if (g == 0 && li < kFuSW - 64 && o6 >= 0 && o6 < h && gx6 + 64 < w)
+                if (g == 0 && li < kFuSW - 64 && o6 >= olo && o6 < ohi && gx6 + 64 < w)
                     oj[(long)o6 * w + gx6 + 64] = vb + b6;
             }
         } else if (k < 2) {
@@ -571,48 +597,53 @@ __device__ __forceinline__ void fu_wave(uint32_t* __restrict__ xr, uint16_t* __r
         }
         lds_barrier();
     };
-    /* every set-up load has arrived before the loop (an empty asm that reads
-     * the values makes the compiler wait for them here), so the loop waits
-     * only for its own */
-    auto ready = [](const auto& v) { asm volatile("" ::"v"(v)); };
-#pragma unroll
-    for (int hf = 0; hf < 2; ++hf) {
-        ready(b4[hf]);
-#pragma unroll
-        for (int t = 0; t < 9; ++t) ready(B[hf][t]);
-    }
-    ready(A1);
-    ready(c1);
-#pragma unroll
-    for (int dx = 0; dx < 3; ++dx) ready(A6[dx]);
-    ready(b6);
+    /* the rings start as zeros */
+    for (int i = tid; i < (int)(kFuLds / 16); i += kFuThreads) ((uint4*)xr)[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
     for (int u = 0; u < h + 12; u += 3) {
         step(std::integral_constant<int, 0>{}, u - 2);
         step(std::integral_constant<int, 1>{}, u - 1);
         step(std::integral_constant<int, 2>{}, u);
     }
+    };
+
+    /* this workgroup's rows of the strips (image j, strip sx) laid end to
+     * end, strip after strip: an even share of all of them, so every
+     * workgroup ends at about the same time */
+    const int S = (w + kFuSW - 1) / kFuSW;
+    const long R = (long)nimg * S * himg;
+    const long L = (R + gridDim.x - 1) / gridDim.x;
+    long g0 = (long)blockIdx.x * L;
+    const long g1 = min(R, g0 + L);
+    while (g0 < g1) {
+        const long ks = g0 / himg;
+        const int r0 = (int)(g0 - ks * himg);
+        const int r1 = (int)min((long)himg, r0 + (g1 - g0));
+        const int j = (int)(ks / S), sx = (int)(ks - (long)j * S);
+        const int top = max(0, r0 - 6), bot = min(himg, r1 + 6);
+        segment(x + (long)(j >> 1) * pstride + (long)(j & 1) * istride + (long)top * w,
+                out + ((long)j * himg + top) * w, sx * kFuSW, bot - top, r0 - top, r1 - top);
+        g0 += r1 - r0;
+    }
 }
 
+/* A persistent grid: each workgroup takes an even share of the rows of all
+ * strips (fu_wave). */
 __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __restrict__ x, long pstride,
                                                                long istride, float scale,
                                                                const float* __restrict__ params,
                                                                const uint16_t* __restrict__ wbf,
-                                                               float* __restrict__ out, int h, int w) {
+                                                               float* __restrict__ out, int nimg, int h, int w) {
     extern __shared__ __align__(16) uint8_t fu_smem[];
     uint32_t* xr = (uint32_t*)fu_smem;                 /* [5][kFuXPitch] */
     uint16_t* ring = (uint16_t*)(fu_smem + kFuXBytes); /* [layer 1..5][2][kFuPitch][32] */
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * kFuSW;
-    const int j = blockIdx.y;
-    for (int i = tid; i < (int)(kFuLds / 16); i += blockDim.x) ((uint4*)fu_smem)[i] = make_uint4(0, 0, 0, 0);
-    __syncthreads();
     /* both kinds of wave run the same steps, one barrier each */
-    const float* xi = x + (long)(j >> 1) * pstride + (long)(j & 1) * istride;
-    float* oj = out + (long)j * h * w;
     if (tid < 256)
-        fu_wave<false>(xr, ring, xi, oj, scale, params, wbf, tid >> 6, tid & 63, x0, h, w);
+        fu_wave<false>(xr, ring, x, pstride, istride, out, scale, params, wbf, tid >> 6, tid & 63, tid, nimg, h, w);
     else
-        fu_wave<true>(xr, ring, xi, oj, scale, params, wbf, (tid >> 6) - 4, tid & 63, x0, h, w);
+        fu_wave<true>(xr, ring, x, pstride, istride, out, scale, params, wbf, (tid >> 6) - 4, tid & 63, tid, nimg, h,
+                      w);
 }
 
 /* ---- backward ---------------------------------------------------------- */
@@ -1099,9 +1130,12 @@ void Surrogate::forward(int nimg, int h, int w) {
 
 static void launch_fused(Ctx& c, const SurModel& m, const float* x, long ps, long is, float scale, float* out,
                          int nimg, int h, int w) {
-    IQM_CHECK(nimg <= 65535, IQM_ERR_INVALID, "too many images for one fused launch");
-    c.launch("sur_fused", sur_fused_kernel, dim3(ceil_div(w, kFuSW), nimg), dim3(kFuThreads), kFuLds, x, ps,
-                 is, scale, (const float*)m.master, (const uint16_t*)m.wbf, out, h, w);
+    /* one workgroup per CU, each with an even share of the strip rows (at
+     * least 48, so the six halo rows on either side stay a small part) */
+    const long rows = (long)nimg * ceil_div(w, kFuSW) * h;
+    const int groups = (int)std::max(1L, std::min((long)c.cus(), (rows + 47) / 48));
+    c.launch("sur_fused", sur_fused_kernel, dim3(groups), dim3(kFuThreads), kFuLds, x, ps, is, scale,
+             (const float*)m.master, (const uint16_t*)m.wbf, out, nimg, h, w);
 }
 
 void Surrogate::infer_strided(const float* d_x, long pstride, long istride, float scale, float* d_out,
