@@ -221,7 +221,8 @@ int iqm_sur_make_patches(iqm_sur* s, int32_t n, int32_t size, uint64_t seed, flo
  * downscale to size (e.g. 256 x 4). */
 int iqm_sur_make_patches_scaled(iqm_sur* s, int32_t n, int32_t size, int32_t scale, uint64_t seed, float* d_x,
                                 float* d_pc);
-/* Mean squared error over n images of h x w (device) and its gradient. */
+/* Mean squared error over n images of h x w (device) and its gradient
+ * (weighted as iqm_sur_set_loss_weight sets). */
 int iqm_sur_loss_grad(iqm_sur* s, const float* d_x, const float* d_target, int32_t n, int32_t h,
                       int32_t w, double* loss);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr. */
@@ -248,7 +249,7 @@ int iqm_sur_load(iqm_sur* s, const uint8_t* buf, int64_t len);
  * the network and run exactly (fell_back = 1, spot_err = the failing error);
  * iqm_sur_set_failure_memory(s, 0) turns that off.  spot_err and fell_back
  * are nullable.
- * Cost with the committed 6-layer network at 1920x1080: about 0.6x the exact
+ * Cost with the committed 6-layer network at 1920x1080: about the exact
  * path's pairs/s without a spot check; a failing check costs network + spot
  * pairs + the exact batch once, then exact speed. */
 int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img, const uint8_t* d_ref,
@@ -258,6 +259,13 @@ int iqm_fsim_surrogate_batch_dev(iqm_ctx* ctx, iqm_sur* s, const uint8_t* d_img,
 /* Failure memory of iqm_fsim_surrogate_batch_dev (default on); setting it
  * clears what was remembered. */
 int iqm_sur_set_failure_memory(iqm_sur* s, int32_t on);
+/* Training loss of iqm_sur_loss_grad / iqm_sur_train_step.  eps < 0 (the
+ * default): mean squared error.  eps >= 0: each pixel's squared error
+ * weighted by max(PC_a, PC_b) + eps of the FSIM pair it belongs to (images
+ * 2i and 2i + 1; n must be even), the weight FSIM gives that pixel's
+ * similarity (the loss is the weighted sum over the pixel count).  No
+ * counterpart in the reference (the network is this library's). */
+int iqm_sur_set_loss_weight(iqm_sur* s, float eps);
 
 /* ---- seeded synthetic workloads on the device -------------------------- */
 /* No counterpart in the reference: the benchmark inputs of iqm_amd/synth.py

@@ -41,7 +41,7 @@ EXPORTS = (
     "iqm_fsim_surrogate_batch_dev", "iqm_synth_pair_dev", "iqm_synth_distort_dev", "iqm_memcpy2d_dev",
     "iqm_batch_dev", "iqm_synth_reference_dev", "iqm_ssim_batch_dev_noalloc",
     "iqm_graph_cache_sizes", "iqm_fsim_batch_host", "iqm_host_alloc", "iqm_host_free",
-    "iqm_set_marks", "iqm_marks", "iqm_sur_set_failure_memory",
+    "iqm_set_marks", "iqm_marks", "iqm_sur_set_failure_memory", "iqm_sur_set_loss_weight",
 )
 
 JOB_SSIM = 1
@@ -162,6 +162,7 @@ def lib():
     L.iqm_fsim_surrogate_batch_dev.argtypes = [vp, vp, vp, vp, i32, i32, i32, i64, i32, i32,
                                                ctypes.c_float, dp, dp, fp, ctypes.POINTER(i32)]
     L.iqm_sur_set_failure_memory.argtypes = [vp, i32]
+    L.iqm_sur_set_loss_weight.argtypes = [vp, ctypes.c_float]
     L.iqm_batch_dev.argtypes = [vp, ctypes.POINTER(PairJob), i32, ctypes.POINTER(SsimParams),
                                 ctypes.POINTER(PairResult)]
     L.iqm_synth_pair_dev.argtypes = [vp, i32, i32, u64, vp, vp]
@@ -567,6 +568,12 @@ class Surrogate:
     def load(self, blob: bytes) -> None:
         buf = (ctypes.c_uint8 * len(blob)).from_buffer_copy(blob)
         _check(lib().iqm_sur_load(self.handle, buf, len(blob)))
+
+    def set_loss_weight(self, eps: float | None) -> None:
+        """Training loss: None (or eps < 0) mean squared error; eps >= 0 each
+        pixel's squared error weighted by max(PC_a, PC_b) + eps of its FSIM
+        pair (images 2i, 2i+1), as FSIM weights the pixel."""
+        _check(lib().iqm_sur_set_loss_weight(self.handle, -1.0 if eps is None else float(eps)))
 
     def set_failure_memory(self, on: bool) -> None:
         """Whether a size whose spot check failed goes straight to exact PC

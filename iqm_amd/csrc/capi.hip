@@ -732,6 +732,11 @@ int iqm_sur_set_failure_memory(iqm_sur* s, int32_t on) {
     return guarded([&] { s->s->set_remember(on != 0); });
 }
 
+int iqm_sur_set_loss_weight(iqm_sur* s, float eps) {
+    SUR_GUARD(s);
+    return guarded([&] { s->s->set_loss_weight(eps); });
+}
+
 int iqm_fft2d(iqm_ctx* ctx, float* data, int32_t w, int32_t h, int32_t batch, int32_t dir,
               int32_t normalize) {
     if (!ctx) return fail(IQM_ERR_INVALID, "null context");

@@ -648,19 +648,28 @@ __global__ __launch_bounds__(kFuThreads) void sur_fused_kernel(const float* __re
 
 /* ---- backward ---------------------------------------------------------- */
 
-/* dpred = 2 (pred - target) / P and per-block FP64 partial sums of the
- * squared error. */
+/* dpred = 2 w (pred - target) / P and per-block FP64 partial sums of the
+ * weighted squared error.  eps < 0: w = 1 (mean squared error); eps >= 0:
+ * w = max(target, target of the FSIM pair partner) + eps, where images 2i
+ * and 2i + 1 (pd pixels each) are a pair: FSIM weights a pixel's similarity
+ * by the larger of the two phase congruencies, so errors where both are
+ * small move the score little. */
 __global__ __launch_bounds__(kBlock) void sur_loss(const float* __restrict__ pred,
                                                    const float* __restrict__ target,
                                                    float* __restrict__ dpred, double* __restrict__ part,
-                                                   long P) {
+                                                   long P, long pd, float eps) {
     __shared__ double sh[kBlock];
     double acc = 0;
     const float sc = 2.0f / (float)P;
     for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
         const float d = pred[p] - target[p];
-        dpred[p] = sc * d;
-        acc += (double)d * (double)d;
+        float wt = 1.0f;
+        if (eps >= 0.0f) {
+            const long j = p / pd;
+            wt = fmaxf(target[p], target[p + ((j ^ 1) - j) * pd]) + eps;
+        }
+        dpred[p] = sc * wt * d;
+        acc += (double)wt * (double)d * (double)d;
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -1156,8 +1165,10 @@ double Surrogate::backward(const float* d_target) {
     const int nimg = acts_.nimg, h = acts_.h, w = acts_.w;
     const long P = (long)nimg * h * w;
     const int lb = 1024;
+    IQM_CHECK(loss_eps_ < 0.0f || nimg % 2 == 0, IQM_ERR_INVALID,
+              "the FSIM-weighted loss takes images in pairs (an even count)");
     c_.launch("sur_loss", sur_loss, dim3(lb), dim3(kBlock), 0, (const float*)acts_.pred, d_target,
-              acts_.dpred, d_loss_, P);
+              acts_.dpred, d_loss_, P, (long)h * w, loss_eps_);
     c_.launch("sur_loss_final", sur_loss_final, dim3(1), dim3(64), 0, (const double*)d_loss_, lb, P,
               d_loss_ + 1024);
     /* layer 6 -> dz5 */

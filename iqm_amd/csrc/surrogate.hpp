@@ -102,6 +102,9 @@ class Surrogate {
                     int spot_every, float tol);
     double* enqueue_graph(const uint8_t* d_img, const uint8_t* d_ref, int w, int h, int ch, long pair_stride,
                           int n, int spot_every, float tol);
+    /* training loss: eps < 0 mean squared error; eps >= 0 each pixel's
+     * squared error weighted by max(PC of the pair's two images) + eps */
+    void set_loss_weight(float eps) { loss_eps_ = eps; }
     /* failure memory on/off (clears what was remembered) */
     void set_remember(bool on) {
         remember_ = on;
@@ -132,6 +135,7 @@ class Surrogate {
     size_t out_cap_ = 0;
     SpotFlags* d_flags_ = nullptr;
     bool remember_ = true;
+    float loss_eps_ = -1.0f;
     std::map<std::pair<int, int>, float> failed_; /* (wd, hd) -> failing spot error */
     void ensure_batch_buffers(int n, int m, long N);
     void drop_graphs();

@@ -5,11 +5,13 @@ the device (half distorted: contrast, noise, 16-level quantisation; with
 `--scale k` made at k*size and box-downscaled by FSIM as large inputs are), targets
 the exact PC maps of the library's FSIM path; Adam on minibatches in a seeded
 order, one checkpoint per epoch (weights, Adam moments, step, cursor), and
-`--resume` continues from a checkpoint.  Evaluation: FSIM/FSIMc of seeded
-1920x1080 RGB pairs with exact PC and with network PC, and the spot-check
-error the fallback uses.
+`--resume` continues from a checkpoint.  `--loss-eps e` (e >= 0) weights each
+pixel's squared error by max(PC of the pair's two images) + e, the weight FSIM
+gives that pixel's similarity; the committed network was trained with e = 0.
+Evaluation: FSIM/FSIMc of seeded 1920x1080 RGB pairs with exact PC and with
+network PC, and the spot-check error the fallback uses.
 
-  python -m iqm_amd.train_surrogate --out pc_surrogate.iqmsur
+  python -m iqm_amd.train_surrogate --scale 4 --loss-eps 0 --out pc_surrogate.iqmsur
   python -m iqm_amd.train_surrogate --eval-only --resume pc_surrogate.iqmsur
 """
 from __future__ import annotations
@@ -101,6 +103,9 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--loss-eps", type=float, default=-1.0,
+                    help=">= 0: weight each pixel's squared error by max(PC of the pair) + this, "
+                         "the weight FSIM gives the pixel; < 0: plain mean squared error")
     ap.add_argument("--lr-halve", type=int, default=3, help="halve the learning rate every this many epochs")
     ap.add_argument("--init-seed", type=int, default=1)
     ap.add_argument("--out", default="")
@@ -118,6 +123,7 @@ def main(argv=None) -> int:
         with open(args.resume, "rb") as f:
             sur.load(f.read())
         print(f"[train] resumed at step {sur.state()[0]}", flush=True)
+    sur.set_loss_weight(args.loss_eps if args.loss_eps >= 0 else None)
     hist = [] if args.eval_only else train(ctx, sur, args)
     ev = evaluate(ctx, sur, args.eval_pairs, args.eval_seed, args.spot_tol) if args.eval_pairs else {}
     out = {"history": hist, "eval": ev, "step": sur.state()[0]}

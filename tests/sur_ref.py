@@ -66,13 +66,22 @@ def forward(p: torch.Tensor, x: torch.Tensor, emulate: bool = True) -> torch.Ten
     raise AssertionError
 
 
-def loss_and_grad(p_np: np.ndarray, x_np: np.ndarray, t_np: np.ndarray, emulate: bool = True):
-    """MSE over all pixels and its gradient with respect to the flat FP32 master parameters."""
+def loss_and_grad(p_np: np.ndarray, x_np: np.ndarray, t_np: np.ndarray, emulate: bool = True,
+                  weight_eps: float | None = None):
+    """MSE over all pixels and its gradient with respect to the flat FP32
+    master parameters.  weight_eps: each pixel's squared error weighted by
+    max(t of image 2i, t of image 2i+1) + weight_eps (images in FSIM pairs),
+    still divided by the pixel count."""
     p = torch.tensor(np.asarray(p_np, np.float64), requires_grad=True)
     x = torch.tensor(np.asarray(x_np, np.float64))
     t = torch.tensor(np.asarray(t_np, np.float64))
     out = forward(p, x, emulate)
-    loss = torch.mean((out - t) ** 2)
+    if weight_eps is None:
+        loss = torch.mean((out - t) ** 2)
+    else:
+        pair = t.reshape(-1, 2, *t.shape[1:])
+        wt = (torch.maximum(pair[:, 0], pair[:, 1]) + weight_eps)[:, None].expand_as(pair).reshape(t.shape)
+        loss = torch.mean(wt * (out - t) ** 2)
     loss.backward()
     return float(loss.detach()), p.grad.numpy().copy(), out.detach().numpy()
 

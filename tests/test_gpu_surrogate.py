@@ -117,6 +117,35 @@ def test_gradient_matches_torch(ctx, sur):
         assert rel <= 3e-2, name
 
 
+def test_weighted_gradient_matches_torch(ctx, sur):
+    """The FSIM-weighted training loss (each pixel's squared error times
+    max(t of its pair's two images) + eps) against the same loss in torch."""
+    n, h, w, eps = 4, 24, 20, 0.05
+    print(f"[sur weighted gradient] n={n} {w}x{h} eps={eps}", flush=True)
+    p = sur_ref.he_uniform(23)
+    sur.set_params(p)
+    rng = np.random.default_rng(24)
+    x = rng.uniform(0, 1, (n, h, w)).astype(np.float32)
+    t = rng.uniform(0, 1, (n, h, w)).astype(np.float32)
+    dx, dt = _dev(ctx, x), _dev(ctx, t)
+    sur.set_loss_weight(eps)
+    try:
+        loss = sur.loss_grad(dx, dt, n, h, w)
+    finally:
+        sur.set_loss_weight(None)
+    g = sur.grad()
+    ctx.free(dx)
+    ctx.free(dt)
+    loss_ref, g_ref, _ = sur_ref.loss_and_grad(p, x, t, weight_eps=eps)
+    print(f"[sur weighted gradient] loss={loss:.6e} ref={loss_ref:.6e}", flush=True)
+    assert abs(loss - loss_ref) <= 1e-2 * loss_ref
+    for name, sl in _layer_slices():
+        a, b = g[sl].astype(np.float64), g_ref[sl]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"[sur weighted gradient] {name}: rel={rel:.3e}", flush=True)
+        assert rel <= 3e-2, name
+
+
 def test_patches_and_targets(ctx, sur):
     n, s = 8, 64
     print(f"[sur patches] n={n} size={s}", flush=True)
@@ -277,15 +306,17 @@ def test_fsim_with_surrogate_pc(ctx, sur):
 
 
 def test_trained_network_gain(ctx):
-    """The committed network against the method without it, on seeded
-    1920x1080 pairs: mean |PC_net - PC_exact| over both PC maps for the
-    trained weights, the same network untrained, and a constant PC map (the
-    FSIM formula with no phase-congruency estimate).  |dFSIM| of the three is
-    printed: the FSIM scalar is dominated by the gradient term and moves by
-    ~1e-3 whatever the PC map, so the gain is asserted on the PC maps."""
+    """The committed network against the method without it.  FSIM: mean and
+    max |FSIM_net - FSIM_exact| over 12 seeded 1920x1080 pairs (seeds
+    4100..4111) for the trained weights and for a constant PC map (the FSIM
+    formula with no phase-congruency estimate); the trained network must do
+    better on the mean, and both are printed against the 1e-3 target, which
+    the network does not reach on every pair.  PC maps: mean |PC_net -
+    PC_exact| over both maps of the first 3 pairs for the trained weights, the
+    same network untrained and the constant map."""
     import os
 
-    w, h, n = 1920, 1080, 3
+    w, h, n, n_pc = 1920, 1080, 12, 3
     print(f"[sur gain] {w}x{h} n={n}", flush=True)
     pb = w * h * 3
     di, dr = ctx.alloc(n * pb), ctx.alloc(n * pb)
@@ -294,7 +325,7 @@ def test_trained_network_gain(ctx):
     ex_fs, _ = iqm_amd.FSIM().compute_batch_dev(ctx, di, dr, w, h, 3, pb, n)
     _, wd, hd = iqm_amd.fsim_dims(w, h)
     ys, pcs = [], []
-    for i in range(n):
+    for i in range(n_pc):
         img = ctx.download(di + i * pb, np.empty((h, w, 3), np.uint8))
         ref = ctx.download(dr + i * pb, np.empty((h, w, 3), np.uint8))
         r = iqm_amd.FSIM().compute_metric(ctx, img, ref, maps=("pc_in", "pc_ref"), detail=True)
@@ -314,16 +345,18 @@ def test_trained_network_gain(ctx):
             p = np.zeros(sur_ref.TOTAL, np.float32)
             p[sur_ref.OFF_B6] = 0.3
             s.set_params(p)
-        s.infer(dx, dy, 2 * n, hd, wd)
+        s.infer(dx, dy, 2 * n_pc, hd, wd)
         pc = ctx.download(dy, np.empty_like(x))
         d = np.abs(pc - pc_ex)
         pc_err[name] = float(np.mean(d[np.isfinite(d)]))
         fs, _, _, _ = s.fsim_batch_dev(di, dr, w, h, 3, pb, n)
-        fs_err[name] = float(np.mean(np.abs(fs - ex_fs)))
+        df = np.abs(fs - ex_fs)
+        fs_err[name] = (float(np.mean(df)), float(np.max(df)))
         s.close()
     for p_ in (di, dr, dx, dy):
         ctx.free(p_)
     print("[sur gain] mean |dPC|: " + " ".join(f"{k}={v:.3e}" for k, v in pc_err.items()), flush=True)
-    print("[sur gain] mean |dFSIM|: " + " ".join(f"{k}={v:.3e}" for k, v in fs_err.items()), flush=True)
+    print("[sur gain] |dFSIM| mean/max (target 1e-3): " +
+          " ".join(f"{k}={m:.3e}/{x_:.3e}" for k, (m, x_) in fs_err.items()), flush=True)
     assert pc_err["trained"] < 0.5 * pc_err["untrained"] and pc_err["trained"] < 0.5 * pc_err["constant"]
-    assert fs_err["trained"] < 5e-3
+    assert fs_err["trained"][0] < fs_err["constant"][0]
