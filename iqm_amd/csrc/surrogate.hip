@@ -721,21 +721,41 @@ __global__ __launch_bounds__(kBlock) void sur_dlast(const float* __restrict__ dp
 }
 
 /* Weight gradient of a 32 -> 32 layer: dW[co][ci][tap] = sum_p dz[p][co] *
- * a[p + offset(tap)][ci] and db[co] = sum_p dz[p][co].  A workgroup of 288
- * threads walks 16 x 16 tiles (grid-stride); thread (tap, co block of 8, ci
- * block of 4) keeps 32 FP32 accumulators and writes one partial row. */
-__global__ __launch_bounds__(288) void sur_dw_mid(const uint16_t* __restrict__ dz,
+ * a[p + offset(tap)][ci] and db[co] = sum_p dz[p][co], on the matrix cores.
+ * A workgroup of 4 waves walks 16 x 16 tiles (grid-stride).  The sum runs
+ * over pixels, so the operands need pixels contiguous: each tile is staged
+ * in LDS channel-major, dz as [co][256 pixels] and the activation halo as
+ * three copies [dx][ci][18 rows][16 columns] shifted by the tap column, so
+ * that every fragment (8 consecutive pixels of one channel) is one aligned
+ * 16-byte read.  One instruction per 32 pixels (two tile rows), tap and
+ * 16 x 16 (co, ci) block: D[co][ci] += dz^T a_tap.  Wave w owns taps w,
+ * w + 4, w + 8; waves 1 and 2 also sum db (one instruction with a B of
+ * ones per co half).  Products of BF16 values are exact in FP32; the sums
+ * run in a fixed order (instruction, 32-pixel block, tile), so the partial
+ * rows are deterministic. */
+constexpr int kDwTaLds = 3 * 32 * 18 * 16;       /* elements of the shifted halo copies */
+constexpr size_t kDwLds = (size_t)(32 * 256 + kDwTaLds) * 2;
+__global__ __launch_bounds__(256) void sur_dw_mid(const uint16_t* __restrict__ dz,
                                                   const uint16_t* __restrict__ a,
                                                   float* __restrict__ part, int nimg, int h, int w) {
-    __shared__ __align__(16) uint16_t tdz[16 * 16 * 32];
-    __shared__ __align__(16) uint16_t ta[18 * 18 * 32];
-    const int tid = threadIdx.x;
-    const int t = tid / 32, cb = (tid % 32) / 8, ib = tid % 8; /* tap, co block, ci block */
-    const int dy = t / 3, dx = t % 3;
-    float acc[8][4];
-    for (int i = 0; i < 8; ++i)
-        for (int k = 0; k < 4; ++k) acc[i][k] = 0.f;
-    float db = 0.f;
+    extern __shared__ __align__(16) uint16_t dw_smem[];
+    uint16_t* tdz = dw_smem;            /* [32 co][256 px] */
+    uint16_t* ta = dw_smem + 32 * 256;  /* [3 dx][32 ci][18][16] */
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int nt = wave == 0 ? 3 : 2; /* taps wave, wave + 4 (, wave + 8) */
+    f32x4 acc[3][2][2];
+    f32x4 accb = {0.f, 0.f, 0.f, 0.f};
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) acc[i][c][d] = z4;
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
     const long HW = (long)h * w;
     const int tx = (w + 15) / 16, ty = (h + 15) / 16;
     const long ntiles = (long)nimg * tx * ty;
@@ -743,56 +763,90 @@ __global__ __launch_bounds__(288) void sur_dw_mid(const uint16_t* __restrict__ d
         const int img = (int)(tile / (tx * ty));
         const int rem = (int)(tile - (long)img * tx * ty);
         const int y0 = (rem / tx) * 16, x0 = (rem % tx) * 16;
+        const uint16_t* dzi = dz + (long)img * HW * 32;
+        const uint16_t* ai = a + (long)img * HW * 32;
         __syncthreads();
-        for (int i = tid; i < 16 * 16 * 4; i += blockDim.x) {
-            const int px = i >> 2, part4 = i & 3;
-            const int gy = y0 + (px >> 4), gx = x0 + (px & 15);
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (gy < h && gx < w) v = ((const uint4*)(dz + ((long)img * HW + (long)gy * w + gx) * 32))[part4];
-            ((uint4*)tdz)[i] = v;
-        }
-        for (int i = tid; i < 18 * 18 * 4; i += blockDim.x) {
-            const int px = i >> 2, part4 = i & 3;
-            const int ly = px / 18, lx = px - ly * 18;
-            const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = ((const uint4*)(a + ((long)img * HW + (long)gy * w + gx) * 32))[part4];
-            ((uint4*)ta)[i] = v;
-        }
-        __syncthreads();
-        for (int px = 0; px < 256; ++px) {
-            const int ly = px >> 4, lx = px & 15;
-            const uint4 dv = *(const uint4*)&tdz[px * 32 + cb * 8];
-            const uint2 av = *(const uint2*)&ta[((ly + dy) * 18 + lx + dx) * 32 + ib * 4];
-            const uint32_t du[4] = {dv.x, dv.y, dv.z, dv.w};
-            const uint32_t au[2] = {av.x, av.y};
-            float dzf[8], af[4];
+        /* dz: pixel pairs (p, p + 1) x 8-channel chunks; one 4-byte store
+         * per channel holds the pair */
+        for (int i = tid; i < 128 * 4; i += blockDim.x) {
+            const int pp = i >> 2, ch = i & 3, p = 2 * pp;
+            const int gy = y0 + (p >> 4), gx = x0 + (p & 15);
+            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
+            if (gy < h && gx < w) v0 = ((const uint4*)(dzi + ((long)gy * w + gx) * 32))[ch];
+            if (gy < h && gx + 1 < w) v1 = ((const uint4*)(dzi + ((long)gy * w + gx + 1) * 32))[ch];
+            const uint32_t u0[4] = {v0.x, v0.y, v0.z, v0.w}, u1[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                dzf[2 * k] = __uint_as_float(du[k] << 16);
-                dzf[2 * k + 1] = __uint_as_float(du[k] & 0xffff0000u);
+                const int co = 8 * ch + 2 * k;
+                *(uint32_t*)&tdz[co * 256 + p] = (u0[k] & 0xffffu) | (u1[k] << 16);
+                *(uint32_t*)&tdz[(co + 1) * 256 + p] = (u0[k] >> 16) | (u1[k] & 0xffff0000u);
             }
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                af[2 * k] = __uint_as_float(au[k] << 16);
-                af[2 * k + 1] = __uint_as_float(au[k] & 0xffff0000u);
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) acc[i][k] += dzf[i] * af[k];
         }
-        if (tid < 32) {
-            for (int px = 0; px < 256; ++px) db += bf2f(tdz[px * 32 + tid]);
+        /* activation halo: copy dx holds column c at c + dx - 1 of the tile's
+         * columns; column pairs (c, c + 1) x rows x 8-channel chunks */
+        for (int i = tid; i < 3 * 18 * 8 * 4; i += blockDim.x) {
+            const int ch = i & 3, cp = (i >> 2) & 7, rest = i >> 5;
+            const int r = rest % 18, dx = rest / 18;
+            const int c = 2 * cp;
+            const int gy = y0 - 1 + r, gx = x0 - 1 + c + dx;
+            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < h) {
+                if (gx >= 0 && gx < w) v0 = ((const uint4*)(ai + ((long)gy * w + gx) * 32))[ch];
+                if (gx + 1 >= 0 && gx + 1 < w) v1 = ((const uint4*)(ai + ((long)gy * w + gx + 1) * 32))[ch];
+            }
+            const uint32_t u0[4] = {v0.x, v0.y, v0.z, v0.w}, u1[4] = {v1.x, v1.y, v1.z, v1.w};
+            uint16_t* base = ta + (dx * 32) * 18 * 16 + r * 16 + c;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int ci = 8 * ch + 2 * k;
+                *(uint32_t*)&base[ci * 18 * 16] = (u0[k] & 0xffffu) | (u1[k] << 16);
+                *(uint32_t*)&base[(ci + 1) * 18 * 16] = (u0[k] >> 16) | (u1[k] & 0xffff0000u);
+            }
+        }
+        __syncthreads();
+        /* 32 pixels per step: tile rows 2kb, 2kb + 1; lane group g holds
+         * pixels 32 kb + 8g .. + 7 (row 2kb + g / 2, columns 8 (g & 1) ..) */
+        for (int kb = 0; kb < 8; ++kb) {
+            bf16x8 A[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) A[c] = *(const bf16x8*)&tdz[(16 * c + li) * 256 + 32 * kb + 8 * g];
+            const int ly = 2 * kb + (g >> 1), lx = 8 * (g & 1);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (i < nt) {
+                    const int t = wave + 4 * i, dy = t / 3, dx = t - 3 * dy;
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        const bf16x8 Bf = *(const bf16x8*)&ta[((dx * 32) + 16 * d + li) * 18 * 16 + (ly + dy) * 16 + lx];
+#pragma unroll
+                        for (int c = 0; c < 2; ++c)
+                            acc[i][c][d] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[c], Bf, acc[i][c][d], 0, 0, 0);
+                    }
+                }
+            }
+            if (wave == 1 || wave == 2)
+                accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[wave - 1], ones, accb, 0, 0, 0);
         }
     }
-    /* partial row: [32][32][9] weights then [32] biases */
+    /* partial row: [32 co][32 ci][9 taps] weights then [32] biases; D layout:
+     * lane holds D[4g + e][li] of its 16 x 16 block */
     float* pr = part + (long)blockIdx.x * (32 * 32 * 9 + 32);
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 3; ++i) {
+        if (i < nt) {
+            const int t = wave + 4 * i;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) pr[((cb * 8 + i) * 32 + ib * 4 + k) * 9 + t] = acc[i][k];
-    if (tid < 32) pr[32 * 32 * 9 + tid] = db;
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int d = 0; d < 2; ++d)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        pr[((16 * c + 4 * g + e) * 32 + 16 * d + li) * 9 + t] = acc[i][c][d][e];
+        }
+    }
+    if ((wave == 1 || wave == 2) && li == 0)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) pr[32 * 32 * 9 + 16 * (wave - 1) + 4 * g + e] = accb[e];
 }
 
 /* Weight gradients of layer 1 (1 -> 32) and layer 6 (32 -> 1) and their
@@ -1180,7 +1234,7 @@ double Surrogate::backward(const float* d_target) {
     int cur = 0;
     for (int l = 5; l >= 2; --l) {
         /* dW_l from dz_l and a_{l-1}; then dz_{l-1} = conv(dz_l, Wt_l) * (a_{l-1} > 0) */
-        c_.launch("sur_dw_mid", sur_dw_mid, dim3(np), dim3(288), 0, (const uint16_t*)acts_.dz[cur],
+        c_.launch("sur_dw_mid", sur_dw_mid, dim3(np), dim3(256), kDwLds, (const uint16_t*)acts_.dz[cur],
                   (const uint16_t*)acts_.a[l - 2], part, nimg, h, w);
         c_.launch("sur_reduce", sur_reduce, dim3(ceil_div(9216 + 32, kBlock)), dim3(kBlock), 0,
                   (const float*)part, np, 9216 + 32, 9216 + 32, m.grad, SurLayout::wl(l));
