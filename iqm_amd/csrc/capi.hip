@@ -35,7 +35,7 @@ Ctx::~Ctx() {
     drop_graphs();
     for (auto& kv : ws_) (void)hipFree(kv.second.first);
     for (auto& kv : line_plans_)
-        for (float2* p : kv.second.owned) (void)hipFree(p);
+        for (void* p : kv.second.owned) (void)hipFree(p);
     for (auto& kv : fsim_plans_) {
         (void)hipFree(kv.second.lg);
         (void)hipFree(kv.second.ang);

@@ -21,7 +21,7 @@ namespace iqm {
 
 struct LinePlanHost {
     LinePlan plan{};
-    std::vector<float2*> owned; /* device tables of the plan */
+    std::vector<void*> owned; /* device tables of the plan */
 };
 
 /* Per-(wd, hd) constants of FSIM: the filter planes and the image-independent

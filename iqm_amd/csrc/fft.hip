@@ -64,10 +64,11 @@ int next_plannable(int n) {
         if (plan_radices(m, rad)) return m;
 }
 
-float2* upload_table(const std::vector<float2>& v) {
-    float2* d = nullptr;
-    IQM_HIP(hipMalloc(&d, sizeof(float2) * v.size()));
-    IQM_HIP(hipMemcpy(d, v.data(), sizeof(float2) * v.size(), hipMemcpyHostToDevice));
+template <class T>
+T* upload_table(const std::vector<T>& v) {
+    T* d = nullptr;
+    IQM_HIP(hipMalloc(&d, sizeof(T) * v.size()));
+    IQM_HIP(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return d;
 }
 
@@ -93,22 +94,66 @@ const LinePlanHost& Ctx::line_plan(int n) {
     P = LinePlan{};
     P.n = n;
     std::vector<int> rad;
-    if (plan_radices(n, rad)) {
-        P.nst = (int)rad.size();
+    const bool radices = plan_radices(n, rad);
+    int s1, s2, rq, rp, m1, m2;
+    if (radices || tl2_split(n, s1, s2)) {
+        /* a direct plan: the two-level split where there is one, else the
+         * Stockham radices */
+        P.nst = radices ? (int)rad.size() : 0;
         for (int i = 0; i < kMaxStages; ++i) P.rad[i] = i < P.nst ? rad[i] : 1;
         float2* tw = upload_table(twiddle_table(n));
         P.tw = tw;
         P.lds_len = n;
         if (tl2_split(n, P.tn1, P.tn2)) P.lds_len = std::max(n, P.tn1 * (P.tn2 + 1));
         lp.owned = {tw};
+    } else if (rader_split(n, rq, rp, m1, m2)) {
+        P.rader = 1;
+        P.rq = rq;
+        P.rp = rp;
+        P.m = rp - 1;
+        P.tm1 = m1;
+        P.tm2 = m2;
+        P.rpitch = rader_pitch(rp, m1, m2);
+        P.lds_len = std::max(n, rq * P.rpitch);
+        std::vector<int> rin, rout;
+        rader_tables(rp, P.rpitch, rin, rout);
+        /* convolution kernels v_c = exp(S 2 pi i g^c / p) of the forward
+         * (S = -1) and inverse (S = +1) transform, as V = FFT(v) / (p - 1) */
+        const double two_pi = 6.283185307179586476925286766559;
+        std::vector<float2> vf(rp - 1), vi(rp - 1);
+        for (int dir = -1; dir <= 1; dir += 2) {
+            std::vector<std::complex<double>> v(rp - 1);
+            for (int c = 0; c < rp - 1; ++c) {
+                const double a = dir * two_pi * (double)rout[c] / (double)rp;
+                v[c] = std::complex<double>(std::cos(a), std::sin(a));
+            }
+            host_dft(v);
+            std::vector<float2>& dst = dir < 0 ? vf : vi;
+            for (int c = 0; c < rp - 1; ++c)
+                dst[c] = make_float2((float)(v[c].real() / (rp - 1)), (float)(v[c].imag() / (rp - 1)));
+        }
+        float2* tw = upload_table(twiddle_table(n));
+        float2* mtw = upload_table(twiddle_table(rp - 1));
+        float2* g1 = upload_table(vf);
+        float2* g2 = upload_table(vi);
+        int* ri = upload_table(rin);
+        int* ro = upload_table(rout);
+        P.tw = tw;
+        P.mtw = mtw;
+        P.gf = g1;
+        P.gi = g2;
+        P.rin = ri;
+        P.rout = ro;
+        lp.owned = {tw, mtw, g1, g2, ri, ro};
     } else {
         /* any plannable m >= 2n - 1 holds the linear convolution */
         const int m = next_plannable(2 * n - 1);
         std::vector<int> mr;
-        IQM_CHECK(plan_radices(m, mr), IQM_ERR_UNSUPPORTED, "Bluestein length not plannable");
+        const bool mradices = plan_radices(m, mr);
+        IQM_CHECK(mradices || tl2_split(m, s1, s2), IQM_ERR_UNSUPPORTED, "Bluestein length not plannable");
         P.blue = 1;
         P.m = m;
-        P.mst = (int)mr.size();
+        P.mst = mradices ? (int)mr.size() : 0;
         for (int i = 0; i < kMaxStages; ++i) P.mrad[i] = i < P.mst ? mr[i] : 1;
         const double pi = 3.14159265358979323846;
         std::vector<float2> chirp(n);

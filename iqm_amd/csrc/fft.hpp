@@ -8,16 +8,20 @@
 // element (i/p)*r*p + k + t*p of the destination buffer.  Buffers ping-pong
 // between stages and the result ends in natural order.
 //
-// Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies.  A
-// length with a larger prime factor goes through Bluestein's chirp-z
-// identity X_k = c_k * sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(S*pi*i*j^2/n),
-// evaluated as a circular convolution of length m >= 2n-1 (the smallest such
-// m the line transforms handle directly).
+// Radices 2, 3, 4, 5, 7, 8, 11 and 13 have compile-time butterflies, and the
+// run-time two-level transform (fft_line.hpp) has register DFTs of every size
+// up to 32, primes 17..31 included.  A length n = q * p with a larger prime p
+// goes through Rader's identity (fft_line.hpp) when p - 1 has a two-level
+// split and q is 1 or a register-DFT size; any other length through
+// Bluestein's chirp-z identity X_k = c_k * sum_j (x_j c_j) conj(c_{k-j}),
+// c_j = exp(S*pi*i*j^2/n), evaluated as a circular convolution of length
+// m >= 2n-1 (the smallest such m the line transforms handle directly).
 // Twiddles and chirps are computed in double on the host and stored as float.
 #pragma once
 
 #include "common.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace iqm {
@@ -32,25 +36,31 @@ struct LinePlan {
     int rad[kMaxStages];   /* direct plan: radices in application order         */
     const float2* tw;      /* direct plan: exp(-2 pi i j / n), j < n            */
     int blue;              /* 1: Bluestein                                      */
-    int m;                 /* Bluestein convolution length (power of two)       */
-    int mst;               /* stages of the length-m plan                       */
+    int m;                 /* convolution length: Bluestein's m, Rader's p - 1  */
+    int mst;               /* Bluestein: stages of the length-m plan            */
     int mrad[kMaxStages];
     const float2* mtw;     /* exp(-2 pi i j / m), j < m                         */
-    const float2* chirp;   /* exp(-pi i (j^2 mod 2n) / n), j < n                */
+    const float2* chirp;   /* Bluestein: exp(-pi i (j^2 mod 2n) / n), j < n     */
     const float2* gf;      /* FFT_m of the forward convolution kernel           */
     const float2* gi;      /* FFT_m of the inverse convolution kernel           */
     int lds_len;           /* elements a line occupies in LDS (n or m, or the
-                              two-level exchange area when that is longer)      */
+                              two-level exchange area when that is longer, or
+                              Rader's q sub-lines)                              */
     int tn1, tn2;          /* runtime two-level split n = tn1 * tn2 (0: none)   */
-    int tm1, tm2;          /* the same for the Bluestein length m               */
+    int tm1, tm2;          /* the same for the convolution length m             */
+    int rader;             /* 1: Rader, n = rq * rp with rp prime               */
+    int rq, rp;
+    int rpitch;            /* Rader: LDS pitch of a sub-line of rp points       */
+    const int* rin;        /* Rader: slot of sub-line input j (rp entries)      */
+    const int* rout;       /* Rader: g^b mod rp, b < rp - 1 (g a primitive root) */
 };
 
 /* Sizes of the register DFTs the runtime two-level transform dispatches to
  * (fft_line.hpp); a length n = n1 * n2 with both in this set is transformed
  * with one LDS exchange instead of one LDS pass per radix stage. */
 #define IQM_TL2_SIZES(X)                                                                                    \
-    X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(18) X(20) X(21) X(22) \
-    X(24) X(25) X(26) X(27) X(28) X(30) X(32)
+    X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) \
+    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
 
 inline bool tl2_size(int m) {
     switch (m) {
@@ -72,6 +82,143 @@ inline bool tl2_split(int n, int& n1, int& n2) {
             n2 = n / a;
         }
     return n1 > 0;
+}
+
+/* ---- compile-time sine and cosine of 2*pi*e/M -------------------------- */
+
+constexpr double kPiD = 3.14159265358979323846;
+constexpr double kHalfSqrt2 = 0.70710678118654752440;
+
+constexpr double cx_sin_poly(double x) { /* |x| <= pi/8 */
+    double x2 = x * x, term = x, sum = x;
+    for (int i = 1; i < 12; ++i) {
+        term *= -x2 / (double)((2 * i) * (2 * i + 1));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cx_cos_poly(double x) {
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int i = 1; i < 12; ++i) {
+        term *= -x2 / (double)((2 * i - 1) * (2 * i));
+        sum += term;
+    }
+    return sum;
+}
+/* cos and sin of 2*pi*e/M: nearest octant q*pi/4 (exact sine/cosine) plus a
+ * remainder r = 2*pi*(8e - qM)/(8M), |r| <= pi/8, from an exact integer. */
+constexpr double cx_cos2pi(long e, long M) {
+    e %= M;
+    if (e < 0) e += M;
+    const long q = (8 * e + M / 2) / M;
+    const double r = 2.0 * kPiD * (double)(8 * e - q * M) / (double)(8 * M);
+    const double cq[8] = {1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2, 0, kHalfSqrt2};
+    const double sq[8] = {0, kHalfSqrt2, 1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2};
+    return cq[q % 8] * cx_cos_poly(r) - sq[q % 8] * cx_sin_poly(r);
+}
+constexpr double cx_sin2pi(long e, long M) {
+    e %= M;
+    if (e < 0) e += M;
+    const long q = (8 * e + M / 2) / M;
+    const double r = 2.0 * kPiD * (double)(8 * e - q * M) / (double)(8 * M);
+    const double cq[8] = {1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2, 0, kHalfSqrt2};
+    const double sq[8] = {0, kHalfSqrt2, 1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2};
+    return sq[q % 8] * cx_cos_poly(r) + cq[q % 8] * cx_sin_poly(r);
+}
+
+template <int M>
+struct RegTw {
+    float c[M];
+    float s[M];
+    constexpr RegTw() : c(), s() {
+        for (int j = 0; j < M; ++j) {
+            c[j] = (float)cx_cos2pi(j, M);
+            s[j] = (float)cx_sin2pi(j, M);
+        }
+    }
+};
+template <int M>
+struct RegTwTab {
+    static constexpr RegTw<M> t{};
+};
+
+/* ---- Rader plans (host) ---------------------------------------------------- */
+
+/* Largest prime factor of n >= 2. */
+inline int largest_prime_factor(int n) {
+    int best = 1;
+    for (int f = 2; (long)f * f <= n; ++f)
+        while (n % f == 0) {
+            best = f;
+            n /= f;
+        }
+    return n > 1 ? std::max(best, n) : best;
+}
+
+/* Rader split of n: n = q * p with p its largest prime factor, p > 13, q = 1
+ * or a register-DFT size, and p - 1 = m1 * m2 a two-level split; false when
+ * n has no such split. */
+inline bool rader_split(int n, int& q, int& p, int& m1, int& m2) {
+    q = p = m1 = m2 = 0;
+    if (n < 17) return false;
+    const int pf = largest_prime_factor(n);
+    if (pf <= 13) return false;
+    const int qq = n / pf;
+    if (qq != 1 && !tl2_size(qq)) return false;
+    if (!tl2_split(pf - 1, m1, m2)) return false;
+    q = qq;
+    p = pf;
+    return true;
+}
+
+/* LDS pitch of a Rader sub-line: the p - 1 convolution points or the
+ * two-level exchange area of p - 1, whichever is longer, then the slots of
+ * r_0 (pitch - 1) and X_0 (pitch - 2); odd, so that sub-lines read in
+ * lockstep spread over the banks. */
+inline int rader_pitch(int p, int m1, int m2) {
+    const int r = std::max(p - 1, m1 * (m2 + 1)) + 2;
+    return (r % 2 == 0) ? r + 1 : r;
+}
+
+/* Smallest primitive root of the prime p. */
+inline int primitive_root(int p) {
+    std::vector<int> fs;
+    int r = p - 1;
+    for (int f = 2; f * f <= r; ++f)
+        if (r % f == 0) {
+            fs.push_back(f);
+            while (r % f == 0) r /= f;
+        }
+    if (r > 1) fs.push_back(r);
+    auto powmod = [&](long b, int e) {
+        long acc = 1;
+        for (b %= p; e; e >>= 1, b = b * b % p)
+            if (e & 1) acc = acc * b % p;
+        return acc;
+    };
+    for (int g = 2; g < p; ++g) {
+        bool ok = true;
+        for (int f : fs) ok = ok && powmod(g, (p - 1) / f) != 1;
+        if (ok) return g;
+    }
+    return 1; /* p = 2 */
+}
+
+/* Index tables of Rader's identity for the prime p with primitive root g:
+ * rout[b] = g^b mod p (b < p - 1); rin[j] = (-log_g j) mod (p - 1), the
+ * position of input j in the convolution sequence u_a = x[g^-a] (j >= 1),
+ * and rin[0] = pitch - 1, the slot that keeps x_0. */
+inline void rader_tables(int p, int pitch, std::vector<int>& rin, std::vector<int>& rout) {
+    const int g = primitive_root(p);
+    rout.assign(p - 1, 0);
+    rin.assign(p, 0);
+    long v = 1;
+    for (int b = 0; b < p - 1; ++b) {
+        rout[b] = (int)v;
+        rin[v] = (p - 1 - b) % (p - 1);
+        v = v * g % p;
+    }
+    rin[0] = pitch - 1;
 }
 
 /* ---- small-radix DFTs (y_t = sum_q x_q exp(S*2*pi*i*q*t/R)) -------------- */
@@ -128,9 +275,26 @@ struct OddTab<13> {
                                     -0.82298386589365639458f, -0.46472317204376854566f};
 };
 
-/* Odd prime radix: pairs q and R-q share the cosine terms. */
-template <int R, int S>
-__host__ __device__ __forceinline__ void dft_odd(float2* x) {
+/* cos and sin of 2*pi*e/R for the odd prime radices: the literal tables
+ * above up to 13, the compile-time table for 17..31 (register DFTs of the
+ * run-time two-level transform only) */
+template <int R>
+__host__ __device__ __forceinline__ constexpr float odd_cos(int e) {
+    if constexpr (R <= 13) return OddTab<R>::C[e];
+    else return RegTwTab<R>::t.c[e];
+}
+template <int R>
+__host__ __device__ __forceinline__ constexpr float odd_sin(int e) {
+    if constexpr (R <= 13) return OddTab<R>::S[e];
+    else return RegTwTab<R>::t.s[e];
+}
+
+/* Odd prime radix: pairs q and R-q share the cosine terms.  Output t goes
+ * to emit(t, y_t) as soon as it is formed (t = 0, then the pairs t, R - t),
+ * so that a caller that stores the outputs keeps only the H sums and
+ * differences live, not R outputs besides them. */
+template <int R, int S, class Emit>
+__host__ __device__ __forceinline__ void dft_odd_emit(const float2* x, const Emit& emit) {
     constexpr int H = (R - 1) / 2;
     float2 a[H + 1], b[H + 1];
     float2 y0 = x[0];
@@ -140,25 +304,30 @@ __host__ __device__ __forceinline__ void dft_odd(float2* x) {
         b[q] = csub(x[q], x[R - q]);
         y0 = cadd(y0, a[q]);
     }
-    float2 out[R];
-    out[0] = y0;
+    const float2 x0 = x[0];
+    emit(0, y0);
 #pragma unroll
     for (int t = 1; t <= H; ++t) {
-        float2 P = x[0];
+        float2 P = x0;
         float2 Q = make_float2(0.f, 0.f);
 #pragma unroll
         for (int q = 1; q <= H; ++q) {
-            const float c = OddTab<R>::C[(q * t) % R];
-            const float s = OddTab<R>::S[(q * t) % R];
+            const float c = odd_cos<R>((q * t) % R);
+            const float s = odd_sin<R>((q * t) % R);
             P.x += a[q].x * c;
             P.y += a[q].y * c;
             Q.x += b[q].x * s;
             Q.y += b[q].y * s;
         }
         const float2 iq = cmul_si<S>(Q);
-        out[t] = cadd(P, iq);
-        out[R - t] = csub(P, iq);
+        emit(t, cadd(P, iq));
+        emit(R - t, csub(P, iq));
     }
+}
+template <int R, int S>
+__host__ __device__ __forceinline__ void dft_odd(float2* x) {
+    float2 out[R];
+    dft_odd_emit<R, S>(x, [&](int t, float2 v) { out[t] = v; });
 #pragma unroll
     for (int t = 0; t < R; ++t) x[t] = out[t];
 }

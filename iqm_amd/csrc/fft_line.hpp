@@ -1,8 +1,9 @@
 // Line transforms in LDS for any length: the runtime two-level transform
 // (one LDS exchange, register DFTs picked by a uniform switch on the split),
-// the Stockham schedule of fft.hpp for lengths without such a split, and
-// Bluestein's chirp-z identity around either for lengths with a prime factor
-// above 13.
+// the Stockham schedule of fft.hpp for lengths without such a split, Rader's
+// identity for lengths q * p with a prime p whose p - 1 has a two-level
+// split, and Bluestein's chirp-z identity around either direct transform for
+// the remaining lengths.
 #pragma once
 
 #include "fft_reg.hpp"
@@ -21,19 +22,14 @@ struct PtrStore {
 
 /* Step 1 of the runtime two-level transform for one item (line, j2) of a
  * line of n = N1 * n2 points: x[j1] = in(n2*j1 + j2), DFT_N1, twiddles
- * W_n^{S j2 k1} (tw holds exp(-2 pi i j / n), j < n), stored to the line's
- * exchange area at z[k1 * (n2 + 1) + j2]. */
+ * W_n^{S j2 k1} (tw holds exp(-2 pi i j / n), j < n), stored to z[k1 * zs]. */
 template <int N1, int S, class Load>
-__host__ __device__ __forceinline__ void tl2_step1_f(const Load& in, int j2, int n2, const float2* __restrict__ tw,
-                                                     float2* __restrict__ z) {
+__host__ __device__ __forceinline__ void tl2_step1_g(const Load& in, int j2, int n2, const float2* __restrict__ tw,
+                                                     float2* __restrict__ z, int zs) {
     float2 x[N1];
 #pragma unroll
     for (int j1 = 0; j1 < N1; ++j1) x[j1] = in(n2 * j1 + j2);
-    RegDft<N1, S>::run(x);
-    const int p2 = n2 + 1;
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) {
-        float2 v = x[k1];
+    RegDft<N1, S>::run_emit(x, [&](int k1, float2 v) {
         /* j2 = 0 multiplies by tw[0] = (1, -0), which leaves a finite v
          * unchanged; no branch on the run-time j2 */
         if (k1 != 0) {
@@ -41,8 +37,14 @@ __host__ __device__ __forceinline__ void tl2_step1_f(const Load& in, int j2, int
             if (S > 0) w.y = -w.y;
             v = cmul(v, w);
         }
-        z[k1 * p2 + j2] = v;
-    }
+        z[k1 * zs] = v;
+    });
+}
+/* ... to the line's exchange area at z[k1 * (n2 + 1) + j2] */
+template <int N1, int S, class Load>
+__host__ __device__ __forceinline__ void tl2_step1_f(const Load& in, int j2, int n2, const float2* __restrict__ tw,
+                                                     float2* __restrict__ z) {
+    tl2_step1_g<N1, S>(in, j2, n2, tw, z + j2, n2 + 1);
 }
 template <int N1, int S>
 __host__ __device__ __forceinline__ void tl2_step1(const float2* __restrict__ line, int j2, int n2,
@@ -58,9 +60,7 @@ __host__ __device__ __forceinline__ void tl2_step2_f(const float2* __restrict__ 
     const int p2 = N2 + 1;
 #pragma unroll
     for (int j2 = 0; j2 < N2; ++j2) y[j2] = z[k1 * p2 + j2];
-    RegDft<N2, S>::run(y);
-#pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) out(k1 + n1 * k2, y[k2]);
+    RegDft<N2, S>::run_emit(y, [&](int k2, float2 v) { out(k1 + n1 * k2, v); });
 }
 template <int N2, int S>
 __host__ __device__ __forceinline__ void tl2_step2(const float2* __restrict__ z, int k1, int n1, float2* __restrict__ out) {
@@ -121,6 +121,108 @@ __device__ __forceinline__ float2* tl2_lines(float2* in, float2* z, int ld, int 
     return in;
 }
 
+/* ---- Rader's identity for a prime factor ---------------------------------
+ * A line of n = q * p points (p prime, q = 1 or a register-DFT size) is split
+ * like the two-level transform: step 1 takes DFT_q over x[p*j1 + j2] and the
+ * twiddles W_n^{S j2 k1}, giving q sub-lines r (k1 < q) of p points whose
+ * DFTs are the outputs X[k1 + q*k2].  A sub-line's DFT follows from Rader's
+ * identity: with g a primitive root of p and w = exp(S*2*pi*i/p),
+ *   R[g^b] = r_0 + sum_{a < p-1} u_a v_{b-a},  u_a = r[g^-a],  v_c = w^(g^c),
+ *   R[0]   = r_0 + sum_a u_a,
+ * a cyclic convolution of length p - 1, evaluated as
+ * IFFT(FFT(u) * V) with V = FFT(v) / (p - 1) (the plan's gf / gi) by the
+ * run-time two-level transform of p - 1 (S = -1 forward, +1 inverse).
+ * Step 1 stores r_j straight to its convolution position rin[j] (rin[0] is
+ * the slot pitch - 1 that keeps r_0); the forward convolution transform's
+ * step 2 multiplies by V on its way out and keeps R[0] = r_0 + U[0] in slot
+ * pitch - 2.  Sub-line (l, k1) of an nlines-line set lives at
+ * (k1 * nlines + l) * pitch of the buffer (pitch = P.rpitch >= the p - 1
+ * points and their two-level exchange area, plus the two slots). */
+template <int S, class Load>
+__host__ __device__ __forceinline__ void rader_item1_f(int q, const Load& in, int j2, int p, const float2* tw,
+                                                       float2* z, int zs) {
+    switch (q) {
+        case 1: tl2_step1_g<1, S>(in, j2, p, tw, z, zs); break;
+#define IQM_RADER_STEP1(k) \
+    case k: tl2_step1_g<k, S>(in, j2, p, tw, z, zs); break;
+        IQM_TL2_SIZES(IQM_RADER_STEP1)
+#undef IQM_RADER_STEP1
+        default: break;
+    }
+}
+
+/* Output k of the forward convolution transform of sub-line s: U[k] * V[k]
+ * back to the sub-line; R[0] = r_0 + U[0] to slot rp - 2. */
+struct RaderMulStore {
+    float2* s;
+    const float2* V;
+    int rp;
+    __host__ __device__ void operator()(int k, float2 v) const {
+        if (k == 0) s[rp - 2] = cadd(v, s[rp - 1]);
+        s[k] = cmul(v, V[k]);
+    }
+};
+
+/* Output b of the inverse convolution transform of sub-line s = (line, k1):
+ * R[g^b] = c_b + r_0 is output k1 + q * g^b of the line; b = 0 also gives
+ * output k1 (R[0]). */
+template <class Store>
+struct RaderOutStore {
+    Store out;
+    const float2* s;
+    const int* rout;
+    int k1, q, rp;
+    __host__ __device__ void operator()(int b, float2 v) const {
+        out(k1 + q * rout[b], cadd(v, s[rp - 1]));
+        if (b == 0) out(k1, s[rp - 2]);
+    }
+};
+
+/* Output k of a line from its sub-lines after the inverse convolution
+ * transform (sub-line k1 = k mod q at s): R[0] for k2 = k / q = 0, else
+ * c_b + r_0 with b = log_g k2 = (p - 1 - rin[k2]) mod (p - 1). */
+__host__ __device__ __forceinline__ float2 rader_output(const float2* s, int k2, int p, int rp, const int* rin) {
+    if (k2 == 0) return s[rp - 2];
+    const int r = rin[k2];
+    return cadd(s[r == 0 ? 0 : p - 1 - r], s[rp - 1]);
+}
+
+/* Rader transform of nlines lines held in b0 (line l at l * ld); the result
+ * is returned in b0.  b1 holds the sub-lines, b0 the exchange areas of the
+ * convolution transforms.  Starts after the caller's barrier, ends with one. */
+template <int S>
+__device__ __forceinline__ float2* line_rader(float2* b0, float2* b1, int ld, int nlines, const LinePlan& P,
+                                              const float2* tw) {
+    const int n = P.n, q = P.rq, p = P.rp, rp = P.rpitch, m1 = P.tm1, m2 = P.tm2;
+    const int nl2 = nlines * q;
+    const float ip = 1.0f / (float)p, i1 = 1.0f / (float)m1, i2 = 1.0f / (float)m2;
+    for (int it = threadIdx.x; it < nlines * p; it += blockDim.x) {
+        const int l = fast_div(it, p, ip), j2 = it - l * p;
+        rader_item1_f<S>(q, PtrLoad{b0 + l * ld}, j2, p, tw, b1 + l * rp + P.rin[j2], nlines * rp);
+    }
+    lds_barrier();
+    for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
+        const int l2 = fast_div(it, m2, i2), j2 = it - l2 * m2;
+        tl2_item1<-1>(m1, b1 + l2 * rp, j2, m2, P.mtw, b0 + l2 * rp);
+    }
+    lds_barrier();
+    const float2* V = (S < 0) ? P.gf : P.gi;
+    for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
+        const int l2 = fast_div(it, m1, i1), k1 = it - l2 * m1;
+        tl2_item2_f<-1>(m2, b0 + l2 * rp, k1, m1, RaderMulStore{b1 + l2 * rp, V, rp});
+    }
+    lds_barrier();
+    tl2_lines<1>(b1, b0, rp, nl2, m1, m2, P.mtw);
+    const float in_ = 1.0f / (float)n, iq = 1.0f / (float)q;
+    for (int it = threadIdx.x; it < nlines * n; it += blockDim.x) {
+        const int l = fast_div(it, n, in_), k = it - l * n;
+        const int k2 = fast_div(k, q, iq), k1 = k - k2 * q;
+        b0[l * ld + k] = rader_output(b1 + (k1 * nlines + l) * rp, k2, p, rp, P.rin);
+    }
+    lds_barrier();
+    return b0;
+}
+
 /* Direct transform of a length-n line set with plan radices or split. */
 template <int S>
 __device__ __forceinline__ float2* line_direct(float2* b0, float2* b1, int ld, int nlines, int n, int s1, int s2,
@@ -135,6 +237,7 @@ __device__ __forceinline__ float2* line_direct(float2* b0, float2* b1, int ld, i
 template <int S>
 __device__ __forceinline__ float2* line_fft(float2* b0, float2* b1, int ld, int nlines, const LinePlan& P,
                             const float2* tw) {
+    if (P.rader) return line_rader<S>(b0, b1, ld, nlines, P, tw);
     if (!P.blue) return line_direct<S>(b0, b1, ld, nlines, P.n, P.tn1, P.tn2, P.nst, P.rad, tw);
     const int n = P.n, m = P.m;
     const float im_ = 1.0f / (float)m, in_ = 1.0f / (float)n;
@@ -208,6 +311,56 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
             int l, k1;
             split(it, n1, i1, l, k1);
             tl2_item2_f<S>(n2, b1 + l * ld, k1, n1, store(l));
+        }
+        lds_barrier();
+        return;
+    }
+    if (P.rader) {
+        /* sub-line l2 = k1 * nlines + l; with LineFast the sub-line index is
+         * fastest, so the final stores of neighbouring lines stay together */
+        const int q = P.rq, p = P.rp, rp = P.rpitch, m1 = P.tm1, m2 = P.tm2;
+        const int nl2 = nlines * q;
+        const float ip = 1.0f / (float)p, i1 = 1.0f / (float)m1, i2 = 1.0f / (float)m2, inl2 = 1.0f / (float)nl2;
+        auto split2 = [&](int it, int len, float ilen, int& l2, int& e) {
+            if (LineFast) {
+                e = fast_div(it, nl2, inl2);
+                l2 = it - e * nl2;
+            } else {
+                l2 = fast_div(it, len, ilen);
+                e = it - l2 * len;
+            }
+        };
+        for (int it = threadIdx.x; it < nlines * p; it += blockDim.x) {
+            int l, j2;
+            split(it, p, ip, l, j2);
+            rader_item1_f<S>(q, load(l), j2, p, tw, b1 + l * rp + P.rin[j2], nlines * rp);
+        }
+        lds_barrier();
+        for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
+            int l2, j2;
+            split2(it, m2, i2, l2, j2);
+            tl2_item1<-1>(m1, b1 + l2 * rp, j2, m2, P.mtw, b0 + l2 * rp);
+        }
+        lds_barrier();
+        const float2* V = (S < 0) ? P.gf : P.gi;
+        for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
+            int l2, k1;
+            split2(it, m1, i1, l2, k1);
+            tl2_item2_f<-1>(m2, b0 + l2 * rp, k1, m1, RaderMulStore{b1 + l2 * rp, V, rp});
+        }
+        lds_barrier();
+        for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
+            int l2, j2;
+            split2(it, m2, i2, l2, j2);
+            tl2_item1<1>(m1, b1 + l2 * rp, j2, m2, P.mtw, b0 + l2 * rp);
+        }
+        lds_barrier();
+        using St = decltype(store(0));
+        for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
+            int l2, k1;
+            split2(it, m1, i1, l2, k1);
+            const int r1 = fast_div(l2, nlines, inl), l = l2 - r1 * nlines;
+            tl2_item2_f<1>(m2, b0 + l2 * rp, k1, m1, RaderOutStore<St>{store(l), b1 + l2 * rp, P.rout, r1, q, rp});
         }
         lds_barrier();
         return;

@@ -3,7 +3,8 @@
 // RegDft<M, S> computes an M-point DFT (y_k = sum_j x_j exp(S*2*pi*i*j*k/M))
 // entirely in registers, fully unrolled, by splitting M = R * Q:
 //   y[k1 + R*k2] = sum_{j2} W_Q^{j2 k2} W_M^{j2 k1} sum_{j1} x[Q*j1 + j2] W_R^{j1 k1},
-// with the radix-R butterflies of fft.hpp and compile-time twiddles.
+// with the radix-R butterflies of fft.hpp and compile-time twiddles; the
+// primes 17..31 are single butterflies (sizes up to 32 need no split by them).
 //
 // A line of n = N1 * N2 points is then transformed with one LDS exchange:
 //   step 1, item j2 < N2:  z[k1][j2] = W_n^{j2 k1} * DFT_N1(x[N2*j1 + j2])[k1]
@@ -16,67 +17,10 @@
 
 namespace iqm {
 
-/* ---- compile-time sine and cosine of 2*pi*e/M -------------------------- */
-
-constexpr double kPiD = 3.14159265358979323846;
-constexpr double kHalfSqrt2 = 0.70710678118654752440;
-
-constexpr double cx_sin_poly(double x) { /* |x| <= pi/8 */
-    double x2 = x * x, term = x, sum = x;
-    for (int i = 1; i < 12; ++i) {
-        term *= -x2 / (double)((2 * i) * (2 * i + 1));
-        sum += term;
-    }
-    return sum;
-}
-constexpr double cx_cos_poly(double x) {
-    double x2 = x * x, term = 1.0, sum = 1.0;
-    for (int i = 1; i < 12; ++i) {
-        term *= -x2 / (double)((2 * i - 1) * (2 * i));
-        sum += term;
-    }
-    return sum;
-}
-/* cos and sin of 2*pi*e/M: nearest octant q*pi/4 (exact sine/cosine) plus a
- * remainder r = 2*pi*(8e - qM)/(8M), |r| <= pi/8, from an exact integer. */
-constexpr double cx_cos2pi(long e, long M) {
-    e %= M;
-    if (e < 0) e += M;
-    const long q = (8 * e + M / 2) / M;
-    const double r = 2.0 * kPiD * (double)(8 * e - q * M) / (double)(8 * M);
-    const double cq[8] = {1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2, 0, kHalfSqrt2};
-    const double sq[8] = {0, kHalfSqrt2, 1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2};
-    return cq[q % 8] * cx_cos_poly(r) - sq[q % 8] * cx_sin_poly(r);
-}
-constexpr double cx_sin2pi(long e, long M) {
-    e %= M;
-    if (e < 0) e += M;
-    const long q = (8 * e + M / 2) / M;
-    const double r = 2.0 * kPiD * (double)(8 * e - q * M) / (double)(8 * M);
-    const double cq[8] = {1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2, 0, kHalfSqrt2};
-    const double sq[8] = {0, kHalfSqrt2, 1, kHalfSqrt2, 0, -kHalfSqrt2, -1, -kHalfSqrt2};
-    return sq[q % 8] * cx_cos_poly(r) + cq[q % 8] * cx_sin_poly(r);
-}
-
-template <int M>
-struct RegTw {
-    float c[M];
-    float s[M];
-    constexpr RegTw() : c(), s() {
-        for (int j = 0; j < M; ++j) {
-            c[j] = (float)cx_cos2pi(j, M);
-            s[j] = (float)cx_sin2pi(j, M);
-        }
-    }
-};
-template <int M>
-struct RegTwTab {
-    static constexpr RegTw<M> t{};
-};
-
 /* radix used to split M: a native butterfly size, largest power of two first */
 constexpr bool is_native_radix(int m) {
-    return m == 2 || m == 3 || m == 4 || m == 5 || m == 7 || m == 8 || m == 11 || m == 13;
+    return m == 2 || m == 3 || m == 4 || m == 5 || m == 7 || m == 8 || m == 11 || m == 13 || m == 17 || m == 19 ||
+           m == 23 || m == 29 || m == 31;
 }
 constexpr int split_radix(int M) {
     if (M % 8 == 0 && M != 8) return 8;
@@ -94,7 +38,19 @@ constexpr bool reg_dft_ok(int M) {
 
 template <int M, int S>
 struct RegDft {
-    static_assert(reg_dft_ok(M), "register DFT size must factor into radices 2..13");
+    static_assert(reg_dft_ok(M), "register DFT size must factor into the native radices");
+    /* y_k to emit(k, y_k), k < M; x is clobbered.  The odd primes above 8
+     * give each output away as it is formed (dft_odd_emit). */
+    template <class Emit>
+    __host__ __device__ __forceinline__ static void run_emit(float2* x, const Emit& emit) {
+        if constexpr (M > 8 && is_native_radix(M)) {
+            dft_odd_emit<M, S>(x, emit);
+        } else {
+            run(x);
+#pragma unroll
+            for (int k = 0; k < M; ++k) emit(k, x[k]);
+        }
+    }
     __host__ __device__ __forceinline__ static void run(float2* x) {
         if constexpr (M == 1) {
             return;

@@ -1,7 +1,8 @@
 // Host-side check of the FFT index math (no GPU needed): runs the same
 // __host__ __device__ building blocks the kernels use -- Stockham stages,
-// register DFTs, the compile-time two-level line split and the run-time
-// two-level split with its size dispatch -- sequentially on the CPU and
+// register DFTs, the compile-time two-level line split, the run-time
+// two-level split with its size dispatch and Rader's identity with its plan
+// tables -- sequentially on the CPU and
 // prints the largest error against a direct double-precision DFT.
 #include <cmath>
 #include <complex>
@@ -118,6 +119,60 @@ static double check_tl2(int n) {
     return relerr(in, dft(x, S));
 }
 
+/* Rader's identity as the kernels run it (fft_line.hpp line_rader and the
+ * Rader branch of line_fft_io), sequentially over every work item, for
+ * three lines of n points: the sub-line outputs through rader_output and
+ * through RaderOutStore must both equal the DFT of each line. */
+template <int S>
+static double check_rader(int n) {
+    int q, p, m1, m2;
+    if (!rader_split(n, q, p, m1, m2)) return -1;
+    const int rp = rader_pitch(p, m1, m2);
+    std::vector<int> rin, rout;
+    rader_tables(p, rp, rin, rout);
+    /* the plan's kernel V = FFT(v) / (p - 1), v_c = exp(S 2 pi i g^c / p) */
+    std::vector<cd> v(p - 1);
+    for (int c = 0; c < p - 1; ++c) v[c] = std::polar(1.0, S * 2.0 * M_PI * rout[c] / p);
+    const auto Vd = dft(v, -1);
+    std::vector<float2> V(p - 1);
+    for (int c = 0; c < p - 1; ++c) V[c] = make_float2((float)(Vd[c].real() / (p - 1)), (float)(Vd[c].imag() / (p - 1)));
+    const int nlines = 3, ld = lds_pitch(std::max(n, q * rp)), nl2 = nlines * q;
+    auto tw = twiddles(n), mtw = twiddles(p - 1);
+    std::vector<std::vector<cd>> xs;
+    std::vector<float2> b0(nlines * ld), b1(nlines * ld), viaio(nlines * n);
+    for (int l = 0; l < nlines; ++l) {
+        xs.push_back(rnd(n, 13 * n + l));
+        for (int j = 0; j < n; ++j) b0[l * ld + j] = make_float2((float)xs[l][j].real(), (float)xs[l][j].imag());
+    }
+    for (int l = 0; l < nlines; ++l)
+        for (int j2 = 0; j2 < p; ++j2)
+            rader_item1_f<S>(q, PtrLoad{&b0[l * ld]}, j2, p, tw.data(), &b1[l * rp + rin[j2]], nlines * rp);
+    for (int l2 = 0; l2 < nl2; ++l2)
+        for (int j2 = 0; j2 < m2; ++j2) tl2_item1<-1>(m1, &b1[l2 * rp], j2, m2, mtw.data(), &b0[l2 * rp]);
+    for (int l2 = 0; l2 < nl2; ++l2)
+        for (int k1 = 0; k1 < m1; ++k1) tl2_item2_f<-1>(m2, &b0[l2 * rp], k1, m1, RaderMulStore{&b1[l2 * rp], V.data(), rp});
+    for (int l2 = 0; l2 < nl2; ++l2)
+        for (int j2 = 0; j2 < m2; ++j2) tl2_item1<1>(m1, &b1[l2 * rp], j2, m2, mtw.data(), &b0[l2 * rp]);
+    /* the store-functor path first (it leaves b0 and b1 as they are) */
+    for (int l2 = 0; l2 < nl2; ++l2) {
+        const int k1 = l2 / nlines, l = l2 - k1 * nlines;
+        for (int kk = 0; kk < m1; ++kk)
+            tl2_item2_f<1>(m2, &b0[l2 * rp], kk, m1,
+                           RaderOutStore<PtrStore>{PtrStore{&viaio[l * n]}, &b1[l2 * rp], rout.data(), k1, q, rp});
+    }
+    for (int l2 = 0; l2 < nl2; ++l2)
+        for (int k1 = 0; k1 < m1; ++k1) tl2_item2<1>(m2, &b0[l2 * rp], k1, m1, &b1[l2 * rp]);
+    double e = 0;
+    for (int l = 0; l < nlines; ++l) {
+        std::vector<float2> out(n);
+        for (int k = 0; k < n; ++k) out[k] = rader_output(&b1[((k % q) * nlines + l) * rp], k / q, p, rp, rin.data());
+        const auto ref = dft(xs[l], S);
+        e = std::max(e, relerr(out, ref));
+        e = std::max(e, relerr(std::vector<float2>(viaio.begin() + l * n, viaio.begin() + (l + 1) * n), ref));
+    }
+    return e;
+}
+
 int main() {
     double worst = 0;
     auto rep = [&](const char* what, int n, double e) {
@@ -135,6 +190,12 @@ int main() {
     rep("reg_inv", 24, check_reg<24, 1>());
     rep("reg_inv", 30, check_reg<30, 1>());
     rep("reg_fwd", 32, check_reg<32, -1>());
+    rep("reg_fwd", 17, check_reg<17, -1>());
+    rep("reg_inv", 19, check_reg<19, 1>());
+    rep("reg_fwd", 23, check_reg<23, -1>());
+    rep("reg_inv", 29, check_reg<29, 1>());
+    rep("reg_fwd", 31, check_reg<31, -1>());
+    rep("reg_inv", 31, check_reg<31, 1>());
     rep("two_level_fwd", 270, check_two_level<15, 18, -1>());
     rep("two_level_inv", 270, check_two_level<15, 18, 1>());
     rep("two_level_fwd", 480, check_two_level<20, 24, -1>());
@@ -148,7 +209,17 @@ int main() {
         rep("tl2_fwd", n, check_tl2<-1>(n));
         rep("tl2_inv", n, check_tl2<1>(n));
     }
+    for (int n : {221, 289, 323, 527, 589, 961, 46, 62}) { /* splits by the primes 17..31 */
+        ++nsplit;
+        rep("tl2_fwd", n, check_tl2<-1>(n));
+        rep("tl2_inv", n, check_tl2<1>(n));
+    }
     std::printf("run-time splits checked: %d\n", nsplit);
+    /* Rader: q = 1 (17, 37, 61, 97, 929), q = 2..17 (74, 202, 611, 629, 1184) */
+    for (int n : {17, 37, 61, 97, 929, 74, 202, 611, 629, 1184}) {
+        rep("rader_fwd", n, check_rader<-1>(n));
+        rep("rader_inv", n, check_rader<1>(n));
+    }
     std::printf("worst %.3e\n", worst);
     return worst < 2e-6 ? 0 : 1;
 }

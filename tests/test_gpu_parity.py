@@ -29,7 +29,11 @@ def _report(tag, **errs):
     print(f"[{tag}] " + " ".join(f"{k}={v:.3e}" for k, v in errs.items()), flush=True)
 
 
-@pytest.mark.parametrize("w,h", [(480, 270), (270, 480), (64, 48), (45, 36), (1, 8), (7, 1), (97, 61), (1009, 3), (659, 17)])
+# line plans: two-level splits (480, 270, 64, 48, 45, 36; 221 = 13*17 and 527 = 17*31 by the prime
+# register DFTs), Rader (97, 61, 17, 23, 31, 929 and 74 = 2*37, 202 = 2*101, 611 = 13*47,
+# 629 = 17*37, 1184 = 32*37), Bluestein (1009, 659), trivial (1, 3, 7, 8)
+@pytest.mark.parametrize("w,h", [(480, 270), (270, 480), (64, 48), (45, 36), (1, 8), (7, 1), (97, 61), (1009, 3), (659, 17),
+                                 (221, 527), (611, 202), (629, 1184), (929, 74), (23, 31)])
 def test_fft2d_matches_numpy(ctx, w, h):
     rng = np.random.default_rng(w * 1000 + h)
     x = (rng.standard_normal((3, h, w)) + 1j * rng.standard_normal((3, h, w))).astype(np.complex64)
