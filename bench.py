@@ -263,9 +263,20 @@ def run_mixed(args, ctx) -> dict:
         dev = timer.stop_ms() / steps
         return out, dev, (time.perf_counter() - t0) * 1e3 / steps
 
+    tune = {}
+    if not args.no_tune:
+        # FFT plan (two-level split) per line length and pass, timed on the device
+        t0 = time.perf_counter()
+        before = ctx.batch_dev(jobs)
+        entries = ctx.autotune_lines(jobs)
+        lines = json.loads(ctx.export_tunes())["lines"]
+        tune = {"seconds": time.perf_counter() - t0, "entries": entries,
+                "non_default_splits": sum(1 for e in lines if e["split1"])}
     t0 = time.perf_counter()
     first = ctx.batch_dev(jobs)  # eager pass (plans, workspaces) + capture
     setup_s = time.perf_counter() - t0
+    if tune:
+        tune["max_abs_diff_vs_default_plans"] = float(np.max(np.abs(first[:, 1:] - before[:, 1:])))
     for _ in range(args.warmup):
         ctx.batch_dev(jobs)
     res, dev_ms, wall_ms = timed(lambda: ctx.batch_dev(jobs), args.steps)
@@ -322,6 +333,7 @@ def run_mixed(args, ctx) -> dict:
         "graph_vs_separate": sep_wall / wall_ms,
         "max_abs_diff_vs_separate": float(diff.max()),
         "setup_s": setup_s,
+        "line_plan_tune": tune,
         "stages_us": stages,
         "device": ctx.device_name,
     }

@@ -314,13 +314,26 @@ int iqm_set_graphs(iqm_ctx* ctx, int32_t on);
  * least recently launched graph and frees its job table. */
 int iqm_graph_cache_sizes(iqm_ctx* ctx, int32_t* same_size, int32_t* mixed);
 
-/* ---- autotuner (tile shapes of the FSIM launch sequence) ---------------- */
+/* ---- autotuner (tile shapes and FFT plans of the FSIM launch sequence) -- */
 
 /* Times candidate tile shapes for images of w x h x ch on n synthetic pairs
  * and keeps the fastest for that downscaled size in the context (used by every
  * later FSIM call of that size).  us_per_pair (nullable) gets its time. */
 int iqm_autotune_fsim(iqm_ctx* ctx, int32_t w, int32_t h, int32_t ch, int32_t n,
                       double* us_per_pair);
+/* FFT plans of a mixed batch (replaces VkFFT's plan choice, which the
+ * reference rebuilds on every call: src/gpu/fsim/fsim.cpp:94,160 and the
+ * configuration at :370-437).  For every downscaled line length of the
+ * batch's FSIM pairs (IQM_JOB_FSIM), as columns (length hd) and as rows
+ * (length wd), times the generic pass alone under each two-level split of
+ * the length's plan (the length itself, Rader's p - 1 or Bluestein's m, both
+ * factor orders); then times the batch's own column and row passes under
+ * three policies -- the default split of every length, the default with its
+ * factors swapped, the per-length winners -- and keeps per pass the fastest
+ * (the default unless another is 1% faster), for every later FSIM call of
+ * those lengths.  n_tuned (nullable) gets the number of (length, role)
+ * entries.  Scores change only by FP32 rounding. */
+int iqm_autotune_lines(iqm_ctx* ctx, const iqm_pair_job* jobs, int32_t n, int32_t* n_tuned);
 /* The context's tile table as JSON (a plan file); returns its length without
  * the terminating NUL and writes at most cap bytes (NUL-terminated when it
  * fits).  Returns -1 on error. */

@@ -33,7 +33,7 @@ EXPORTS = (
     "iqm_ssim", "iqm_ssim_batch_dev", "iqm_fsim_dims", "iqm_fsim", "iqm_fsim_batch_dev",
     "iqm_fsim_batch_dev_async",
     "iqm_fft2d", "iqm_dev_alloc", "iqm_dev_free", "iqm_memcpy_h2d", "iqm_memcpy_d2h",
-    "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_tune_export",
+    "iqm_stream", "iqm_sync", "iqm_set_graphs", "iqm_autotune_fsim", "iqm_autotune_lines", "iqm_tune_export",
     "iqm_tune_import", "iqm_set_stage_timing", "iqm_stage_times", "iqm_reset_stage_times",
     "iqm_sur_param_count", "iqm_sur_create", "iqm_sur_destroy", "iqm_sur_get_params",
     "iqm_sur_set_params", "iqm_sur_get_grad", "iqm_sur_state", "iqm_sur_make_patches", "iqm_sur_make_patches_scaled",
@@ -129,6 +129,7 @@ def lib():
     L.iqm_set_graphs.argtypes = [vp, i32]
     L.iqm_graph_cache_sizes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.iqm_autotune_fsim.argtypes = [vp, i32, i32, i32, i32, dp]
+    L.iqm_autotune_lines.argtypes = [vp, ctypes.POINTER(PairJob), i32, ctypes.POINTER(i32)]
     L.iqm_tune_export.argtypes = [vp, ctypes.c_char_p, i64]
     L.iqm_tune_export.restype = i64
     L.iqm_tune_import.argtypes = [vp, ctypes.c_char_p]
@@ -298,6 +299,19 @@ class Context:
         us = np.zeros(1, np.float64)
         _check(lib().iqm_autotune_fsim(self.handle, w, h, ch, n, _dptr(us)))
         return float(us[0])
+
+    def autotune_lines(self, jobs) -> int:
+        """Tunes the FFT plan (two-level split) of every downscaled line length of the
+        batch's FSIM pairs, as columns and as rows; returns the number of entries timed.
+        jobs as for batch_dev."""
+        n = len(jobs)
+        arr = (PairJob * n)()
+        for i, j in enumerate(jobs):
+            arr[i] = PairJob(j.get("d_img"), j.get("d_ref"), j["w"], j["h"], j.get("ch", 3),
+                             j.get("methods", JOB_SSIM | JOB_FSIM), None, None)
+        cnt = ctypes.c_int32()
+        _check(lib().iqm_autotune_lines(self.handle, arr, n, ctypes.byref(cnt)))
+        return cnt.value
 
     def export_tunes(self) -> str:
         n = lib().iqm_tune_export(self.handle, None, 0)

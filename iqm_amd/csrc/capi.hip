@@ -841,6 +841,24 @@ int iqm_autotune_fsim(iqm_ctx* ctx, int32_t w, int32_t h, int32_t ch, int32_t n,
     });
 }
 
+int iqm_autotune_lines(iqm_ctx* ctx, const iqm_pair_job* jobs, int32_t n, int32_t* n_tuned) {
+    if (!ctx) return fail(IQM_ERR_INVALID, "null context");
+    return guarded([&] {
+        IQM_CHECK(jobs && n > 0, IQM_ERR_INVALID, "invalid batch arguments");
+        std::vector<iqm::PairJob> v((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            v[i].img = jobs[i].d_img;
+            v[i].ref = jobs[i].d_ref;
+            v[i].w = jobs[i].w;
+            v[i].h = jobs[i].h;
+            v[i].ch = jobs[i].ch;
+            v[i].methods = jobs[i].methods;
+        }
+        const int k = ctx->c->autotune_lines_batch(v.data(), n);
+        if (n_tuned) *n_tuned = k;
+    });
+}
+
 int64_t iqm_tune_export(iqm_ctx* ctx, char* buf, int64_t cap) {
     if (!ctx) return -1;
     int64_t len = -1;

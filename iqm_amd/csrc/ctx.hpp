@@ -143,7 +143,15 @@ class Ctx {
     void* host_pinned(size_t bytes);
     void sync();
 
-    const LinePlanHost& line_plan(int n);
+    /* Line plan of length n; split1 > 0 picks the two-level split
+     * (split1, L / split1) of the plan's inner length L (n for a direct
+     * plan, p - 1 for Rader's, m for Bluestein's) instead of the most
+     * balanced one (split1 = 0). */
+    const LinePlanHost& line_plan(int n, int split1 = 0);
+    /* The plan of length n for the column (LINE_COL) or row (LINE_ROW)
+     * passes of FSIM: the tuned split where autotune_line has run, else the
+     * default. */
+    const LinePlan& line_plan_for(int n, int role);
     const FsimSizePlan& fsim_plan(int wd, int hd);
 
     /* batched in-place 2-D FFT of device planes */
@@ -270,6 +278,31 @@ class Ctx {
     /* Times candidate tiles on n synthetic pairs of w x h x ch and keeps the
      * fastest for the downscaled size; returns it. */
     FsimTune autotune_fsim(int w, int h, int ch, int n);
+    /* Times the generic pass of the role on synthetic lines of length n
+     * under every two-level split of the plan's inner length and keeps the
+     * fastest; returns its first factor (0: the default split). */
+    int autotune_line(int n, int role);
+    /* Line plans of a mixed batch chosen in the batch: every (length, role)
+     * of its FSIM pairs timed alone (autotune_line), then whole policies --
+     * the default split of every length, the default with its factors
+     * swapped, the per-length winners -- timed over the batch's own column
+     * and row passes; each role keeps its fastest policy.  Returns the
+     * number of (length, role) entries. */
+    int autotune_lines_batch(const PairJob* jobs, int n);
+    /* Mean microseconds of one launch of the generic column or row pass of
+     * plan P over a synthetic workload (fsim_gen.hip). */
+    double time_line_pass(const LinePlan& P, int role, hipEvent_t e0, hipEvent_t e1);
+    /* a line plan choice: the split's first factor (0: the default split)
+     * and, when timed here, the pass's microseconds under it and under the
+     * default */
+    struct LineTune {
+        int split1 = 0;
+        double us = 0, us_default = 0;
+    };
+    void set_line_tune(int n, int role, const LineTune& t) {
+        line_tunes_[{n, role}] = t;
+        drop_graphs();
+    }
     std::string export_tunes() const;
     void import_tunes(const std::string& json);
     bool graphs() const { return graphs_; }
@@ -313,7 +346,8 @@ class Ctx {
     hipEvent_t comp_done_[2] = {nullptr, nullptr};
     void* stage_ = nullptr; /* pinned staging of pageable inputs */
     size_t stage_bytes_ = 0;
-    std::map<int, LinePlanHost> line_plans_;
+    std::map<std::pair<int, int>, LinePlanHost> line_plans_; /* (n, split1) */
+    std::map<std::pair<int, int>, LineTune> line_tunes_;      /* (n, role) */
     std::map<std::pair<int, int>, FsimSizePlan> fsim_plans_;
     std::map<std::pair<int, int>, FsimTune> tunes_;
     char* fsim_ws_ = nullptr;
@@ -387,9 +421,16 @@ void ssim_noalloc_setup(int device);
 
 /* workspace slots */
 enum { WS_IN_IMG = 0, WS_IN_REF, WS_FSIM, WS_SSIM, WS_MAPS, WS_FFT, WS_PLAN_TMP, WS_SUR, WS_GEN, WS_BATCH,
-       WS_MIX, WS_MIXTAB, WS_STREAM, WS_STREAM_SCORES };
+       WS_MIX, WS_MIXTAB, WS_STREAM, WS_STREAM_SCORES, WS_TUNE };
+
+/* roles of a line plan in FSIM: the column passes (line length hd) and the
+ * row passes (line length wd) */
+enum { LINE_COL = 0, LINE_ROW = 1 };
 
 size_t line_lds_bytes(const LinePlan& P, int lines, int bufs);
 int lines_that_fit(const LinePlan& P, int bufs, size_t budget, int cap);
+/* First factors a of the two-level splits (a, L / a) of the plan's inner
+ * length L, both orders, ascending (empty: the plan has none). */
+std::vector<int> line_split_candidates(const LinePlan& P);
 
 }  // namespace iqm

@@ -5,7 +5,6 @@
 #include "ctx.hpp"
 
 #include <algorithm>
-
 #include <cmath>
 #include <complex>
 
@@ -84,11 +83,63 @@ std::vector<float2> twiddle_table(int n) {
 
 }  // namespace
 
-const LinePlanHost& Ctx::line_plan(int n) {
-    auto it = line_plans_.find(n);
+/* Inner length of a plan: the length its two-level split factors. */
+static int inner_length(const LinePlan& P) { return P.rader ? P.rp - 1 : P.blue ? P.m : P.n; }
+
+std::vector<int> line_split_candidates(const LinePlan& P) {
+    std::vector<int> c;
+    const int L = inner_length(P);
+    const bool has = P.rader || (P.blue ? P.tm1 > 0 : P.tn1 > 0);
+    if (!has) return c;
+    for (int a = 2; a <= 32 && a < L; ++a)
+        if (L % a == 0 && tl2_size(a) && tl2_size(L / a)) c.push_back(a);
+    return c;
+}
+
+const LinePlan& Ctx::line_plan_for(int n, int role) {
+    auto it = line_tunes_.find({n, role});
+    return line_plan(n, it == line_tunes_.end() ? 0 : it->second.split1).plan;
+}
+
+const LinePlanHost& Ctx::line_plan(int n, int split1) {
+    auto it = line_plans_.find({n, split1});
     if (it != line_plans_.end()) return it->second;
     IQM_CHECK(n >= 1 && n <= 8192, IQM_ERR_UNSUPPORTED,
               "FFT length " + std::to_string(n) + " outside [1, 8192]");
+    if (split1 != 0) {
+        /* the default plan with another split of its inner length; its
+         * tables stay with the default plan (a Rader variant owns its input
+         * slot table, which depends on the sub-line pitch) */
+        const LinePlan& B = line_plan(n, 0).plan;
+        const std::vector<int> cand = line_split_candidates(B);
+        IQM_CHECK(std::find(cand.begin(), cand.end(), split1) != cand.end(), IQM_ERR_INVALID,
+                  "FFT length " + std::to_string(n) + ": " + std::to_string(split1) +
+                      " is not a factor of a two-level split of its plan");
+        LinePlanHost lv;
+        LinePlan& P = lv.plan;
+        P = B;
+        const int L = inner_length(B), s2 = L / split1;
+        if (P.rader) {
+            P.tm1 = split1;
+            P.tm2 = s2;
+            P.rpitch = rader_pitch(P.rp, split1, s2);
+            P.lds_len = std::max(n, P.rq * P.rpitch);
+            std::vector<int> rin, rout;
+            rader_tables(P.rp, P.rpitch, rin, rout);
+            int* ri = upload_table(rin);
+            P.rin = ri;
+            lv.owned = {ri};
+        } else if (P.blue) {
+            P.tm1 = split1;
+            P.tm2 = s2;
+            P.lds_len = std::max(P.m, split1 * (s2 + 1));
+        } else {
+            P.tn1 = split1;
+            P.tn2 = s2;
+            P.lds_len = std::max(n, split1 * (s2 + 1));
+        }
+        return line_plans_.emplace(std::make_pair(n, split1), lv).first->second;
+    }
     LinePlanHost lp;
     LinePlan& P = lp.plan;
     P = LinePlan{};
@@ -193,7 +244,7 @@ const LinePlanHost& Ctx::line_plan(int n) {
         if (tl2_split(m, P.tm1, P.tm2)) P.lds_len = std::max(m, P.tm1 * (P.tm2 + 1));
         lp.owned = {mtw, ch, g1, g2};
     }
-    return line_plans_.emplace(n, lp).first->second;
+    return line_plans_.emplace(std::make_pair(n, 0), lp).first->second;
 }
 
 /* LDS bytes of a kernel holding `bufs` buffers of `lines` lines of plan P,

@@ -304,8 +304,8 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     IQM_CHECK(wd >= 2 && hd >= 2, IQM_ERR_TOO_SMALL, "image too small for FSIM");
     IQM_CHECK(fsim_planes_fit32((long)wd * hd), IQM_ERR_UNSUPPORTED, "downscaled image too large for FSIM");
     const FsimSizePlan& plan = fsim_plan(wd, hd);
-    const LinePlan& Px = line_plan(wd).plan;
-    const LinePlan& Py = line_plan(hd).plan;
+    const LinePlan& Px = line_plan_for(wd, LINE_ROW);
+    const LinePlan& Py = line_plan_for(hd, LINE_COL);
     mark("FFT library initialized"); /* fsim.cpp:95: the per-size plans are ready */
     const long N = (long)wd * hd;
     const int nblk = ceil_div(wd, kPcTx) * ceil_div(hd, kPcTy);

@@ -46,17 +46,15 @@ __global__ __launch_bounds__(kBlock) void mix_rowfft_kernel(const MixJob* __rest
     fsim_rowfft_body(J.yiq, J.A, J.wd, J.hd, J.grad_rb, J.Px, blockIdx.x - st[j], 0);
 }
 
-template <int B>
-__global__ __launch_bounds__(B) void mix_colpass_kernel(const MixJob* __restrict__ jobs,
-                                                        const int* __restrict__ st, int n) {
+__global__ __launch_bounds__(kBlockFft) void mix_colpass_kernel(const MixJob* __restrict__ jobs,
+                                                                const int* __restrict__ st, int n) {
     const int j = mix_find(st, n, blockIdx.x);
     const MixJob& J = jobs[j];
     fsim_colpass_body(J.A, J.T, J.bank, J.wd, J.hd, J.col_bx, J.nbx, J.Py, blockIdx.x - st[j]);
 }
 
-template <int B>
-__global__ __launch_bounds__(B) void mix_rowpass_kernel(const MixJob* __restrict__ jobs,
-                                                        const int* __restrict__ st, int n) {
+__global__ __launch_bounds__(kBlockFft) void mix_rowpass_kernel(const MixJob* __restrict__ jobs,
+                                                                const int* __restrict__ st, int n) {
     const int j = mix_find(st, n, blockIdx.x);
     const MixJob& J = jobs[j];
     const int local = blockIdx.x - st[j];
@@ -100,8 +98,8 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     IQM_CHECK(F <= kPrepMaxRows, IQM_ERR_UNSUPPORTED, "image too large for the FSIM prep tile");
     IQM_CHECK(fsim_planes_fit32((long)wd * hd), IQM_ERR_UNSUPPORTED, "downscaled image too large for FSIM");
     const FsimSizePlan& plan = fsim_plan(wd, hd);
-    const LinePlan& Px = line_plan(wd).plan;
-    const LinePlan& Py = line_plan(hd).plan;
+    const LinePlan& Px = line_plan_for(wd, LINE_ROW);
+    const LinePlan& Py = line_plan_for(hd, LINE_COL);
     const long N = (long)wd * hd;
     const FsimTiles t = pick_tiles(*this, Px, Py, N, fsim_tune(wd, hd));
     mj.F = F;
@@ -160,13 +158,10 @@ void Ctx::mix_fsim_launch(const MixJob* d_jobs, const int* d_starts, int n, cons
         void (*fn)(const MixJob*, const int*, int);
         int threads;
     };
-    const bool fft256 = env_int("IQM_MIX_FFT_THREADS", kBlockFft) == 256;
     const K ks[] = {{MIX_PREP, "fsim_prep", mix_prep_kernel, kBlock},
                     {MIX_ROWFFT, "fsim_rowfft", mix_rowfft_kernel, kBlock},
-                    {MIX_COL, "fsim_colpass", fft256 ? mix_colpass_kernel<256> : mix_colpass_kernel<kBlockFft>,
-                     fft256 ? 256 : kBlockFft},
-                    {MIX_ROW, "fsim_rowpass", fft256 ? mix_rowpass_kernel<256> : mix_rowpass_kernel<kBlockFft>,
-                     fft256 ? 256 : kBlockFft},
+                    {MIX_COL, "fsim_colpass", mix_colpass_kernel, kBlockFft},
+                    {MIX_ROW, "fsim_rowpass", mix_rowpass_kernel, kBlockFft},
                     {MIX_SELECT, "fsim_select", mix_select_kernel, 1024},
                     {MIX_PCF, "fsim_pcfinal", mix_pcfinal_kernel, kBlock},
                     {MIX_FIN, "fsim_finalize", mix_finalize_kernel, kBlock}};

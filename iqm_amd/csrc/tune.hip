@@ -6,6 +6,8 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <set>
 #include <sstream>
 #include <vector>
 
@@ -123,6 +125,120 @@ FsimTune Ctx::autotune_fsim(int w, int h, int ch, int n) {
     return best;
 }
 
+int Ctx::autotune_line(int n, int role) {
+    IQM_CHECK(role == LINE_COL || role == LINE_ROW, IQM_ERR_INVALID, "line role must be 0 (columns) or 1 (rows)");
+    const LinePlan& B = line_plan(n, 0).plan;
+    const std::vector<int> cand = line_split_candidates(B);
+    const int dflt = B.rader || B.blue ? B.tm1 : B.tn1;
+    if (cand.size() <= 1) {
+        set_line_tune(n, role, LineTune{});
+        return 0;
+    }
+    struct Restore {
+        Ctx& c;
+        bool was_timing;
+        hipEvent_t e0, e1;
+        ~Restore() {
+            c.timing_ = was_timing;
+            c.free_events_.push_back(e0);
+            c.free_events_.push_back(e1);
+        }
+    } restore{*this, timing_, take_event(), take_event()};
+    timing_ = false;
+    /* another split replaces the default only when it is at least 4% faster
+     * in this timing: measured over the 256-pair mixed batch, splits chosen
+     * on smaller margins made its row pass 2.5% slower, not faster (more
+     * distinct register-DFT code running side by side) */
+    constexpr double kMargin = 0.96;
+    LineTune t;
+    t.us_default = time_line_pass(line_plan(n, 0).plan, role, restore.e0, restore.e1);
+    t.us = t.us_default;
+    for (int a : cand) {
+        if (a == dflt) continue;
+        const double us = time_line_pass(line_plan(n, a).plan, role, restore.e0, restore.e1);
+        if (us < kMargin * t.us_default && us < t.us) {
+            t.us = us;
+            t.split1 = a;
+        }
+    }
+    set_line_tune(n, role, t);
+    return t.split1;
+}
+
+int Ctx::autotune_lines_batch(const PairJob* jobs, int n) {
+    std::vector<PairJob> fj;
+    std::set<std::pair<int, int>> keys; /* (length, role) */
+    for (int i = 0; i < n; ++i) {
+        if (!(jobs[i].methods & IQM_JOB_FSIM)) continue;
+        PairJob j = jobs[i];
+        j.methods = IQM_JOB_FSIM;
+        j.ssim_map = nullptr;
+        j.simc_map = nullptr;
+        fj.push_back(j);
+        const int F = std::max(1, (int)std::round(std::min(j.w, j.h) / 256.0));
+        keys.insert({(int)std::round((float)j.h / (float)F), LINE_COL});
+        keys.insert({(int)std::round((float)j.w / (float)F), LINE_ROW});
+    }
+    if (fj.empty()) return 0;
+    std::map<std::pair<int, int>, LineTune> alone;
+    for (auto& k : keys) {
+        autotune_line(k.first, k.second);
+        alone[k] = line_tunes_[k];
+    }
+    /* policy 0: default split, 1: default swapped, 2: per-length winners */
+    auto apply = [&](int role, int policy) {
+        for (auto& k : keys) {
+            if (k.second != role) continue;
+            LineTune t = alone[k];
+            if (policy != 2) {
+                const LinePlan& B = line_plan(k.first, 0).plan;
+                const int d1 = B.rader || B.blue ? B.tm1 : B.tn1, d2 = B.rader || B.blue ? B.tm2 : B.tn2;
+                t.split1 = (policy == 1 && d1 > 0 && d2 != d1) ? d2 : 0;
+            }
+            set_line_tune(k.first, k.second, t);
+        }
+    };
+    const bool was_timing = timing_;
+    auto saved = totals_;
+    SsimArgs sp;
+    auto pass_ms = [&](const char* tag) {
+        auto it = totals_.find(tag);
+        return it == totals_.end() ? 0.0 : it->second.first;
+    };
+    double col[3], row[3];
+    try {
+        for (int p = 0; p < 3; ++p) {
+            apply(LINE_COL, p);
+            apply(LINE_ROW, p);
+            timing_ = false;
+            mixed_batch(fj.data(), (int)fj.size(), sp); /* plans and workspaces */
+            sync();
+            totals_.clear();
+            timing_ = true;
+            mixed_batch(fj.data(), (int)fj.size(), sp);
+            sync();
+            col[p] = pass_ms("fsim_colpass");
+            row[p] = pass_ms("fsim_rowpass");
+        }
+    } catch (...) {
+        timing_ = was_timing;
+        totals_ = saved;
+        throw;
+    }
+    timing_ = was_timing;
+    totals_ = saved;
+    /* another policy replaces the default only on a 1% gain over the batch */
+    auto pick = [](const double* t) {
+        int b = 0;
+        for (int p = 1; p < 3; ++p)
+            if (t[p] < 0.99 * t[0] && t[p] < t[b]) b = p;
+        return b;
+    };
+    apply(LINE_COL, pick(col));
+    apply(LINE_ROW, pick(row));
+    return (int)keys.size();
+}
+
 std::string Ctx::export_tunes() const {
     std::ostringstream o;
     o << "{\"fsim\": [";
@@ -133,6 +249,14 @@ std::string Ctx::export_tunes() const {
           << ", \"two_level\": " << t.two_level << ", \"col_bx\": " << t.col_bx
           << ", \"row_rb\": " << t.row_rb << ", \"fft_rb\": " << t.fft_rb << ", \"prep_rb\": " << t.prep_rb
           << ", \"us_per_pair\": " << t.us_per_pair << "}";
+        first = false;
+    }
+    o << "], \"lines\": [";
+    first = true;
+    for (auto& kv : line_tunes_) {
+        o << (first ? "" : ", ") << "{\"line\": " << kv.first.first << ", \"role\": " << kv.first.second
+          << ", \"split1\": " << kv.second.split1 << ", \"us\": " << kv.second.us
+          << ", \"us_default\": " << kv.second.us_default << "}";
         first = false;
     }
     o << "]}";
@@ -157,6 +281,21 @@ void Ctx::import_tunes(const std::string& json) {
         IQM_CHECK(end != std::string::npos, IQM_ERR_INVALID, "tune plan: unterminated entry");
         const std::string obj = json.substr(pos, end - pos + 1);
         double wd, hd, tl, bx, rb, fr, us = 0, pr = 1;
+        double ln, role, s1;
+        if (field(obj, "line", ln)) {
+            IQM_CHECK(field(obj, "role", role) && field(obj, "split1", s1), IQM_ERR_INVALID,
+                      "tune plan: line entry misses a field");
+            IQM_CHECK(ln >= 1 && ln <= 8192 && (role == LINE_COL || role == LINE_ROW) && s1 >= 0 && s1 <= 32,
+                      IQM_ERR_INVALID, "tune plan: line entry out of range");
+            if (s1 > 0) line_plan((int)ln, (int)s1); /* checks that it splits the plan */
+            LineTune t;
+            t.split1 = (int)s1;
+            field(obj, "us", t.us);
+            field(obj, "us_default", t.us_default);
+            set_line_tune((int)ln, (int)role, t);
+            pos = end;
+            continue;
+        }
         if (!field(obj, "wd", wd)) continue; /* the outer object */
         IQM_CHECK(field(obj, "hd", hd) && field(obj, "two_level", tl) && field(obj, "col_bx", bx) &&
                       field(obj, "row_rb", rb) && field(obj, "fft_rb", fr),
