@@ -1058,8 +1058,8 @@ inline FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py
      * Bluestein) fill a workgroup's threads only with enough lines: without
      * a measured tune they take up to 32 columns, 8 x 4 rows and 32 rows per
      * workgroup, in up to the whole LDS (mixed 256-pair batch: 39.6 ms
-     * against 49.3 ms with 16 / 2 x 4 / 8 in 64 KiB; row caps 4 -> 8 and
-     * 16 -> 32 took its row passes from 10.3 to 9.2 ms) */
+     * against 49.3 ms with 16 / 2 x 4 / 8 in 64 KiB; the row-pass cap
+     * 4 -> 8 took its row pass from 8.9 to 8.0 ms) */
     const bool measured = tune.us_per_pair != 0; /* tuned or explicitly set tiles */
     const size_t pref = (size_t)env_int("IQM_FSIM_LDS_PREF", (int)c.lds_per_block());
     auto fit = [&](const LinePlan& P, int bufs, int cap, int group) {
@@ -1067,7 +1067,13 @@ inline FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py
         if (l >= std::min(cap, 4)) return l;
         return std::max(l, lines_that_fit(P, bufs, c.lds_per_block(), group * cap) / group);
     };
-    t.grad_rb = fit(Px, 2, env_int("IQM_FSIM_GRB", measured ? tune.fft_rb : 32), 1);
+    /* row FFT (256 threads): at most 64 KiB, so that two or more workgroups
+     * share a CU (mixed batch: 0.50 -> 0.33 ms against the whole LDS) */
+    {
+        const int cap = env_int("IQM_FSIM_GRB", measured ? tune.fft_rb : 32);
+        const int l = lines_that_fit(Px, 2, std::min<size_t>({c.lds_per_block(), pref, 65536}), cap);
+        t.grad_rb = l > 0 ? l : lines_that_fit(Px, 2, c.lds_per_block(), cap);
+    }
     t.col_bx = fit(Py, colpass_bufs(Py), env_int("IQM_FSIM_BX", measured ? tune.col_bx : 32), 1);
     /* row pass: 4 scales per row, plus the histogram and the An sums */
     auto fit_row = [&](int cap) {
