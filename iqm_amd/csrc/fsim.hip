@@ -350,7 +350,7 @@ double* Ctx::fsim_batch(const uint8_t* d_img, const uint8_t* d_ref, int w, int h
     const bool tl_row = tl && tl_supported(wd);
     for (int p0 = 0; p0 < n; p0 += chunk) {
         const int np = std::min(chunk, n - p0);
-        const float2* Ach = A + (long)p0 * 2 * N;
+        float2* Ach = A + (long)p0 * 2 * N;
         if (tl_col) {
             TlColArgs a{Ach, T, plan.bank, Py.tw, wd, np, env_int("IQM_FSIM_TL_BX", tune.col_bx), xcds_, tl_budget,
                         gate_};

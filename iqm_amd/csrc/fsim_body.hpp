@@ -381,16 +381,30 @@ __device__ __forceinline__ void fsim_rowfft_body(const float* __restrict__ yiq, 
  * column-block count padded to a multiple of 8, so column block cb always
  * runs on XCD cb mod 8 and each XCD's L2 holds only its slice of the bank. */
 /* element y of column line c: spectrum times filter; output y to the plane */
-/* LDS line buffers of the column pass: the spectrum and the two-level
- * exchange area, or for Stockham, Rader and Bluestein lines (whose
- * transforms use two work buffers) one more */
-__host__ __device__ __forceinline__ int colpass_bufs(const LinePlan& P) { return (!P.blue && P.tn1 > 0) ? 2 : 3; }
+/* LDS line buffers of the column pass (the spectrum stays in global
+ * memory): the two-level exchange area, or the two work buffers of a
+ * Stockham, Rader or Bluestein transform */
+__host__ __device__ __forceinline__ int colpass_bufs(const LinePlan& P) { return (!P.blue && P.tn1 > 0) ? 1 : 2; }
 
+/* element y of column line c of a plane (row pitch wd), and its store */
+struct ColLoad {
+    const float2* a; /* the plane at the column */
+    int wd;
+    __device__ float2 operator()(int y) const { return *at32(a, (unsigned)(y * wd)); }
+};
+struct ColStore {
+    float2* a;
+    int wd;
+    __device__ void operator()(int y, float2 v) const { *at32(a, (unsigned)(y * wd)) = v; }
+};
 struct ColFilterLoad {
-    const float2* fs; /* the column's spectrum in LDS */
+    const float2* fs; /* the column's spectrum (global, in place of its input) */
     const float* h;   /* the filter plane at the column */
     int wd;
-    __device__ float2 operator()(int y) const { return cscale(fs[y], *at32(h, (unsigned)(y * wd))); }
+    __device__ float2 operator()(int y) const {
+        const unsigned o = (unsigned)(y * wd);
+        return cscale(*at32(fs, o), *at32(h, o));
+    }
 };
 struct ColStreamStore {
     float2* d; /* the response plane at the column */
@@ -398,7 +412,7 @@ struct ColStreamStore {
     __device__ void operator()(int y, float2 v) const { nt_store(at32(d, (unsigned)(y * wd)), v); }
 };
 
-__device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, float2* __restrict__ Tout,
+__device__ __forceinline__ void fsim_colpass_body(float2* __restrict__ A, float2* __restrict__ Tout,
                                                   const float* __restrict__ bank, int wd, int hd, int bx,
                                                   int nbx_pad, const LinePlan& P, int blk) {
     extern __shared__ float2 lds[];
@@ -410,33 +424,27 @@ __device__ __forceinline__ void fsim_colpass_body(const float2* __restrict__ A, 
     const int ld = lds_pitch(P.lds_len);
     const int nbuf = colpass_bufs(P);
     float2* b0 = lds;
-    float2* b1 = lds + bx * ld;
-    /* a two-level inverse needs only its exchange area beside the spectrum */
-    float2* b2 = (nbuf == 2) ? nullptr : lds + 2 * bx * ld;
+    float2* b1 = lds + (nbuf - 1) * bx * ld; /* one buffer: the exchange area only */
     const float2* tw = stage_twiddles(P, lds + nbuf * bx * ld);
     const long N = (long)wd * hd;
     const int nc = min(bx, wd - x0);
-    const float inc = 1.0f / (float)nc;
     /* plane offsets y * wd + c (< N) in 32 bits from workgroup-uniform bases */
-    const float2* src = A + ((long)pair * 2 + img) * N + x0;
-    staged_copy<8>(nc * hd, [&](int idx) {
-        const int y = fast_div(idx, nc, inc), c = idx - y * nc;
-        return *at32(src, (unsigned)(y * wd + c));
-    }, [&](int idx, float2 v) {
-        const int y = fast_div(idx, nc, inc), c = idx - y * nc;
-        b0[c * ld + y] = v;
-    });
-    __syncthreads();
-    float2* Fs = line_fft<-1>(b0, b1, ld, nc, P, tw);
-    float2* W = (Fs == b0) ? b1 : b0;
+    float2* sp = A + ((long)pair * 2 + img) * N + x0;
+    __syncthreads(); /* twiddles staged */
+    /* forward column transform in place: every transform reads all its
+     * inputs in its first phase and stores in its last, and no other
+     * workgroup reads these columns */
+    line_fft_io<-1, true>([&](int c) { return ColLoad{sp + c, wd}; }, [&](int c) { return ColStore{sp + c, wd}; },
+                          b0, b1, ld, nc, P, tw);
+    __syncthreads(); /* the spectrum's stores visible to the whole workgroup */
     for (int z = 0; z < 16; ++z) {
         const float* hz = bank + (long)z * N + x0;
         float2* dst = Tout + ((long)pair * 32 + img * 16 + z) * N + x0;
         /* the inverse transform reads H_z * F straight from the spectrum and
-         * the bank, and streams its outputs to the row pass (out of the L2) */
-        line_fft_io<1, true>([&](int c) { return ColFilterLoad{Fs + c * ld, hz + c, wd}; },
-                       [&](int c) { return ColStreamStore{dst + c, wd}; }, W, nbuf == 2 ? W : b2, ld, nc, P,
-                       tw);
+         * the bank (requested together), and streams its outputs to the row
+         * pass (out of the L2) */
+        line_fft_io<1, true>([&](int c) { return ColFilterLoad{sp + c, hz + c, wd}; },
+                             [&](int c) { return ColStreamStore{dst + c, wd}; }, b0, b1, ld, nc, P, tw);
     }
 }
 
@@ -993,7 +1001,7 @@ __device__ __forceinline__ void fsim_pcfinal_body(const float* __restrict__ yiq,
 __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __restrict__ yiq,
                                                              float2* __restrict__ A, int wd, int hd,
                                                              int rb, LinePlan P, const int* __restrict__ gate);
-__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* __restrict__ A,
+__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(float2* __restrict__ A,
                                                               float2* __restrict__ Tout,
                                                               const float* __restrict__ bank,
                                                               int wd, int hd, int bx, int nbx_pad,
@@ -1074,7 +1082,7 @@ inline FsimTiles pick_tiles(const Ctx& c, const LinePlan& Px, const LinePlan& Py
         const int l = lines_that_fit(Px, 2, std::min<size_t>({c.lds_per_block(), pref, 65536}), cap);
         t.grad_rb = l > 0 ? l : lines_that_fit(Px, 2, c.lds_per_block(), cap);
     }
-    t.col_bx = fit(Py, colpass_bufs(Py), env_int("IQM_FSIM_BX", measured ? tune.col_bx : 32), 1);
+    t.col_bx = fit(Py, colpass_bufs(Py), env_int("IQM_FSIM_BX", measured ? tune.col_bx : 64), 1);
     /* row pass: 4 scales per row, plus the histogram and the An sums */
     auto fit_row = [&](int cap) {
         auto best = [&](size_t budget) {

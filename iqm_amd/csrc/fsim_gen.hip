@@ -13,7 +13,7 @@ __global__ __launch_bounds__(kBlock) void fsim_rowfft_kernel(const float* __rest
     fsim_rowfft_body(yiq, A, wd, hd, rb, P, blockIdx.x, blockIdx.y);
 }
 
-__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(const float2* __restrict__ A,
+__global__ __launch_bounds__(kBlockFft) void fsim_colpass_kernel(float2* __restrict__ A,
                                                               float2* __restrict__ Tout,
                                                               const float* __restrict__ bank,
                                                               int wd, int hd, int bx, int nbx_pad,
@@ -74,7 +74,7 @@ double Ctx::time_line_pass(const LinePlan& P, int role, hipEvent_t e0, hipEvent_
     auto one = [&] {
         if (role == LINE_COL) {
             const int nbx = ceil_div(wd, t.col_bx), nbx_pad = ceil_div(nbx, xcds_) * xcds_;
-            const float2* A = (const float2*)rest;
+            float2* A = (float2*)rest;
             const float* bank = (const float*)(rest + sizeof(float2) * 2 * N);
             launch("line_tune", fsim_colpass_kernel, dim3(nbx_pad * 2), dim3(kBlockFft),
                    line_lds_bytes(P, t.col_bx, colpass_bufs(P)), A, T, bank, wd, hd, t.col_bx, nbx_pad, P,
