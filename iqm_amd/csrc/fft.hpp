@@ -172,11 +172,13 @@ inline bool rader_split(int n, int& q, int& p, int& m1, int& m2) {
 }
 
 /* LDS pitch of a Rader sub-line: the p - 1 convolution points or the
- * two-level exchange area of p - 1, whichever is longer, then the slots of
- * r_0 (pitch - 1) and X_0 (pitch - 2); odd, so that sub-lines read in
- * lockstep spread over the banks. */
+ * two-level exchange areas of p - 1 (m1 rows of m2 + 1 for the forward
+ * transform, m2 rows of m1 + 1 for the inverse, which takes the split the
+ * other way round), whichever is longest, then the slots of r_0 (pitch - 1)
+ * and X_0 (pitch - 2); odd, so that sub-lines read in lockstep spread over
+ * the banks. */
 inline int rader_pitch(int p, int m1, int m2) {
-    const int r = std::max(p - 1, m1 * (m2 + 1)) + 2;
+    const int r = std::max({p - 1, m1 * (m2 + 1), m2 * (m1 + 1)}) + 2;
     return (r % 2 == 0) ? r + 1 : r;
 }
 

@@ -131,13 +131,17 @@ __device__ __forceinline__ float2* tl2_lines(float2* in, float2* z, int ld, int 
  *   R[0]   = r_0 + sum_a u_a,
  * a cyclic convolution of length p - 1, evaluated as
  * IFFT(FFT(u) * V) with V = FFT(v) / (p - 1) (the plan's gf / gi) by the
- * run-time two-level transform of p - 1 (S = -1 forward, +1 inverse).
- * Step 1 stores r_j straight to its convolution position rin[j] (rin[0] is
- * the slot pitch - 1 that keeps r_0); the forward convolution transform's
- * step 2 multiplies by V on its way out and keeps R[0] = r_0 + U[0] in slot
- * pitch - 2.  Sub-line (l, k1) of an nlines-line set lives at
- * (k1 * nlines + l) * pitch of the buffer (pitch = P.rpitch >= the p - 1
- * points and their two-level exchange area, plus the two slots). */
+ * run-time two-level transform of p - 1 = m1 * m2 (S = -1 forward, +1
+ * inverse).  Step 1 stores r_j straight to its convolution position rin[j]
+ * (rin[0] is the slot pitch - 1 that keeps r_0).  The inverse transform
+ * takes the split the other way round (m2 * m1): its step-1 item j2' = k1
+ * needs U[k1 + m1 * k2] * V[k1 + m1 * k2] for k2 < m2, exactly the outputs
+ * of the forward transform's step-2 item k1, so one phase (rader_mid) runs
+ * forward step 2, the product with V and inverse step 1 in registers, and
+ * keeps R[0] = r_0 + U[0] in slot pitch - 2.  Sub-line (l, k1) of an
+ * nlines-line set lives at (k1 * nlines + l) * pitch of the buffer
+ * (pitch = P.rpitch >= the p - 1 points and both exchange areas, plus the
+ * two slots). */
 template <int S, class Load>
 __host__ __device__ __forceinline__ void rader_item1_f(int q, const Load& in, int j2, int p, const float2* tw,
                                                        float2* z, int zs) {
@@ -151,17 +155,48 @@ __host__ __device__ __forceinline__ void rader_item1_f(int q, const Load& in, in
     }
 }
 
-/* Output k of the forward convolution transform of sub-line s: U[k] * V[k]
- * back to the sub-line; R[0] = r_0 + U[0] to slot rp - 2. */
-struct RaderMulStore {
-    float2* s;
-    const float2* V;
-    int rp;
-    __host__ __device__ void operator()(int k, float2 v) const {
-        if (k == 0) s[rp - 2] = cadd(v, s[rp - 1]);
-        s[k] = cmul(v, V[k]);
+/* The middle phase of a Rader sub-line for one item k1 < m1: forward step 2
+ * (DFT_M2 of exchange row k1 of z, the forward transform's exchange area),
+ * giving U[k1 + m1 * k2]; R[0] = r_0 + U[0] to slot rp - 2 of the sub-line s
+ * (item 0); the products with V; then inverse step 1 for j2' = k1 of the
+ * split m2 * m1 (DFT_M2 with S = +1 and the twiddles W_m^{+k1 k1'}, tw =
+ * exp(-2 pi i j / m)), stored to the inverse exchange area at
+ * s[k1' * (m1 + 1) + k1], k1' < m2. */
+template <int M2>
+__host__ __device__ __forceinline__ void rader_mid(const float2* __restrict__ z, int k1, int m1,
+                                                   const float2* __restrict__ V, const float2* __restrict__ tw,
+                                                   float2* __restrict__ s, int rp) {
+    float2 y[M2];
+#pragma unroll
+    for (int j2 = 0; j2 < M2; ++j2) y[j2] = z[k1 * (M2 + 1) + j2];
+    RegDft<M2, -1>::run(y);
+    if (k1 == 0) s[rp - 2] = cadd(y[0], s[rp - 1]);
+#pragma unroll
+    for (int k2 = 0; k2 < M2; ++k2) y[k2] = cmul(y[k2], V[k1 + m1 * k2]);
+    RegDft<M2, 1>::run(y);
+    const int p2 = m1 + 1;
+#pragma unroll
+    for (int kk = 0; kk < M2; ++kk) {
+        float2 v = y[kk];
+        /* k1 = 0 multiplies by tw[0] = (1, -0): a finite v is unchanged */
+        if (kk != 0) {
+            float2 w = tw[k1 * kk];
+            w.y = -w.y;
+            v = cmul(v, w);
+        }
+        s[kk * p2 + k1] = v;
     }
-};
+}
+__host__ __device__ __forceinline__ void rader_mid_item(int m2, const float2* z, int k1, int m1, const float2* V,
+                                                        const float2* tw, float2* s, int rp) {
+    switch (m2) {
+#define IQM_RADER_MID(k) \
+    case k: rader_mid<k>(z, k1, m1, V, tw, s, rp); break;
+        IQM_TL2_SIZES(IQM_RADER_MID)
+#undef IQM_RADER_MID
+        default: break;
+    }
+}
 
 /* Output b of the inverse convolution transform of sub-line s = (line, k1):
  * R[g^b] = c_b + r_0 is output k1 + q * g^b of the line; b = 0 also gives
@@ -179,17 +214,20 @@ struct RaderOutStore {
 };
 
 /* Output k of a line from its sub-lines after the inverse convolution
- * transform (sub-line k1 = k mod q at s): R[0] for k2 = k / q = 0, else
- * c_b + r_0 with b = log_g k2 = (p - 1 - rin[k2]) mod (p - 1). */
-__host__ __device__ __forceinline__ float2 rader_output(const float2* s, int k2, int p, int rp, const int* rin) {
+ * transform (sub-line k1 = k mod q: its convolution output c in natural
+ * order, its slots in s): R[0] for k2 = k / q = 0, else c_b + r_0 with
+ * b = log_g k2 = (p - 1 - rin[k2]) mod (p - 1). */
+__host__ __device__ __forceinline__ float2 rader_output(const float2* c, const float2* s, int k2, int p, int rp,
+                                                        const int* rin) {
     if (k2 == 0) return s[rp - 2];
     const int r = rin[k2];
-    return cadd(s[r == 0 ? 0 : p - 1 - r], s[rp - 1]);
+    return cadd(c[r == 0 ? 0 : p - 1 - r], s[rp - 1]);
 }
 
 /* Rader transform of nlines lines held in b0 (line l at l * ld); the result
- * is returned in b0.  b1 holds the sub-lines, b0 the exchange areas of the
- * convolution transforms.  Starts after the caller's barrier, ends with one. */
+ * is returned in b1.  The sub-lines and exchange areas of the convolution
+ * transforms alternate between b1 and b0.  Starts after the caller's
+ * barrier, ends with one. */
 template <int S>
 __device__ __forceinline__ float2* line_rader(float2* b0, float2* b1, int ld, int nlines, const LinePlan& P,
                                               const float2* tw) {
@@ -209,18 +247,30 @@ __device__ __forceinline__ float2* line_rader(float2* b0, float2* b1, int ld, in
     const float2* V = (S < 0) ? P.gf : P.gi;
     for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
         const int l2 = fast_div(it, m1, i1), k1 = it - l2 * m1;
-        tl2_item2_f<-1>(m2, b0 + l2 * rp, k1, m1, RaderMulStore{b1 + l2 * rp, V, rp});
+        rader_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, rp);
     }
     lds_barrier();
-    tl2_lines<1>(b1, b0, rp, nl2, m1, m2, P.mtw);
+    /* inverse step 2 (split m2 * m1): c in natural order to b0, with the
+     * sub-line's two slots */
+    for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
+        const int l2 = fast_div(it, m2, i2), kk = it - l2 * m2;
+        tl2_item2<1>(m1, b1 + l2 * rp, kk, m2, b0 + l2 * rp);
+        if (kk == 0) {
+            b0[l2 * rp + rp - 2] = b1[l2 * rp + rp - 2];
+            b0[l2 * rp + rp - 1] = b1[l2 * rp + rp - 1];
+        }
+    }
+    lds_barrier();
+    /* the line outputs to b1 (line l at l * ld) */
     const float in_ = 1.0f / (float)n, iq = 1.0f / (float)q;
     for (int it = threadIdx.x; it < nlines * n; it += blockDim.x) {
         const int l = fast_div(it, n, in_), k = it - l * n;
         const int k2 = fast_div(k, q, iq), k1 = k - k2 * q;
-        b0[l * ld + k] = rader_output(b1 + (k1 * nlines + l) * rp, k2, p, rp, P.rin);
+        const float2* so = b0 + (k1 * nlines + l) * rp;
+        b1[l * ld + k] = rader_output(so, so, k2, p, rp, P.rin);
     }
     lds_barrier();
-    return b0;
+    return b1;
 }
 
 /* Direct transform of a length-n line set with plan radices or split. */
@@ -346,21 +396,16 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
         for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
             int l2, k1;
             split2(it, m1, i1, l2, k1);
-            tl2_item2_f<-1>(m2, b0 + l2 * rp, k1, m1, RaderMulStore{b1 + l2 * rp, V, rp});
+            rader_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, rp);
         }
         lds_barrier();
-        for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
-            int l2, j2;
-            split2(it, m2, i2, l2, j2);
-            tl2_item1<1>(m1, b1 + l2 * rp, j2, m2, P.mtw, b0 + l2 * rp);
-        }
-        lds_barrier();
+        /* inverse step 2 (split m2 * m1) */
         using St = decltype(store(0));
-        for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
-            int l2, k1;
-            split2(it, m1, i1, l2, k1);
+        for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
+            int l2, kk;
+            split2(it, m2, i2, l2, kk);
             const int r1 = fast_div(l2, nlines, inl), l = l2 - r1 * nlines;
-            tl2_item2_f<1>(m2, b0 + l2 * rp, k1, m1, RaderOutStore<St>{store(l), b1 + l2 * rp, P.rout, r1, q, rp});
+            tl2_item2_f<1>(m1, b1 + l2 * rp, kk, m2, RaderOutStore<St>{store(l), b1 + l2 * rp, P.rout, r1, q, rp});
         }
         lds_barrier();
         return;

@@ -149,23 +149,30 @@ static double check_rader(int n) {
             rader_item1_f<S>(q, PtrLoad{&b0[l * ld]}, j2, p, tw.data(), &b1[l * rp + rin[j2]], nlines * rp);
     for (int l2 = 0; l2 < nl2; ++l2)
         for (int j2 = 0; j2 < m2; ++j2) tl2_item1<-1>(m1, &b1[l2 * rp], j2, m2, mtw.data(), &b0[l2 * rp]);
+    /* forward step 2, V and inverse step 1 (split m2 * m1) in one phase */
     for (int l2 = 0; l2 < nl2; ++l2)
-        for (int k1 = 0; k1 < m1; ++k1) tl2_item2_f<-1>(m2, &b0[l2 * rp], k1, m1, RaderMulStore{&b1[l2 * rp], V.data(), rp});
-    for (int l2 = 0; l2 < nl2; ++l2)
-        for (int j2 = 0; j2 < m2; ++j2) tl2_item1<1>(m1, &b1[l2 * rp], j2, m2, mtw.data(), &b0[l2 * rp]);
+        for (int k1 = 0; k1 < m1; ++k1)
+            rader_mid_item(m2, &b0[l2 * rp], k1, m1, V.data(), mtw.data(), &b1[l2 * rp], rp);
     /* the store-functor path first (it leaves b0 and b1 as they are) */
     for (int l2 = 0; l2 < nl2; ++l2) {
         const int k1 = l2 / nlines, l = l2 - k1 * nlines;
-        for (int kk = 0; kk < m1; ++kk)
-            tl2_item2_f<1>(m2, &b0[l2 * rp], kk, m1,
+        for (int kk = 0; kk < m2; ++kk)
+            tl2_item2_f<1>(m1, &b1[l2 * rp], kk, m2,
                            RaderOutStore<PtrStore>{PtrStore{&viaio[l * n]}, &b1[l2 * rp], rout.data(), k1, q, rp});
     }
-    for (int l2 = 0; l2 < nl2; ++l2)
-        for (int k1 = 0; k1 < m1; ++k1) tl2_item2<1>(m2, &b0[l2 * rp], k1, m1, &b1[l2 * rp]);
+    /* line_rader: inverse step 2 to b0 with the two slots, outputs from b0 */
+    for (int l2 = 0; l2 < nl2; ++l2) {
+        for (int kk = 0; kk < m2; ++kk) tl2_item2<1>(m1, &b1[l2 * rp], kk, m2, &b0[l2 * rp]);
+        b0[l2 * rp + rp - 2] = b1[l2 * rp + rp - 2];
+        b0[l2 * rp + rp - 1] = b1[l2 * rp + rp - 1];
+    }
     double e = 0;
     for (int l = 0; l < nlines; ++l) {
         std::vector<float2> out(n);
-        for (int k = 0; k < n; ++k) out[k] = rader_output(&b1[((k % q) * nlines + l) * rp], k / q, p, rp, rin.data());
+        for (int k = 0; k < n; ++k) {
+            const float2* so = &b0[((k % q) * nlines + l) * rp];
+            out[k] = rader_output(so, so, k / q, p, rp, rin.data());
+        }
         const auto ref = dft(xs[l], S);
         e = std::max(e, relerr(out, ref));
         e = std::max(e, relerr(std::vector<float2>(viaio.begin() + l * n, viaio.begin() + (l + 1) * n), ref));
