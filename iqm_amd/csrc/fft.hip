@@ -132,7 +132,7 @@ const LinePlanHost& Ctx::line_plan(int n, int split1) {
         } else if (P.blue) {
             P.tm1 = split1;
             P.tm2 = s2;
-            P.lds_len = std::max(P.m, split1 * (s2 + 1));
+            P.lds_len = std::max({P.m, split1 * (s2 + 1), s2 * (split1 + 1)});
         } else {
             P.tn1 = split1;
             P.tn2 = s2;
@@ -241,7 +241,9 @@ const LinePlanHost& Ctx::line_plan(int n, int split1) {
         P.gf = g1;
         P.gi = g2;
         P.lds_len = m;
-        if (tl2_split(m, P.tm1, P.tm2)) P.lds_len = std::max(m, P.tm1 * (P.tm2 + 1));
+        /* the forward exchange area (tm1 rows of tm2 + 1) and the inverse
+         * one (the split the other way round) */
+        if (tl2_split(m, P.tm1, P.tm2)) P.lds_len = std::max({m, P.tm1 * (P.tm2 + 1), P.tm2 * (P.tm1 + 1)});
         lp.owned = {mtw, ch, g1, g2};
     }
     return line_plans_.emplace(std::make_pair(n, 0), lp).first->second;

@@ -136,7 +136,7 @@ __device__ __forceinline__ float2* tl2_lines(float2* in, float2* z, int ld, int 
  * (rin[0] is the slot pitch - 1 that keeps r_0).  The inverse transform
  * takes the split the other way round (m2 * m1): its step-1 item j2' = k1
  * needs U[k1 + m1 * k2] * V[k1 + m1 * k2] for k2 < m2, exactly the outputs
- * of the forward transform's step-2 item k1, so one phase (rader_mid) runs
+ * of the forward transform's step-2 item k1, so one phase (conv_mid) runs
  * forward step 2, the product with V and inverse step 1 in registers, and
  * keeps R[0] = r_0 + U[0] in slot pitch - 2.  Sub-line (l, k1) of an
  * nlines-line set lives at (k1 * nlines + l) * pitch of the buffer
@@ -155,22 +155,24 @@ __host__ __device__ __forceinline__ void rader_item1_f(int q, const Load& in, in
     }
 }
 
-/* The middle phase of a Rader sub-line for one item k1 < m1: forward step 2
- * (DFT_M2 of exchange row k1 of z, the forward transform's exchange area),
- * giving U[k1 + m1 * k2]; R[0] = r_0 + U[0] to slot rp - 2 of the sub-line s
- * (item 0); the products with V; then inverse step 1 for j2' = k1 of the
- * split m2 * m1 (DFT_M2 with S = +1 and the twiddles W_m^{+k1 k1'}, tw =
- * exp(-2 pi i j / m)), stored to the inverse exchange area at
- * s[k1' * (m1 + 1) + k1], k1' < m2. */
+/* The middle phase of a convolution by two-level transforms of length
+ * m = m1 * m2 (Rader's p - 1, Bluestein's m) for one item k1 < m1: forward
+ * step 2 (DFT_M2 of exchange row k1 of z, the forward transform's exchange
+ * area), giving U[k1 + m1 * k2]; with a slot (Rader: item 0 keeps
+ * R[0] = r_0 + U[0] in slot[0], r_0 being slot[1]); the products with the
+ * kernel spectrum V; then step 1 of the inverse transform with the split the
+ * other way round (m2 * m1), whose item j2' = k1 needs exactly these values:
+ * DFT_M2 with S = +1 and the twiddles W_m^{+k1 k1'} (tw = exp(-2 pi i j / m)),
+ * stored to the inverse exchange area zo[k1' * (m1 + 1) + k1], k1' < m2. */
 template <int M2>
-__host__ __device__ __forceinline__ void rader_mid(const float2* __restrict__ z, int k1, int m1,
-                                                   const float2* __restrict__ V, const float2* __restrict__ tw,
-                                                   float2* __restrict__ s, int rp) {
+__host__ __device__ __forceinline__ void conv_mid(const float2* __restrict__ z, int k1, int m1,
+                                                  const float2* __restrict__ V, const float2* __restrict__ tw,
+                                                  float2* __restrict__ zo, float2* slot) {
     float2 y[M2];
 #pragma unroll
     for (int j2 = 0; j2 < M2; ++j2) y[j2] = z[k1 * (M2 + 1) + j2];
     RegDft<M2, -1>::run(y);
-    if (k1 == 0) s[rp - 2] = cadd(y[0], s[rp - 1]);
+    if (slot != nullptr && k1 == 0) slot[0] = cadd(y[0], slot[1]);
 #pragma unroll
     for (int k2 = 0; k2 < M2; ++k2) y[k2] = cmul(y[k2], V[k1 + m1 * k2]);
     RegDft<M2, 1>::run(y);
@@ -184,19 +186,49 @@ __host__ __device__ __forceinline__ void rader_mid(const float2* __restrict__ z,
             w.y = -w.y;
             v = cmul(v, w);
         }
-        s[kk * p2 + k1] = v;
+        zo[kk * p2 + k1] = v;
     }
 }
-__host__ __device__ __forceinline__ void rader_mid_item(int m2, const float2* z, int k1, int m1, const float2* V,
-                                                        const float2* tw, float2* s, int rp) {
+__host__ __device__ __forceinline__ void conv_mid_item(int m2, const float2* z, int k1, int m1, const float2* V,
+                                                       const float2* tw, float2* zo, float2* slot) {
     switch (m2) {
-#define IQM_RADER_MID(k) \
-    case k: rader_mid<k>(z, k1, m1, V, tw, s, rp); break;
-        IQM_TL2_SIZES(IQM_RADER_MID)
-#undef IQM_RADER_MID
+#define IQM_CONV_MID(k) \
+    case k: conv_mid<k>(z, k1, m1, V, tw, zo, slot); break;
+        IQM_TL2_SIZES(IQM_CONV_MID)
+#undef IQM_CONV_MID
         default: break;
     }
 }
+
+/* Bluestein's first and last steps folded into the convolution's loads and
+ * stores: input j < n times the chirp c_j (zero for n <= j < m), and output
+ * k < n times c_k / m (outputs k >= n dropped); S > 0 takes the conjugate
+ * chirp. */
+template <int S, class Load>
+struct ChirpLoad {
+    Load in;
+    const float2* chirp;
+    int n;
+    __host__ __device__ float2 operator()(int j) const {
+        if (j >= n) return make_float2(0.f, 0.f);
+        float2 c = chirp[j];
+        if (S > 0) c.y = -c.y;
+        return cmul(in(j), c);
+    }
+};
+template <int S, class Store>
+struct ChirpStore {
+    Store out;
+    const float2* chirp;
+    int n;
+    float inv_m;
+    __host__ __device__ void operator()(int k, float2 v) const {
+        if (k >= n) return;
+        float2 c = chirp[k];
+        if (S > 0) c.y = -c.y;
+        out(k, cscale(cmul(v, c), inv_m));
+    }
+};
 
 /* Output b of the inverse convolution transform of sub-line s = (line, k1):
  * R[g^b] = c_b + r_0 is output k1 + q * g^b of the line; b = 0 also gives
@@ -247,7 +279,7 @@ __device__ __forceinline__ float2* line_rader(float2* b0, float2* b1, int ld, in
     const float2* V = (S < 0) ? P.gf : P.gi;
     for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
         const int l2 = fast_div(it, m1, i1), k1 = it - l2 * m1;
-        rader_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, rp);
+        conv_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, b1 + l2 * rp + rp - 2);
     }
     lds_barrier();
     /* inverse step 2 (split m2 * m1): c in natural order to b0, with the
@@ -290,6 +322,31 @@ __device__ __forceinline__ float2* line_fft(float2* b0, float2* b1, int ld, int 
     if (P.rader) return line_rader<S>(b0, b1, ld, nlines, P, tw);
     if (!P.blue) return line_direct<S>(b0, b1, ld, nlines, P.n, P.tn1, P.tn2, P.nst, P.rad, tw);
     const int n = P.n, m = P.m;
+    if (P.tm1 > 0) {
+        /* three phases: forward step 1 with the chirp on its loads (b0 ->
+         * b1), forward step 2 + kernel product + inverse step 1 (b1 -> b0),
+         * inverse step 2 with the chirp and 1/m on its stores (b0 -> b1) */
+        const int m1 = P.tm1, m2 = P.tm2;
+        const float i1 = 1.0f / (float)m1, i2 = 1.0f / (float)m2;
+        for (int it = threadIdx.x; it < nlines * m2; it += blockDim.x) {
+            const int l = fast_div(it, m2, i2), j2 = it - l * m2;
+            tl2_item1_f<-1>(m1, ChirpLoad<S, PtrLoad>{PtrLoad{b0 + l * ld}, P.chirp, n}, j2, m2, P.mtw, b1 + l * ld);
+        }
+        lds_barrier();
+        const float2* g = (S < 0) ? P.gf : P.gi;
+        for (int it = threadIdx.x; it < nlines * m1; it += blockDim.x) {
+            const int l = fast_div(it, m1, i1), k1 = it - l * m1;
+            conv_mid_item(m2, b1 + l * ld, k1, m1, g, P.mtw, b0 + l * ld, nullptr);
+        }
+        lds_barrier();
+        const float inv_m = 1.0f / (float)m;
+        for (int it = threadIdx.x; it < nlines * m2; it += blockDim.x) {
+            const int l = fast_div(it, m2, i2), kk = it - l * m2;
+            tl2_item2_f<1>(m1, b0 + l * ld, kk, m2, ChirpStore<S, PtrStore>{PtrStore{b1 + l * ld}, P.chirp, n, inv_m});
+        }
+        lds_barrier();
+        return b1;
+    }
     const float im_ = 1.0f / (float)m, in_ = 1.0f / (float)n;
     for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
         const int l = fast_div(idx, m, im_), j = idx - l * m;
@@ -396,7 +453,7 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
         for (int it = threadIdx.x; it < nl2 * m1; it += blockDim.x) {
             int l2, k1;
             split2(it, m1, i1, l2, k1);
-            rader_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, rp);
+            conv_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, b1 + l2 * rp + rp - 2);
         }
         lds_barrier();
         /* inverse step 2 (split m2 * m1) */
@@ -428,6 +485,34 @@ __device__ __forceinline__ void line_fft_io(const MakeLoad& load, const MakeStor
         return;
     }
     const int m = P.m;
+    if (P.tm1 > 0) {
+        /* Bluestein in three phases, as in line_fft */
+        const int m1 = P.tm1, m2 = P.tm2;
+        const float i1 = 1.0f / (float)m1, i2 = 1.0f / (float)m2;
+        using Ld = decltype(load(0));
+        using St = decltype(store(0));
+        for (int it = threadIdx.x; it < nlines * m2; it += blockDim.x) {
+            int l, j2;
+            split(it, m2, i2, l, j2);
+            tl2_item1_f<-1>(m1, ChirpLoad<S, Ld>{load(l), P.chirp, n}, j2, m2, P.mtw, b1 + l * ld);
+        }
+        lds_barrier();
+        const float2* g = (S < 0) ? P.gf : P.gi;
+        for (int it = threadIdx.x; it < nlines * m1; it += blockDim.x) {
+            int l, k1;
+            split(it, m1, i1, l, k1);
+            conv_mid_item(m2, b1 + l * ld, k1, m1, g, P.mtw, b0 + l * ld, nullptr);
+        }
+        lds_barrier();
+        const float inv_m = 1.0f / (float)m;
+        for (int it = threadIdx.x; it < nlines * m2; it += blockDim.x) {
+            int l, kk;
+            split(it, m2, i2, l, kk);
+            tl2_item2_f<1>(m1, b0 + l * ld, kk, m2, ChirpStore<S, St>{store(l), P.chirp, n, inv_m});
+        }
+        lds_barrier();
+        return;
+    }
     const float im_ = 1.0f / (float)m;
     for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
         int l, j;

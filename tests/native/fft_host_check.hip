@@ -1,8 +1,8 @@
 // Host-side check of the FFT index math (no GPU needed): runs the same
 // __host__ __device__ building blocks the kernels use -- Stockham stages,
 // register DFTs, the compile-time two-level line split, the run-time
-// two-level split with its size dispatch and Rader's identity with its plan
-// tables -- sequentially on the CPU and
+// two-level split with its size dispatch, Rader's identity with its plan
+// tables and Bluestein's in three phases -- sequentially on the CPU and
 // prints the largest error against a direct double-precision DFT.
 #include <cmath>
 #include <complex>
@@ -152,7 +152,7 @@ static double check_rader(int n) {
     /* forward step 2, V and inverse step 1 (split m2 * m1) in one phase */
     for (int l2 = 0; l2 < nl2; ++l2)
         for (int k1 = 0; k1 < m1; ++k1)
-            rader_mid_item(m2, &b0[l2 * rp], k1, m1, V.data(), mtw.data(), &b1[l2 * rp], rp);
+            conv_mid_item(m2, &b0[l2 * rp], k1, m1, V.data(), mtw.data(), &b1[l2 * rp], &b1[l2 * rp + rp - 2]);
     /* the store-functor path first (it leaves b0 and b1 as they are) */
     for (int l2 = 0; l2 < nl2; ++l2) {
         const int k1 = l2 / nlines, l = l2 - k1 * nlines;
@@ -176,6 +176,78 @@ static double check_rader(int n) {
         const auto ref = dft(xs[l], S);
         e = std::max(e, relerr(out, ref));
         e = std::max(e, relerr(std::vector<float2>(viaio.begin() + l * n, viaio.begin() + (l + 1) * n), ref));
+    }
+    return e;
+}
+
+/* Bluestein in the three phases the kernels run (fft_line.hpp line_fft and
+ * the Bluestein branch of line_fft_io), sequentially over every work item,
+ * for two lines of n points, with the plan's tables made as Ctx::line_plan
+ * makes them: chirp c_j = exp(-pi i (j^2 mod 2n) / n), m the smallest
+ * length >= 2n - 1 with a two-level split, and the spectra of the
+ * convolution kernels conj(c_d) (forward) and c_d (inverse), d in (-n, n),
+ * from a double-precision DFT.  Both the in-place path (line_fft: input in
+ * b0, output in b1) and the load/store path must equal the DFT of each
+ * line. */
+template <int S>
+static double check_bluestein(int n) {
+    int m = 2 * n - 1, m1 = 0, m2 = 0;
+    while (m <= 32 * 32 && !tl2_split(m, m1, m2)) ++m;
+    if (m1 == 0) return -1;
+    std::vector<cd> cdd(n);
+    std::vector<float2> chirp(n);
+    for (int j = 0; j < n; ++j) {
+        const double a = M_PI * (double)((long long)j * j % (2LL * n)) / n;
+        cdd[j] = cd(std::cos(a), -std::sin(a));
+        chirp[j] = make_float2((float)cdd[j].real(), (float)cdd[j].imag());
+    }
+    std::vector<cd> kf(m, 0.0), ki(m, 0.0);
+    for (int d = 0; d < n; ++d) {
+        kf[d] = std::conj(cdd[d]);
+        ki[d] = cdd[d];
+        if (d) {
+            kf[m - d] = kf[d];
+            ki[m - d] = ki[d];
+        }
+    }
+    const auto Kf = dft(kf, -1), Ki = dft(ki, -1);
+    std::vector<float2> g(m);
+    for (int j = 0; j < m; ++j) {
+        const cd v = S < 0 ? Kf[j] : Ki[j];
+        g[j] = make_float2((float)v.real(), (float)v.imag());
+    }
+    const auto mtw = twiddles(m);
+    const int nlines = 2, ld = lds_pitch(std::max({m, m1 * (m2 + 1), m2 * (m1 + 1)}));
+    std::vector<std::vector<cd>> xs;
+    std::vector<float2> b0(nlines * ld), b1(nlines * ld), in(nlines * n), viaio(nlines * n);
+    for (int l = 0; l < nlines; ++l) {
+        xs.push_back(rnd(n, 29 * n + l));
+        for (int j = 0; j < n; ++j) in[l * n + j] = b0[l * ld + j] = make_float2((float)xs[l][j].real(), (float)xs[l][j].imag());
+    }
+    const float inv_m = 1.0f / (float)m;
+    auto run = [&](auto load, auto store) {
+        for (int l = 0; l < nlines; ++l)
+            for (int j2 = 0; j2 < m2; ++j2)
+                tl2_item1_f<-1>(m1, ChirpLoad<S, decltype(load(0))>{load(l), chirp.data(), n}, j2, m2, mtw.data(),
+                                &b1[l * ld]);
+        for (int l = 0; l < nlines; ++l)
+            for (int k1 = 0; k1 < m1; ++k1) conv_mid_item(m2, &b1[l * ld], k1, m1, g.data(), mtw.data(), &b0[l * ld], nullptr);
+        for (int l = 0; l < nlines; ++l)
+            for (int kk = 0; kk < m2; ++kk)
+                tl2_item2_f<1>(m1, &b0[l * ld], kk, m2,
+                               ChirpStore<S, decltype(store(0))>{store(l), chirp.data(), n, inv_m});
+    };
+    /* load/store path: input from `in`, output to `viaio` */
+    run([&](int l) { return PtrLoad{&in[l * n]}; }, [&](int l) { return PtrStore{&viaio[l * n]}; });
+    /* in-place path: input in b0 (reloaded), output in b1 */
+    for (int l = 0; l < nlines; ++l)
+        for (int j = 0; j < n; ++j) b0[l * ld + j] = in[l * n + j];
+    run([&](int l) { return PtrLoad{&b0[l * ld]}; }, [&](int l) { return PtrStore{&b1[l * ld]}; });
+    double e = 0;
+    for (int l = 0; l < nlines; ++l) {
+        const auto ref = dft(xs[l], S);
+        e = std::max(e, relerr(std::vector<float2>(viaio.begin() + l * n, viaio.begin() + (l + 1) * n), ref));
+        e = std::max(e, relerr(std::vector<float2>(b1.begin() + l * ld, b1.begin() + l * ld + n), ref));
     }
     return e;
 }
@@ -226,6 +298,12 @@ int main() {
     for (int n : {17, 37, 61, 97, 929, 74, 202, 611, 629, 1184}) {
         rep("rader_fwd", n, check_rader<-1>(n));
         rep("rader_inv", n, check_rader<1>(n));
+    }
+    /* Bluestein: primes whose p - 1 has no two-level split (263, 269, 431, 83)
+     * and lengths with such a prime factor (3 * 107, 223, 511 = 7 * 73); 509 takes m = 1024 */
+    for (int n : {83, 223, 263, 269, 321, 431, 509, 511}) {
+        rep("bluestein_fwd", n, check_bluestein<-1>(n));
+        rep("bluestein_inv", n, check_bluestein<1>(n));
     }
     std::printf("worst %.3e\n", worst);
     return worst < 2e-6 ? 0 : 1;
