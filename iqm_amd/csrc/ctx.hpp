@@ -84,7 +84,7 @@ struct MixJob {
     double* ssim_partial;
     double* ssim_out;
     /* FSIM */
-    int F, wd, hd, cols, grad_rb, col_bx, nbx, row_rb, pc_gx, pc_gy;
+    int F, wd, hd, cols, prep_rb, grad_rb, col_bx, nbx, row_rb, pc_gx, pc_gy;
     long N;
     float invN;
     LinePlan Px, Py;

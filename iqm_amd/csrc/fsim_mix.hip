@@ -25,17 +25,17 @@ __global__ __launch_bounds__(kBlock) void mix_prep_kernel(const MixJob* __restri
     const int local = blockIdx.x - st[j];
     const int gx = (J.wd + J.cols - 1) / J.cols;
     switch (J.F) {
-#define IQM_MIXPREP_CASE(f)                                                                                   \
-    case f:                                                                                                   \
-        fsim_prep_body<f>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx, \
-                          local / gx, 0);                                                                     \
+#define IQM_MIXPREP_CASE(f)                                                                                  \
+    case f:                                                                                                  \
+        fsim_prep_body<f>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, J.prep_rb, J.yiq, J.hist, \
+                          local % gx, local / gx, 0);                                                        \
         break;
         IQM_MIXPREP_CASE(1) IQM_MIXPREP_CASE(2) IQM_MIXPREP_CASE(3) IQM_MIXPREP_CASE(4)
         IQM_MIXPREP_CASE(5) IQM_MIXPREP_CASE(6) IQM_MIXPREP_CASE(7) IQM_MIXPREP_CASE(8)
 #undef IQM_MIXPREP_CASE
         default:
-            fsim_prep_body<0>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, 1, J.yiq, J.hist, local % gx,
-                              local / gx, 0);
+            fsim_prep_body<0>(J.img, J.ref, 0, J.w, J.h, J.ch, J.F, J.wd, J.hd, J.cols, J.prep_rb, J.yiq, J.hist,
+                              local % gx, local / gx, 0);
     }
 }
 
@@ -107,7 +107,13 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
     mj.hd = hd;
     mj.N = N;
     mj.invN = (float)(1.0 / (double)N);
-    mj.cols = prep_cols(F, pj.ch, 1);
+    /* prep: 4 downscaled rows per workgroup where kPrepMaxRows full rows
+     * allow it (256-pair batch: 1.62 ms with 1 row, 1.20 with 2, 1.06 with
+     * 4: fewer, larger workgroups, each finding its pair once) */
+    int prb = std::max(1, env_int("IQM_MIX_PREP_RB", 4));
+    while (prb > 1 && prb * F > kPrepMaxRows) --prb;
+    mj.prep_rb = prb;
+    mj.cols = prep_cols(F, pj.ch, prb);
     mj.grad_rb = t.grad_rb;
     mj.col_bx = t.col_bx;
     mj.nbx = ceil_div(wd, t.col_bx);
@@ -142,7 +148,7 @@ void Ctx::mix_fsim_fill(const PairJob& pj, MixJob& mj, char* base, size_t& off, 
         cnt.blocks[stage][c] += nb;
         cnt.lds[stage][c] = std::max(cnt.lds[stage][c], lds);
     };
-    add(MIX_PREP, ceil_div(wd, mj.cols) * hd, prep_lds_bytes(F, pj.ch, mj.cols, 1));
+    add(MIX_PREP, ceil_div(wd, mj.cols) * ceil_div(hd, prb), prep_lds_bytes(F, pj.ch, mj.cols, prb));
     add(MIX_ROWFFT, ceil_div(hd, t.grad_rb), line_lds_bytes(Px, t.grad_rb, 2));
     add(MIX_COL, mj.nbx * 2, line_lds_bytes(Py, t.col_bx, colpass_bufs(Py)));
     add(MIX_ROW, ceil_div(hd, t.row_rb) * 2, rowpass_lds_bytes(Px, t.row_rb, wd));
