@@ -458,8 +458,8 @@ __device__ __forceinline__ void fsim_colpass_body(float2* __restrict__ A, float2
  * their level-1 histogram per orientation. */
 __host__ __device__ __forceinline__ size_t rowpass_lds_bytes(const LinePlan& P, int rb, int wd) {
     const size_t bufbytes = (size_t)4 * rb * lds_pitch(P.lds_len) * sizeof(float2);
-    const size_t hist = bufbytes >= kHistBins * sizeof(unsigned) ? 0 : kHistBins * sizeof(unsigned);
-    return 2 * bufbytes + (size_t)lds_twiddles(P) * sizeof(float2) + hist + sizeof(float) * (size_t)rb * wd;
+    return 2 * bufbytes + (size_t)lds_twiddles(P) * sizeof(float2) + kHistBins * sizeof(unsigned) +
+           sizeof(float) * (size_t)rb * wd;
 }
 
 __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin, float* __restrict__ an_e,
@@ -473,10 +473,11 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
     float2* b1 = lds + nl * ld;
     const int ntw = lds_twiddles(P);
     const float2* tw = stage_twiddles(P, lds + 2 * nl * ld);
-    const size_t bufbytes = (size_t)nl * ld * sizeof(float2);
-    const bool hist_aliased = bufbytes >= kHistBins * sizeof(unsigned);
-    unsigned* hsep = (unsigned*)(lds + 2 * nl * ld + ntw);
-    float* sa = (float*)(hsep + (hist_aliased ? 0 : kHistBins)); /* rb * wd running sums of An */
+    /* the histogram has its own LDS: cleared once here, then by each thread
+     * for the bins it flushes, so no orientation spends a phase on it */
+    unsigned* hl = (unsigned*)(lds + 2 * nl * ld + ntw);
+    float* sa = (float*)(hl + kHistBins); /* rb * wd running sums of An */
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
     const long N = (long)wd * hd;
     const int y0 = bx * rb;
     const int nr = min(rb, hd - y0);
@@ -547,9 +548,6 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
         }
         lds_barrier(); /* also orders the twiddles staged before the loop */
         float2* R = line_fft<1>(b0, b1, ld, nl, P, tw);
-        unsigned* hl = hist_aliased ? (unsigned*)((R == b0) ? b1 : b0) : hsep;
-        for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) hl[i] = 0u;
-        lds_barrier();
         float* enp = ap + (long)(1 + o) * N;
         float* kp = keys + ((long)pair * 8 + io) * N;
         const int sstride = rb * ld; /* lines of one scale */
@@ -575,8 +573,10 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
         for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) {
             const unsigned c = hl[i];
             if (c) atomicAdd(&hg[i], c);
+            hl[i] = 0u;
         }
-        lds_barrier(); /* hl may alias b0/b1, which the next orientation loads into */
+        /* the next orientation's first barrier orders these before its
+         * epilogue; its loads go to b0 only */
     }
 }
 
