@@ -245,19 +245,8 @@ struct RaderOutStore {
     }
 };
 
-/* Output k of a line from its sub-lines after the inverse convolution
- * transform (sub-line k1 = k mod q: its convolution output c in natural
- * order, its slots in s): R[0] for k2 = k / q = 0, else c_b + r_0 with
- * b = log_g k2 = (p - 1 - rin[k2]) mod (p - 1). */
-__host__ __device__ __forceinline__ float2 rader_output(const float2* c, const float2* s, int k2, int p, int rp,
-                                                        const int* rin) {
-    if (k2 == 0) return s[rp - 2];
-    const int r = rin[k2];
-    return cadd(c[r == 0 ? 0 : p - 1 - r], s[rp - 1]);
-}
-
 /* Rader transform of nlines lines held in b0 (line l at l * ld); the result
- * is returned in b1.  The sub-lines and exchange areas of the convolution
+ * is returned in b0.  The sub-lines and exchange areas of the convolution
  * transforms alternate between b1 and b0.  Starts after the caller's
  * barrier, ends with one. */
 template <int S>
@@ -282,27 +271,18 @@ __device__ __forceinline__ float2* line_rader(float2* b0, float2* b1, int ld, in
         conv_mid_item(m2, b0 + l2 * rp, k1, m1, V, P.mtw, b1 + l2 * rp, b1 + l2 * rp + rp - 2);
     }
     lds_barrier();
-    /* inverse step 2 (split m2 * m1): c in natural order to b0, with the
-     * sub-line's two slots */
+    /* inverse step 2 (split m2 * m1), each output stored straight to its
+     * place in its line in b0 (line l at l * ld), free since the middle
+     * phase; the sub-line's slots stay in b1 */
+    const float inl = 1.0f / (float)nlines;
     for (int it = threadIdx.x; it < nl2 * m2; it += blockDim.x) {
         const int l2 = fast_div(it, m2, i2), kk = it - l2 * m2;
-        tl2_item2<1>(m1, b1 + l2 * rp, kk, m2, b0 + l2 * rp);
-        if (kk == 0) {
-            b0[l2 * rp + rp - 2] = b1[l2 * rp + rp - 2];
-            b0[l2 * rp + rp - 1] = b1[l2 * rp + rp - 1];
-        }
+        const int r1 = fast_div(l2, nlines, inl), l = l2 - r1 * nlines;
+        tl2_item2_f<1>(m1, b1 + l2 * rp, kk, m2,
+                       RaderOutStore<PtrStore>{PtrStore{b0 + l * ld}, b1 + l2 * rp, P.rout, r1, q, rp});
     }
     lds_barrier();
-    /* the line outputs to b1 (line l at l * ld) */
-    const float in_ = 1.0f / (float)n, iq = 1.0f / (float)q;
-    for (int it = threadIdx.x; it < nlines * n; it += blockDim.x) {
-        const int l = fast_div(it, n, in_), k = it - l * n;
-        const int k2 = fast_div(k, q, iq), k1 = k - k2 * q;
-        const float2* so = b0 + (k1 * nlines + l) * rp;
-        b1[l * ld + k] = rader_output(so, so, k2, p, rp, P.rin);
-    }
-    lds_barrier();
-    return b1;
+    return b0;
 }
 
 /* Direct transform of a length-n line set with plan radices or split. */

@@ -121,8 +121,9 @@ static double check_tl2(int n) {
 
 /* Rader's identity as the kernels run it (fft_line.hpp line_rader and the
  * Rader branch of line_fft_io), sequentially over every work item, for
- * three lines of n points: the sub-line outputs through rader_output and
- * through RaderOutStore must both equal the DFT of each line. */
+ * three lines of n points: the outputs stored through RaderOutStore to a
+ * separate array and to the line buffer must both equal the DFT of each
+ * line. */
 template <int S>
 static double check_rader(int n) {
     int q, p, m1, m2;
@@ -160,19 +161,16 @@ static double check_rader(int n) {
             tl2_item2_f<1>(m1, &b1[l2 * rp], kk, m2,
                            RaderOutStore<PtrStore>{PtrStore{&viaio[l * n]}, &b1[l2 * rp], rout.data(), k1, q, rp});
     }
-    /* line_rader: inverse step 2 to b0 with the two slots, outputs from b0 */
+    /* line_rader: the same inverse step 2 storing into the line buffer b0 */
     for (int l2 = 0; l2 < nl2; ++l2) {
-        for (int kk = 0; kk < m2; ++kk) tl2_item2<1>(m1, &b1[l2 * rp], kk, m2, &b0[l2 * rp]);
-        b0[l2 * rp + rp - 2] = b1[l2 * rp + rp - 2];
-        b0[l2 * rp + rp - 1] = b1[l2 * rp + rp - 1];
+        const int k1 = l2 / nlines, l = l2 - k1 * nlines;
+        for (int kk = 0; kk < m2; ++kk)
+            tl2_item2_f<1>(m1, &b1[l2 * rp], kk, m2,
+                           RaderOutStore<PtrStore>{PtrStore{&b0[l * ld]}, &b1[l2 * rp], rout.data(), k1, q, rp});
     }
     double e = 0;
     for (int l = 0; l < nlines; ++l) {
-        std::vector<float2> out(n);
-        for (int k = 0; k < n; ++k) {
-            const float2* so = &b0[((k % q) * nlines + l) * rp];
-            out[k] = rader_output(so, so, k / q, p, rp, rin.data());
-        }
+        std::vector<float2> out(b0.begin() + l * ld, b0.begin() + l * ld + n);
         const auto ref = dft(xs[l], S);
         e = std::max(e, relerr(out, ref));
         e = std::max(e, relerr(std::vector<float2>(viaio.begin() + l * n, viaio.begin() + (l + 1) * n), ref));
