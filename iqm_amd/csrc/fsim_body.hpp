@@ -449,6 +449,14 @@ __device__ __forceinline__ void fsim_colpass_body(float2* __restrict__ A, float2
 }
 
 
+/* element j of one row of a response plane (zeros for rows past the image),
+ * read once */
+struct RowLoad {
+    const float2* p;
+    bool ok;
+    __device__ float2 operator()(int j) const { return ok ? nt_load(p + j) : make_float2(0.f, 0.f); }
+};
+
 /* Row half of F11 (normalised by 1/N) fused with F12
  * (fsim_sum_filter_responses.glsl:33-83) and the F13 packing
  * (fsim_pack_for_median.glsl:33-44) for `rb` rows of one image, one
@@ -488,7 +496,10 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
      * transformed (the transform's barriers order LDS only) */
     constexpr int kRowPrefetch = 8;
     const int tot = nl * wd;
-    const bool pf = tot <= kRowPrefetch * (int)blockDim.x;
+    /* two-level lines read their step-1 inputs straight from the planes
+     * (no copy into LDS first); the other kinds copy the tile into b0 */
+    const bool direct = !P.blue && !P.rader && P.tn1 > 0;
+    const bool pf = !direct && tot <= kRowPrefetch * (int)blockDim.x;
     float2 pre[kRowPrefetch];
     const float iwd = 1.0f / (float)wd, irb = 1.0f / (float)rb;
     /* row offsets (y0 + r) * wd + x (< N) in 32 bits from per-plane bases */
@@ -509,45 +520,57 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
     if (pf) fetch(0);
     for (int o = 0; o < 4; ++o) {
         const int io = img * 4 + o;
-        if (pf) {
-#pragma unroll
-            for (int q = 0; q < kRowPrefetch; ++q) {
-                const int idx = threadIdx.x + q * blockDim.x;
-                if (idx < tot) {
-                    const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
-                    b0[l * ld + x] = pre[q];
-                }
-            }
-            if (o < 3) fetch(o + 1);
+        float2* R = b0;
+        if (direct) {
+            lds_barrier(); /* also orders the twiddles staged before the loop */
+            line_fft_io<1, false>(
+                [&](int l) {
+                    const int s = fast_div(l, rb, irb), r = l - s * rb;
+                    return RowLoad{Tin + ((long)pair * 32 + io * 4 + s) * N + (long)(y0 + min(r, nr - 1)) * wd,
+                                   r < nr};
+                },
+                [&](int l) { return PtrStore{b0 + l * ld}; }, b0, b1, ld, nl, P, tw);
         } else {
-            /* rounds of kRowPrefetch elements per thread, all loads of a
-             * round in flight before its LDS stores */
-            for (int base = 0; base < tot; base += kRowPrefetch * (int)blockDim.x) {
-                float2 v[kRowPrefetch];
+            if (pf) {
 #pragma unroll
                 for (int q = 0; q < kRowPrefetch; ++q) {
-                    const int idx = base + threadIdx.x + q * blockDim.x;
-                    v[q] = make_float2(0.f, 0.f);
+                    const int idx = threadIdx.x + q * blockDim.x;
                     if (idx < tot) {
                         const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
-                        const int s = fast_div(l, rb, irb), r = l - s * rb;
-                        if (r < nr)
-                            v[q] = nt_load(
-                                at32(Tin + ((long)pair * 32 + io * 4 + s) * N, (unsigned)((y0 + r) * wd + x)));
+                        b0[l * ld + x] = pre[q];
                     }
                 }
+                if (o < 3) fetch(o + 1);
+            } else {
+                /* rounds of kRowPrefetch elements per thread, all loads of a
+                 * round in flight before its LDS stores */
+                for (int base = 0; base < tot; base += kRowPrefetch * (int)blockDim.x) {
+                    float2 v[kRowPrefetch];
 #pragma unroll
-                for (int q = 0; q < kRowPrefetch; ++q) {
-                    const int idx = base + threadIdx.x + q * blockDim.x;
-                    if (idx < tot) {
-                        const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
-                        b0[l * ld + x] = v[q];
+                    for (int q = 0; q < kRowPrefetch; ++q) {
+                        const int idx = base + threadIdx.x + q * blockDim.x;
+                        v[q] = make_float2(0.f, 0.f);
+                        if (idx < tot) {
+                            const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
+                            const int s = fast_div(l, rb, irb), r = l - s * rb;
+                            if (r < nr)
+                                v[q] = nt_load(
+                                    at32(Tin + ((long)pair * 32 + io * 4 + s) * N, (unsigned)((y0 + r) * wd + x)));
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kRowPrefetch; ++q) {
+                        const int idx = base + threadIdx.x + q * blockDim.x;
+                        if (idx < tot) {
+                            const int l = fast_div(idx, wd, iwd), x = idx - l * wd;
+                            b0[l * ld + x] = v[q];
+                        }
                     }
                 }
             }
+            lds_barrier(); /* also orders the twiddles staged before the loop */
+            R = line_fft<1>(b0, b1, ld, nl, P, tw);
         }
-        lds_barrier(); /* also orders the twiddles staged before the loop */
-        float2* R = line_fft<1>(b0, b1, ld, nl, P, tw);
         float* enp = ap + (long)(1 + o) * N;
         float* kp = keys + ((long)pair * 8 + io) * N;
         const int sstride = rb * ld; /* lines of one scale */
