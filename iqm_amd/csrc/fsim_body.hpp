@@ -496,9 +496,10 @@ __device__ __forceinline__ void fsim_rowpass_body(const float2* __restrict__ Tin
      * transformed (the transform's barriers order LDS only) */
     constexpr int kRowPrefetch = 8;
     const int tot = nl * wd;
-    /* two-level lines read their step-1 inputs straight from the planes
-     * (no copy into LDS first); the other kinds copy the tile into b0 */
-    const bool direct = !P.blue && !P.rader && P.tn1 > 0;
+    /* two-level and Rader lines read their first-phase inputs straight from
+     * the planes (no copy into LDS first); Stockham and Bluestein lines copy
+     * the tile into b0 */
+    const bool direct = !P.blue && (P.rader || P.tn1 > 0);
     const bool pf = !direct && tot <= kRowPrefetch * (int)blockDim.x;
     float2 pre[kRowPrefetch];
     const float iwd = 1.0f / (float)wd, irb = 1.0f / (float)rb;
